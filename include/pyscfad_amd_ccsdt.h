@@ -1,0 +1,98 @@
+/*
+ * pyscfad_amd: closed-shell CCSD(T) triples energy E(T) and its adjoint on one
+ * AMD MI355X (gfx950).  C ABI of libpyscfad_amd_cc.so.
+ *
+ * All array arguments are DEVICE pointers to C-contiguous float64 arrays in the
+ * reference's own layouts (pyscfad/cc/ccsd_t.py:36-42, pyscfad/cc/ccsd.py:30-31):
+ *   mo_energy (n)            n = nocc + nvir
+ *   t1        (o, v)
+ *   t2        (o, o, v, v)
+ *   ovvv      (o, v, v, v) full, or (o, v, v(v+1)/2) lower-triangle packed
+ *             (the DF layout, pyscfad/cc/dfccsd.py:88) when cfg->ovvv_packed
+ *   ovoo      (o, v, o, o)   chemists' (ia|jk)
+ *   ovov      (o, v, o, v)   chemists' (ia|jb)
+ *   fvo       (v, o)         fock[o:, :o]; may be NULL when cfg->has_fvo == 0
+ * The stream argument is a hipStream_t (0 = the null stream).  Calls on one
+ * workspace must not overlap.  Every entry point returns a PAMD_* status.
+ */
+#ifndef PYSCFAD_AMD_CCSDT_H
+#define PYSCFAD_AMD_CCSDT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PAMD_OK 0
+#define PAMD_ERR_ARG 1          /* bad sizes, NULL pointer, unknown option     */
+#define PAMD_ERR_HIP 2          /* a HIP runtime call failed                  */
+#define PAMD_ERR_WORKSPACE 3    /* workspace smaller than pamd_ccsdt_workspace_size */
+#define PAMD_ERR_UNSUPPORTED 4  /* option not built into this library         */
+
+#define PAMD_PREC_FP64 0        /* plain form: FP64 MFMA                       */
+#define PAMD_PREC_BF16X3 1      /* split-BF16x3 MFMA, FP32 tile accumulation   */
+#define PAMD_PREC_FP32X2 2      /* FP32 hi/lo split on FP32 MFMA (comparison)  */
+
+typedef struct {
+    int nocc;
+    int nvir;
+    int ovvv_packed;  /* 1: ovvv (and ovvv_bar) are (o, v, v(v+1)/2) packed   */
+    int has_fvo;      /* 0: fvo is zero (the t2*fvo terms are skipped)          */
+    int precision;    /* PAMD_PREC_*                                            */
+    int reserved;
+} pamd_ccsdt_config;
+
+/* Device workspace needed by pamd_ccsdt_energy (with_vjp = 0) or
+ * pamd_ccsdt_energy_vjp (with_vjp = 1).  Host-only call; no GPU work. */
+int pamd_ccsdt_workspace_size(const pamd_ccsdt_config *cfg, int with_vjp, size_t *bytes);
+
+/* E(T) -> *e_out (device, 1 double, overwritten).
+ * Replaces the forward of the reference's (T) call: pyscf ccsd_t.kernel ->
+ * libcc CCsd_t_contract at pyscfad/cc/ccsd_t.py:41, and its in-tree C sibling
+ * lnoccsdt_contract (pyscfadlib/pyscfadlib/vjp/cc/lno_ccsd_t.c:96-100) with
+ * mat = I, et = 2/3 e_tot (pyscfad/lno/ccsd_t.py:109-112). */
+int pamd_ccsdt_energy(const pamd_ccsdt_config *cfg,
+                      const double *mo_energy, const double *t1, const double *t2,
+                      const double *ovvv, const double *ovoo, const double *ovov,
+                      const double *fvo,
+                      void *workspace, size_t workspace_bytes,
+                      double *e_out, void *stream);
+
+/* E(T) and its VJP for the cotangent et_bar, in one pass over the triples
+ * (W/V are recomputed per triple block, nothing o^3 v^3 is stored).
+ * Replaces ccsd_t_energy_vjp (pyscfadlib/pyscfadlib/vjp/cc/ccsd_t_vjp.c:332-346)
+ * together with its host driver _ccsd_t_energy_vjp (pyscfad/cc/ccsd_t.py:82-192):
+ * outputs are OVERWRITTEN with the cotangents in the reference's output layout
+ * (ccsd_t.py:179-191): mo_energy_bar (n), t1_bar (o,v), t2_bar (o,o,v,v),
+ * ovvv_bar (packed fold g+g^T, diagonal halved, when cfg->ovvv_packed; else the
+ * unsymmetrised full (o,v,v,v) block), ovoo_bar (o,v,o,o), ovov_bar (o,v,o,v),
+ * fvo_bar (v,o) (the vo block of fock_bar; required even when fvo == 0, since
+ * dE/dfvo = 1/2 sum Vbar t2 does not vanish with fvo).  e_out receives E(T). */
+int pamd_ccsdt_energy_vjp(const pamd_ccsdt_config *cfg,
+                          const double *mo_energy, const double *t1, const double *t2,
+                          const double *ovvv, const double *ovoo, const double *ovov,
+                          const double *fvo, double et_bar,
+                          void *workspace, size_t workspace_bytes,
+                          double *e_out, double *mo_energy_bar, double *t1_bar,
+                          double *t2_bar, double *ovvv_bar, double *ovoo_bar,
+                          double *ovov_bar, double *fvo_bar, void *stream);
+
+/* Occupied-pair resolved E(T): e_pair[i*o + j] = 1/3 sum_{k,abc} W Z / D,
+ * sum(e_pair) = E(T).  Row sums are the occupied-weighted (T) of
+ * lnoccsdt_contract with mat = e_i e_i^T (lno_ccsd_t.c:16-26); the labels of the
+ * learned screen.  e_pair: device, o*o doubles, overwritten. */
+int pamd_ccsdt_pair_energy(const pamd_ccsdt_config *cfg,
+                           const double *mo_energy, const double *t1, const double *t2,
+                           const double *ovvv, const double *ovoo, const double *ovov,
+                           const double *fvo,
+                           void *workspace, size_t workspace_bytes,
+                           double *e_pair, void *stream);
+
+/* Library version string (static storage). */
+const char *pamd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYSCFAD_AMD_CCSDT_H */

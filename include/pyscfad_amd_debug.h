@@ -1,0 +1,28 @@
+/*
+ * Host-only inspection entry points of libpyscfad_amd_cc.so.  They run no GPU
+ * work and have no counterpart in the reference: they let tests check, on the
+ * CPU, every address the contraction engine will touch before a kernel runs.
+ */
+#ifndef PYSCFAD_AMD_DEBUG_H
+#define PYSCFAD_AMD_DEBUG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Record (do not launch) the contraction descriptors of one occupied triple
+ * i <= j <= k: the three W-build GEMMs, plus the adjoint GEMMs when with_vjp.
+ * bases[10]: base addresses to put in the descriptors for
+ *   Vp, T2m, OOt, W, t2, Vb, Vpb, T2mb, part_p, part_h.
+ * out receives min(n, max_desc) descriptors of *desc_bytes bytes each. */
+int pamd_debug_gemm_descriptors(int nocc, int nvir, int i, int j, int k, int with_vjp,
+                                const unsigned long long *bases, void *out, int max_desc,
+                                int *n_desc, int *desc_bytes);
+
+/* Split-K factors the adjoint uses for (particle, hole) partial buffers. */
+int pamd_debug_split_factors(int nocc, int nvir, int *S_p, int *S_h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,83 @@
+"""ctypes binding of libpyscfad_amd_cc.so (the C ABI in include/pyscfad_amd_ccsdt.h).
+
+The library is built in-tree by ``python -m pyscfad_amd.csrc.build``.  There is
+no CPU fallback: every product entry point raises if the library or a GPU is
+missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libpyscfad_amd_cc.so')
+
+PAMD_OK = 0
+PAMD_ERR_ARG = 1
+PAMD_ERR_HIP = 2
+PAMD_ERR_WORKSPACE = 3
+PAMD_ERR_UNSUPPORTED = 4
+STATUS_NAMES = {0: 'ok', 1: 'bad argument', 2: 'HIP runtime error', 3: 'workspace too small',
+                4: 'unsupported option'}
+
+PRECISIONS = {'fp64': 0, 'bf16x3': 1, 'fp32x2': 2}
+
+# exported symbols of include/pyscfad_amd_ccsdt.h and include/pyscfad_amd_mlp.h
+EXPORTS = ('pamd_version', 'pamd_ccsdt_workspace_size', 'pamd_ccsdt_energy',
+           'pamd_ccsdt_energy_vjp', 'pamd_ccsdt_pair_energy')
+
+
+class Config(ctypes.Structure):
+    _fields_ = [('nocc', ctypes.c_int), ('nvir', ctypes.c_int), ('ovvv_packed', ctypes.c_int),
+                ('has_fvo', ctypes.c_int), ('precision', ctypes.c_int),
+                ('reserved', ctypes.c_int)]
+
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises RuntimeError when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f'{LIB_PATH} is missing: build it with '
+                           '`python -m pyscfad_amd.csrc.build` (hipcc, gfx950)')
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, i, d, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+    cfgp = ctypes.POINTER(Config)
+    lib.pamd_version.restype = ctypes.c_char_p
+    lib.pamd_version.argtypes = []
+    lib.pamd_ccsdt_workspace_size.restype = i
+    lib.pamd_ccsdt_workspace_size.argtypes = [cfgp, i, ctypes.POINTER(sz)]
+    lib.pamd_ccsdt_energy.restype = i
+    lib.pamd_ccsdt_energy.argtypes = [cfgp] + [vp] * 7 + [vp, sz, vp, vp]
+    lib.pamd_ccsdt_pair_energy.restype = i
+    lib.pamd_ccsdt_pair_energy.argtypes = [cfgp] + [vp] * 7 + [vp, sz, vp, vp]
+    lib.pamd_ccsdt_energy_vjp.restype = i
+    lib.pamd_ccsdt_energy_vjp.argtypes = [cfgp] + [vp] * 7 + [d, vp, sz] + [vp] * 8 + [vp]
+    _lib = lib
+    return lib
+
+
+def check(status, what):
+    if status != PAMD_OK:
+        name = STATUS_NAMES.get(status, f'status {status}')
+        if status in (PAMD_ERR_ARG, PAMD_ERR_UNSUPPORTED):
+            raise ValueError(f'{what}: {name}')
+        raise RuntimeError(f'{what}: {name}')
+
+
+def make_config(nocc, nvir, ovvv_packed=False, has_fvo=False, precision='fp64'):
+    if precision not in PRECISIONS:
+        raise ValueError(f'unknown precision {precision!r}; choose from {sorted(PRECISIONS)}')
+    return Config(int(nocc), int(nvir), int(bool(ovvv_packed)), int(bool(has_fvo)),
+                  PRECISIONS[precision], 0)
+
+
+def workspace_size(cfg, with_vjp):
+    lib = load()
+    out = ctypes.c_size_t(0)
+    check(lib.pamd_ccsdt_workspace_size(ctypes.byref(cfg), int(bool(with_vjp)), ctypes.byref(out)),
+          'pamd_ccsdt_workspace_size')
+    return out.value

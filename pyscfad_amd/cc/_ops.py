@@ -1,0 +1,137 @@
+"""Device-level (T) operations on torch tensors, calling the C ABI.
+
+These functions take device (or host) arrays in the reference's layouts, move
+them to the GPU as contiguous float64 if needed, and run the HIP kernels on the
+current torch stream.  There is no CPU path: without a GPU they raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import numpy as np
+import torch
+
+from .. import _lib
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError('pyscfad_amd: no GPU is visible; the (T) kernels are HIP-only '
+                           'and have no CPU path')
+
+
+def _dev(x, device):
+    if x is None:
+        return None
+    if isinstance(x, torch.Tensor):
+        t = x.detach()
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(np.asarray(x, dtype=np.float64)))
+    return t.to(device=device, dtype=torch.float64).contiguous()
+
+
+def _ptr(t):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _stream(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+class Inputs:
+    """Device copies of the (T) inputs plus the C-ABI config."""
+
+    def __init__(self, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo=None, precision='fp64',
+                 device=None):
+        _require_gpu()
+        if device is None:
+            device = t2.device if isinstance(t2, torch.Tensor) and t2.is_cuda else \
+                torch.device('cuda', torch.cuda.current_device())
+        self.device = torch.device(device)
+        self.t1 = _dev(t1, self.device)
+        nocc, nvir = self.t1.shape
+        self.nocc, self.nvir = int(nocc), int(nvir)
+        self.mo_energy = _dev(mo_energy, self.device)
+        self.t2 = _dev(t2, self.device)
+        self.ovvv = _dev(ovvv, self.device)
+        self.ovoo = _dev(ovoo, self.device)
+        self.ovov = _dev(ovov, self.device)
+        self.fvo = _dev(fvo, self.device)
+        o, v = self.nocc, self.nvir
+        self.packed = self.ovvv.dim() == 3
+        expect = {
+            'mo_energy': (self.mo_energy, (o + v,)),
+            't2': (self.t2, (o, o, v, v)),
+            'ovvv': (self.ovvv, (o, v, v * (v + 1) // 2) if self.packed else (o, v, v, v)),
+            'ovoo': (self.ovoo, (o, v, o, o)),
+            'ovov': (self.ovov, (o, v, o, v)),
+        }
+        if self.fvo is not None:
+            expect['fvo'] = (self.fvo, (v, o))
+        for name, (t, shape) in expect.items():
+            if tuple(t.shape) != shape:
+                raise ValueError(f'{name} has shape {tuple(t.shape)}, expected {shape}')
+        self.has_fvo = self.fvo is not None and bool(torch.count_nonzero(self.fvo).item() > 0)
+        if not self.has_fvo:
+            self.fvo = None
+        self.precision = precision
+        self.cfg = _lib.make_config(o, v, self.packed, self.has_fvo, precision)
+
+    def args(self):
+        return [_ptr(self.mo_energy), _ptr(self.t1), _ptr(self.t2), _ptr(self.ovvv),
+                _ptr(self.ovoo), _ptr(self.ovov), _ptr(self.fvo)]
+
+    def workspace(self, with_vjp):
+        nbytes = _lib.workspace_size(self.cfg, with_vjp)
+        free, _total = torch.cuda.mem_get_info(self.device)
+        if nbytes > free + torch.cuda.memory_reserved(self.device):
+            raise RuntimeError(f'(T): at least {nbytes / 1e6:.0f} MB of device memory needed, '
+                               f'{free / 1e6:.0f} MB free')
+        return torch.empty(max(nbytes, 1), dtype=torch.uint8, device=self.device), nbytes
+
+
+def energy(inp: Inputs):
+    """E(T) as a 0-d float64 device tensor."""
+    lib = _lib.load()
+    ws, nbytes = inp.workspace(False)
+    e = torch.zeros((), dtype=torch.float64, device=inp.device)
+    st = lib.pamd_ccsdt_energy(ctypes.byref(inp.cfg), *inp.args(), _ptr(ws), nbytes, _ptr(e),
+                               _stream(inp.device))
+    _lib.check(st, 'pamd_ccsdt_energy')
+    return e
+
+
+def pair_energy(inp: Inputs):
+    """(o, o) occupied-pair resolved E(T); its sum is E(T)."""
+    lib = _lib.load()
+    ws, nbytes = inp.workspace(False)
+    e = torch.zeros((inp.nocc, inp.nocc), dtype=torch.float64, device=inp.device)
+    st = lib.pamd_ccsdt_pair_energy(ctypes.byref(inp.cfg), *inp.args(), _ptr(ws), nbytes,
+                                    _ptr(e), _stream(inp.device))
+    _lib.check(st, 'pamd_ccsdt_pair_energy')
+    return e
+
+
+def energy_vjp(inp: Inputs, et_bar=1.0):
+    """(E(T), cotangents) with the cotangents in the reference's output layout."""
+    lib = _lib.load()
+    o, v = inp.nocc, inp.nvir
+    dev = inp.device
+    ws, nbytes = inp.workspace(True)
+    f64 = dict(dtype=torch.float64, device=dev)
+    e = torch.zeros((), **f64)
+    out = dict(
+        mo_energy_bar=torch.empty(o + v, **f64),
+        t1_bar=torch.empty((o, v), **f64),
+        t2_bar=torch.empty((o, o, v, v), **f64),
+        ovvv_bar=torch.empty(tuple(inp.ovvv.shape), **f64),
+        ovoo_bar=torch.empty((o, v, o, o), **f64),
+        ovov_bar=torch.empty((o, v, o, v), **f64),
+        fvo_bar=torch.empty((v, o), **f64),
+    )
+    st = lib.pamd_ccsdt_energy_vjp(
+        ctypes.byref(inp.cfg), *inp.args(), ctypes.c_double(float(et_bar)), _ptr(ws), nbytes,
+        _ptr(e), _ptr(out['mo_energy_bar']), _ptr(out['t1_bar']), _ptr(out['t2_bar']),
+        _ptr(out['ovvv_bar']), _ptr(out['ovoo_bar']), _ptr(out['ovov_bar']),
+        _ptr(out['fvo_bar']), _stream(dev))
+    _lib.check(st, 'pamd_ccsdt_energy_vjp')
+    return e, out

@@ -1,0 +1,106 @@
+// Shared declarations of the (T) kernels and their host driver.
+//
+// The triples are walked over occupied index triples i <= j <= k.  For one
+// triple the connected triples amplitude W_ijk[a,b,c] (v^3) is the sum of six
+// particle and six hole contractions (the joint (ijk,abc) permutations of
+// w_abc[ijk] = sum_f ovvv[i,a,f,b] t2[k,j,c,f] - sum_m ovoo[i,a,m,j] t2[k,m,c,b],
+// pyscfadlib/pyscfadlib/vjp/cc/ccsd_t.c:121-142).  Every contraction -- the
+// W build and all adjoint contractions of the backward -- is one instance of a
+// single strided "segmented GEMM" problem run by the MFMA engine:
+//
+//   C[m, n]  (=, +=, or split-K partial)  sum_s alpha_s sum_k A_s(m, k) B_s(k, n)
+//
+// where every operand element is addressed as
+//   A_s(m, k) = A.p[(m / A.rdiv) * A.rx + (m % A.rdiv) * A.ry
+//                   + (k / A.kdiv) * A.kx + (k % A.kdiv) * A.ky]
+// (likewise B_s(k, n) with n in the role of m), so that pairs of virtual indices
+// can sit on the M, N or K side of a contraction without repacking tensors.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pamd {
+
+constexpr int kMaxSeg = 4;
+
+struct OpDesc {
+  const double *p;
+  long long rx, ry, kx, ky;  // strides: row index (m or n) split (r / rdiv, r % rdiv), K likewise
+  int rdiv, kdiv;
+  int rfast;                 // 1: contiguous along the row index -> stage with threads along rows
+  int pad_;
+};
+
+struct SegDesc {
+  OpDesc A, B;
+  int L;          // contraction length of this segment
+  int pad_;
+  double alpha;
+};
+
+struct OutDesc {
+  double *p;
+  long long mx, my, nx, ny;
+  int mdiv, ndiv;
+};
+
+enum GemmMode { kStore = 0, kAccumulate = 1, kPartial = 2 };
+
+struct GemmDesc {
+  int M, N, nseg, mode;
+  int chunk_start[kMaxSeg + 1];  // prefix sums of ceil(L_s / KC) in K chunks
+  int splits;                    // split-K factor (grid z); partial mode only when > 1
+  int pad_;
+  SegDesc seg[kMaxSeg];
+  OutDesc C;
+  double *partial;               // kPartial: partial[(split * M + m) * N + n]
+};
+
+// Epilogue over orbits of virtual tiles for one occupied triple.
+struct OrbitArgs {
+  double *W;            // v^3; W in, Wbar out (vjp)
+  double *Vb;           // v^3; Vbar out (vjp)
+  const double *t1, *t2, *ovov, *fvo, *mo;
+  int o, v, i, j, k, has_fvo, vjp, nt;
+  long long norb;
+  double omega;         // 2 s_ijk: 2, 1 (two indices equal)
+  double ebar;          // cotangent of E
+  double *part_t;       // [G] per-workgroup unweighted triple energy (overwritten)
+  double *part_mo;      // [G][n] per-workgroup mo_energy_bar accumulators (vjp)
+};
+
+// V-bar reductions for one triple.
+struct VRArgs {
+  const double *Vb, *t1, *fvo, *ovov, *t2;
+  int o, v, i, j, k, has_fvo;
+  double *R;            // scratch: 10 v^2 + 2 v doubles (layout in kernels)
+};
+
+// Per-triple gather of the adjoint pieces into the output cotangents.
+struct CombineArgs {
+  const double *R;                // VR scratch
+  const double *part_p;           // [6][S][v][v] particle t2_bar partials
+  const double *part_h;           // [6][S][v][o] hole ovoo_bar partials
+  int S_p, S_h;
+  int o, v, i, j, k, has_fvo;
+  double *t2_bar, *ovov_bar, *ovoo_bar, *t1_bar, *fvo_bar;
+};
+
+// host-side launchers (ccsdt_kernels.hip)
+hipError_t launch_fill(double *p, size_t n, double val, hipStream_t s);
+hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, const double *ovoo,
+                          int o, int v, double *Vp, double *T2m, double *OOt, hipStream_t s);
+hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s);
+hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s);
+hipError_t launch_vr(const VRArgs &a, hipStream_t s);
+hipError_t launch_combine(const CombineArgs &a, hipStream_t s);
+hipError_t launch_fold_ovvv(const double *Vpb, int o, int v, int packed, double *out, hipStream_t s);
+hipError_t launch_fold_t2m(const double *T2mb, int o, int v, double *t2_bar, hipStream_t s);
+hipError_t launch_reduce_rows(const double *part, int rows, int cols, double *out, int accumulate,
+                              hipStream_t s);
+hipError_t launch_triple_energy(const double *part_t, int G, double *dst, hipStream_t s);
+int orbit_grid(int v);
+hipError_t launch_pair_energy(const double *tri_e, int o, double *e_pair, hipStream_t s);
+hipError_t launch_total_energy(const double *tri_e, int o, double *e_out, hipStream_t s);
+
+}  // namespace pamd

@@ -1,0 +1,630 @@
+// Host driver and C ABI for the (T) energy and its adjoint.
+//
+// Walk: occupied triples i <= j <= k (i == j == k contributes nothing: X is then
+// fully symmetric and r3(X) = 0).  Per triple:
+//   1. W build -- three segmented MFMA GEMMs write W_ijk[a,b,c] (v^3):
+//        pass C  (rows (a,b), cols c):  P1 + P3 + H5 + H6
+//        pass B  (rows b, cols (a,c)):  P2 + P5 + H3 + H4
+//        pass A  (rows a, cols (b,c)):  P4 + P6 + H1 + H2
+//      (P1..P6 / H1..H6: the six joint permutations of the particle / hole
+//      term of w_abc[ijk], pyscfadlib/pyscfadlib/vjp/cc/ccsd_t.c:121-142 with
+//      the permutation tables of ccsd_t.c:74-96).
+//   2. orbit epilogue: V, Z = r3(W + V), 1/D, energy; for the adjoint also
+//      Wbar, Vbar and dE/d(mo_energy) (ccsd_t_vjp.c:42-117).
+//   3. adjoint (ccsd_t_vjp.c:139-269 restated as large contractions):
+//        ovvv_bar  -- 3 GEMMs into the Vp-layout accumulator (rows = pairs)
+//        t2_bar    -- 6 split-K GEMMs (K = a virtual pair) + V-term pieces
+//        ovoo_bar  -- 6 split-K GEMMs (hole terms)
+//        t2_bar    -- 3 GEMMs for the hole-term t2 reads (T2m layout)
+//        ovov_bar, t1_bar, fvo_bar -- Vbar contractions (k_vr_*) + gather.
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstring>
+#include <algorithm>
+#include <vector>
+#include "ccsdt_common.h"
+#include "../../include/pyscfad_amd_ccsdt.h"
+
+namespace pamd {
+
+static const int kBig = 1 << 30;
+static const int kGmax = 1024;
+
+// When set, GEMM launches are recorded instead of run and the other per-triple
+// kernels are skipped (host-only descriptor inspection, no GPU work).
+static std::vector<GemmDesc> *g_record = nullptr;
+
+static hipError_t run_gemm(const GemmDesc &d, hipStream_t s) {
+  if (g_record) {
+    g_record->push_back(d);
+    return hipSuccess;
+  }
+  return launch_gemm_f64(d, s);
+}
+
+static OpDesc op(const double *p, long long rx, long long ry, int rdiv, long long kx, long long ky,
+                 int kdiv, int rfast) {
+  OpDesc d;
+  d.p = p;
+  d.rx = rx;
+  d.ry = ry;
+  d.rdiv = rdiv;
+  d.kx = kx;
+  d.ky = ky;
+  d.kdiv = kdiv;
+  d.rfast = rfast;
+  d.pad_ = 0;
+  return d;
+}
+// row index is a single index with stride ry; K single with stride ky
+static OpDesc op1(const double *p, long long ry, long long ky, int rfast) {
+  return op(p, 0, ry, kBig, 0, ky, kBig, rfast);
+}
+// row index is a pair (r / rdiv, r % rdiv); K single
+static OpDesc opR(const double *p, long long rx, long long ry, int rdiv, long long ky, int rfast) {
+  return op(p, rx, ry, rdiv, 0, ky, kBig, rfast);
+}
+// row index single; K is a pair (k / kdiv, k % kdiv)
+static OpDesc opK(const double *p, long long ry, long long kx, long long ky, int kdiv, int rfast) {
+  return op(p, 0, ry, kBig, kx, ky, kdiv, rfast);
+}
+
+static SegDesc seg(OpDesc A, OpDesc B, int L, double alpha) {
+  SegDesc s;
+  s.A = A;
+  s.B = B;
+  s.L = L;
+  s.pad_ = 0;
+  s.alpha = alpha;
+  return s;
+}
+
+static OutDesc out(double *p, long long mx, long long my, int mdiv, long long nx, long long ny,
+                   int ndiv) {
+  OutDesc c;
+  c.p = p;
+  c.mx = mx;
+  c.my = my;
+  c.mdiv = mdiv;
+  c.nx = nx;
+  c.ny = ny;
+  c.ndiv = ndiv;
+  return c;
+}
+
+static const int KC = 16;
+
+static GemmDesc gemm(int M, int N, int mode, std::initializer_list<SegDesc> segs, OutDesc C,
+                     int splits = 1, double *partial = nullptr) {
+  GemmDesc d;
+  memset(&d, 0, sizeof(d));
+  d.M = M;
+  d.N = N;
+  d.mode = mode;
+  d.nseg = 0;
+  d.chunk_start[0] = 0;
+  for (const SegDesc &s : segs) {
+    d.seg[d.nseg] = s;
+    d.chunk_start[d.nseg + 1] = d.chunk_start[d.nseg] + (s.L + KC - 1) / KC;
+    ++d.nseg;
+  }
+  for (int q = d.nseg + 1; q <= kMaxSeg; ++q) d.chunk_start[q] = d.chunk_start[d.nseg];
+  d.splits = std::max(1, std::min(splits, d.chunk_start[d.nseg]));
+  d.C = C;
+  d.partial = partial;
+  return d;
+}
+
+struct Layout {
+  size_t off_Vp, off_T2m, off_OOt, off_W, off_tri, off_part_t;
+  size_t off_Vb, off_Vpb, off_T2mb, off_part_mo, off_R, off_part_p, off_part_h;
+  size_t total;
+  int S_p, S_h;
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static int splits_for(int tiles, int chunks) {
+  int s = (512 + tiles - 1) / std::max(1, tiles);
+  s = std::max(1, std::min(s, 64));
+  return std::min(s, std::max(1, chunks));
+}
+
+static Layout make_layout(int o, int v, int vjp) {
+  Layout L;
+  memset(&L, 0, sizeof(L));
+  const size_t d = sizeof(double);
+  const size_t v2 = (size_t)v * v, v3 = v2 * v, n = (size_t)o + v;
+  size_t off = 0;
+  auto take = [&](size_t count) {
+    size_t at = off;
+    off = align256(off + count * d);
+    return at;
+  };
+  L.off_Vp = take((size_t)o * v3);
+  L.off_T2m = take((size_t)o * o * v2);
+  L.off_OOt = take((size_t)o * o * o * v);
+  L.off_W = take(v3);
+  L.off_tri = take((size_t)o * o * o);
+  L.off_part_t = take(kGmax);
+  const int tiles_p = ((v + 63) / 64) * ((v + 63) / 64);
+  const int tiles_h = ((v + 63) / 64) * ((o + 63) / 64);
+  const int chunks = (int)((v2 + KC - 1) / KC);
+  L.S_p = splits_for(tiles_p, chunks);
+  L.S_h = splits_for(tiles_h, chunks);
+  if (vjp) {
+    L.off_Vb = take(v3);
+    L.off_Vpb = take((size_t)o * v3);
+    L.off_T2mb = take((size_t)o * o * v2);
+    L.off_part_mo = take((size_t)kGmax * n);
+    L.off_R = take(10 * v2 + 2 * (size_t)v);
+    L.off_part_p = take((size_t)6 * L.S_p * v2);
+    L.off_part_h = take((size_t)6 * L.S_h * v * o);
+  }
+  L.total = off;
+  return L;
+}
+
+struct Ctx {
+  int o, v, packed, has_fvo, vjp;
+  hipStream_t s;
+  const double *mo, *t1, *t2, *ovvv, *ovoo, *ovov, *fvo;
+  double *Vp, *T2m, *OOt, *W, *tri, *part_t;
+  double *Vb, *Vpb, *T2mb, *part_mo, *R, *part_p, *part_h;
+  int S_p, S_h, G;
+  double ebar;
+  // outputs
+  double *t1_bar, *t2_bar, *ovoo_bar, *ovov_bar, *fvo_bar;
+};
+
+#define PAMD_CHECK(expr)                                                      \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      fprintf(stderr, "pyscfad_amd: %s failed: %s\n", #expr, hipGetErrorString(e_)); \
+      return PAMD_ERR_HIP;                                                    \
+    }                                                                         \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// W build for one triple
+// ---------------------------------------------------------------------------
+static int build_w(const Ctx &c, int i, int j, int k) {
+  const long long o = c.o, v = c.v;
+  const long long v2 = v * v, v3 = v2 * v, vo = v * o;
+  auto Vp = [&](int x) { return c.Vp + x * v3; };
+  auto T2 = [&](int x, int y) { return c.t2 + (x * o + y) * v2; };
+  auto T2m = [&](int x) { return c.T2m + x * v2 * o; };
+  auto OOt = [&](int x, int y) { return c.OOt + (x * o + y) * vo; };
+  const int V = c.v, O = c.o;
+  // pass C: rows (a,b), cols c
+  {
+    GemmDesc d = gemm(V * V, V, kStore,
+                      {seg(opR(Vp(i), v2, v, V, 1, 0), op1(T2(k, j), v, 1, 0), V, 1.0),      // P1
+                       seg(opR(Vp(j), v, v2, V, 1, 0), op1(T2(k, i), v, 1, 0), V, 1.0),      // P3
+                       seg(opR(T2m(j), o, vo, V, 1, 0), op1(OOt(k, i), o, 1, 0), O, -1.0),   // H5
+                       seg(opR(T2m(i), vo, o, V, 1, 0), op1(OOt(k, j), o, 1, 0), O, -1.0)},  // H6
+                      out(c.W, v2, v, V, 0, 1, kBig));
+    PAMD_CHECK(run_gemm(d, c.s));
+  }
+  // pass B: rows b, cols (a,c)
+  {
+    GemmDesc d = gemm(V, V * V, kAccumulate,
+                      {seg(op1(T2(j, k), v, 1, 0), opR(Vp(i), v2, v, V, 1, 0), V, 1.0),      // P2
+                       seg(op1(T2(j, i), v, 1, 0), opR(Vp(k), v, v2, V, 1, 0), V, 1.0),      // P5
+                       seg(op1(OOt(j, i), o, 1, 0), opR(T2m(k), o, vo, V, 1, 0), O, -1.0),   // H3
+                       seg(op1(OOt(j, k), o, 1, 0), opR(T2m(i), vo, o, V, 1, 0), O, -1.0)},  // H4
+                      out(c.W, 0, v, kBig, v2, 1, V));
+    PAMD_CHECK(run_gemm(d, c.s));
+  }
+  // pass A: rows a, cols (b,c)
+  {
+    GemmDesc d = gemm(V, V * V, kAccumulate,
+                      {seg(op1(T2(i, k), v, 1, 0), opR(Vp(j), v2, v, V, 1, 0), V, 1.0),      // P4
+                       seg(op1(T2(i, j), v, 1, 0), opR(Vp(k), v, v2, V, 1, 0), V, 1.0),      // P6
+                       seg(op1(OOt(i, j), o, 1, 0), opR(T2m(k), o, vo, V, 1, 0), O, -1.0),   // H1
+                       seg(op1(OOt(i, k), o, 1, 0), opR(T2m(j), vo, o, V, 1, 0), O, -1.0)},  // H2
+                      out(c.W, 0, v2, kBig, v, 1, V));
+    PAMD_CHECK(run_gemm(d, c.s));
+  }
+  return PAMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// adjoint contractions for one triple (c.W holds Wbar, c.Vb holds Vbar)
+// ---------------------------------------------------------------------------
+static int adjoint(const Ctx &c, int i, int j, int k) {
+  const long long o = c.o, v = c.v;
+  const long long v2 = v * v, v3 = v2 * v, vo = v * o;
+  const int V = c.v, O = c.o;
+  const double *Wb = c.W;
+  auto Vp = [&](int x) { return c.Vp + x * v3; };
+  auto Vpb = [&](int x) { return c.Vpb + x * v3; };
+  auto T2 = [&](int x, int y) { return c.t2 + (x * o + y) * v2; };
+  auto T2m = [&](int x) { return c.T2m + x * v2 * o; };
+  auto T2mb = [&](int x) { return c.T2mb + x * v2 * o; };
+  auto OOt = [&](int x, int y) { return c.OOt + (x * o + y) * vo; };
+
+  // V-term contractions (needs Vbar only)
+  VRArgs vr;
+  vr.Vb = c.Vb;
+  vr.t1 = c.t1;
+  vr.fvo = c.fvo;
+  vr.ovov = c.ovov;
+  vr.t2 = c.t2;
+  vr.o = c.o;
+  vr.v = c.v;
+  vr.i = i;
+  vr.j = j;
+  vr.k = k;
+  vr.has_fvo = c.has_fvo;
+  vr.R = c.R;
+  if (!g_record) PAMD_CHECK(launch_vr(vr, c.s));
+
+  // ovvv_bar, rows = virtual pairs, cols = f
+  {  // Vpb[i][a][b|c][f]: P1 (K=c), P2 (K=b)
+    GemmDesc d = gemm(V * V, V, kAccumulate,
+                      {seg(opR(Wb, v2, v, V, 1, 0), op1(T2(k, j), 1, v, 1), V, 1.0),
+                       seg(opR(Wb, v2, 1, V, v, 1), op1(T2(j, k), 1, v, 1), V, 1.0)},
+                      out(Vpb(i), v2, v, V, 0, 1, kBig));
+    PAMD_CHECK(run_gemm(d, c.s));
+  }
+  {  // Vpb[j][b][a|c][f]: P3 (K=c), P4 (K=a)
+    GemmDesc d = gemm(V * V, V, kAccumulate,
+                      {seg(opR(Wb, v, v2, V, 1, 0), op1(T2(k, i), 1, v, 1), V, 1.0),
+                       seg(opR(Wb, v, 1, V, v2, 1), op1(T2(i, k), 1, v, 1), V, 1.0)},
+                      out(Vpb(j), v2, v, V, 0, 1, kBig));
+    PAMD_CHECK(run_gemm(d, c.s));
+  }
+  {  // Vpb[k][c][a|b][f]: rows m = y*v + x with x = c: P5 (K=b), P6 (K=a)
+    GemmDesc d = gemm(V * V, V, kAccumulate,
+                      {seg(opR(Wb, v2, 1, V, v, 1), op1(T2(j, i), 1, v, 1), V, 1.0),
+                       seg(opR(Wb, v, 1, V, v2, 1), op1(T2(i, j), 1, v, 1), V, 1.0)},
+                      out(Vpb(k), v, v2, V, 0, 1, kBig));
+    PAMD_CHECK(run_gemm(d, c.s));
+  }
+
+  // particle t2_bar pieces: split-K over the virtual pair, partials per problem
+  const size_t pp = (size_t)c.S_p * v2;
+  {
+    const GemmDesc ds[6] = {
+        // P1: t2_bar[k,j][c][f] = sum_(a,b) Wb[a,b,c] Vp(i)[a][b][f]
+        gemm(V, V, kPartial, {seg(opK(Wb, 1, v2, v, V, 1), opK(Vp(i), 1, v2, v, V, 1), V * V, 1.0)},
+             out(nullptr, 0, 0, 1, 0, 0, 1), c.S_p, c.part_p + 0 * pp),
+        // P2: t2_bar[j,k][b][f] = sum_(a,c) Wb[a,b,c] Vp(i)[a][c][f]
+        gemm(V, V, kPartial, {seg(opK(Wb, v, v2, 1, V, 0), opK(Vp(i), 1, v2, v, V, 1), V * V, 1.0)},
+             out(nullptr, 0, 0, 1, 0, 0, 1), c.S_p, c.part_p + 1 * pp),
+        // P3: t2_bar[k,i][c][f] = sum_(a,b) Wb[a,b,c] Vp(j)[b][a][f]
+        gemm(V, V, kPartial, {seg(opK(Wb, 1, v2, v, V, 1), opK(Vp(j), 1, v, v2, V, 1), V * V, 1.0)},
+             out(nullptr, 0, 0, 1, 0, 0, 1), c.S_p, c.part_p + 2 * pp),
+        // P4: t2_bar[i,k][a][f] = sum_(b,c) Wb[a,b,c] Vp(j)[b][c][f]
+        gemm(V, V, kPartial, {seg(opK(Wb, v2, v, 1, V, 0), opK(Vp(j), 1, v2, v, V, 1), V * V, 1.0)},
+             out(nullptr, 0, 0, 1, 0, 0, 1), c.S_p, c.part_p + 3 * pp),
+        // P5: t2_bar[j,i][b][f] = sum_(a,c) Wb[a,b,c] Vp(k)[c][a][f]
+        gemm(V, V, kPartial, {seg(opK(Wb, v, v2, 1, V, 0), opK(Vp(k), 1, v, v2, V, 1), V * V, 1.0)},
+             out(nullptr, 0, 0, 1, 0, 0, 1), c.S_p, c.part_p + 4 * pp),
+        // P6: t2_bar[i,j][a][f] = sum_(b,c) Wb[a,b,c] Vp(k)[c][b][f]
+        gemm(V, V, kPartial, {seg(opK(Wb, v2, v, 1, V, 0), opK(Vp(k), 1, v, v2, V, 1), V * V, 1.0)},
+             out(nullptr, 0, 0, 1, 0, 0, 1), c.S_p, c.part_p + 5 * pp),
+    };
+    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(ds[q], c.s));
+  }
+  // hole ovoo_bar pieces: split-K over the virtual pair
+  const size_t ph = (size_t)c.S_h * v * o;
+  {
+    const GemmDesc ds[6] = {
+        // H1: ovoo_bar[i,a,m,j] = -sum_(b,c) Wb[a,b,c] T2m(k)[c][b][m]
+        gemm(V, O, kPartial, {seg(opK(Wb, v2, v, 1, V, 0), opK(T2m(k), 1, o, vo, V, 1), V * V, -1.0)},
+             out(nullptr, 0, 0, 1, 0, 0, 1), c.S_h, c.part_h + 0 * ph),
+        // H2: ovoo_bar[i,a,m,k] = -sum_(b,c) Wb[a,b,c] T2m(j)[b][c][m]
+        gemm(V, O, kPartial, {seg(opK(Wb, v2, v, 1, V, 0), opK(T2m(j), 1, vo, o, V, 1), V * V, -1.0)},
+             out(nullptr, 0, 0, 1, 0, 0, 1), c.S_h, c.part_h + 1 * ph),
+        // H3: ovoo_bar[j,b,m,i] = -sum_(a,c) Wb[a,b,c] T2m(k)[c][a][m]
+        gemm(V, O, kPartial, {seg(opK(Wb, v, v2, 1, V, 0), opK(T2m(k), 1, o, vo, V, 1), V * V, -1.0)},
+             out(nullptr, 0, 0, 1, 0, 0, 1), c.S_h, c.part_h + 2 * ph),
+        // H4: ovoo_bar[j,b,m,k] = -sum_(a,c) Wb[a,b,c] T2m(i)[a][c][m]
+        gemm(V, O, kPartial, {seg(opK(Wb, v, v2, 1, V, 0), opK(T2m(i), 1, vo, o, V, 1), V * V, -1.0)},
+             out(nullptr, 0, 0, 1, 0, 0, 1), c.S_h, c.part_h + 3 * ph),
+        // H5: ovoo_bar[k,c,m,i] = -sum_(a,b) Wb[a,b,c] T2m(j)[b][a][m]
+        gemm(V, O, kPartial, {seg(opK(Wb, 1, v2, v, V, 1), opK(T2m(j), 1, o, vo, V, 1), V * V, -1.0)},
+             out(nullptr, 0, 0, 1, 0, 0, 1), c.S_h, c.part_h + 4 * ph),
+        // H6: ovoo_bar[k,c,m,j] = -sum_(a,b) Wb[a,b,c] T2m(i)[a][b][m]
+        gemm(V, O, kPartial, {seg(opK(Wb, 1, v2, v, V, 1), opK(T2m(i), 1, vo, o, V, 1), V * V, -1.0)},
+             out(nullptr, 0, 0, 1, 0, 0, 1), c.S_h, c.part_h + 5 * ph),
+    };
+    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(ds[q], c.s));
+  }
+  // hole-term t2 reads: T2mb[x][p][q][m] (= dE/d t2[x,m,p,q])
+  {  // T2mb[i]: H4 (p=a,q=c,K=b), H6 (p=a,q=b,K=c)
+    GemmDesc d = gemm(V * V, O, kAccumulate,
+                      {seg(opR(Wb, v2, 1, V, v, 1), op1(OOt(j, k), 1, o, 1), V, -1.0),
+                       seg(opR(Wb, v2, v, V, 1, 0), op1(OOt(k, j), 1, o, 1), V, -1.0)},
+                      out(T2mb(i), vo, o, V, 0, 1, kBig));
+    PAMD_CHECK(run_gemm(d, c.s));
+  }
+  {  // T2mb[j]: H2 (p=b,q=c,K=a), H5 (p=b,q=a,K=c)
+    GemmDesc d = gemm(V * V, O, kAccumulate,
+                      {seg(opR(Wb, v, 1, V, v2, 1), op1(OOt(i, k), 1, o, 1), V, -1.0),
+                       seg(opR(Wb, v, v2, V, 1, 0), op1(OOt(k, i), 1, o, 1), V, -1.0)},
+                      out(T2mb(j), vo, o, V, 0, 1, kBig));
+    PAMD_CHECK(run_gemm(d, c.s));
+  }
+  {  // T2mb[k]: rows m = q*v + p with p = c: H1 (q=b,K=a), H3 (q=a,K=b)
+    GemmDesc d = gemm(V * V, O, kAccumulate,
+                      {seg(opR(Wb, v, 1, V, v2, 1), op1(OOt(i, j), 1, o, 1), V, -1.0),
+                       seg(opR(Wb, v2, 1, V, v, 1), op1(OOt(j, i), 1, o, 1), V, -1.0)},
+                      out(T2mb(k), o, vo, V, 0, 1, kBig));
+    PAMD_CHECK(run_gemm(d, c.s));
+  }
+  // gather into the cotangents
+  CombineArgs ca;
+  ca.R = c.R;
+  ca.part_p = c.part_p;
+  ca.part_h = c.part_h;
+  ca.S_p = c.S_p;
+  ca.S_h = c.S_h;
+  ca.o = c.o;
+  ca.v = c.v;
+  ca.i = i;
+  ca.j = j;
+  ca.k = k;
+  ca.has_fvo = c.has_fvo;
+  ca.t2_bar = c.t2_bar;
+  ca.ovov_bar = c.ovov_bar;
+  ca.ovoo_bar = c.ovoo_bar;
+  ca.t1_bar = c.t1_bar;
+  ca.fvo_bar = c.fvo_bar;
+  if (!g_record) PAMD_CHECK(launch_combine(ca, c.s));
+  return PAMD_OK;
+}
+
+static int run(Ctx &c) {
+  const int o = c.o, v = c.v;
+  PAMD_CHECK(launch_prepare(c.ovvv, c.packed, c.t2, c.ovoo, o, v, c.Vp, c.T2m, c.OOt, c.s));
+  PAMD_CHECK(launch_fill(c.tri, (size_t)o * o * o, 0.0, c.s));
+  const int nt = (v + 7) / 8;
+  const long long norb = (long long)nt * (nt + 1) * (nt + 2) / 6;
+  c.G = std::min(orbit_grid(v), kGmax);
+  if (c.vjp) {
+    const size_t v2 = (size_t)v * v;
+    PAMD_CHECK(launch_fill(c.Vpb, (size_t)o * v2 * v, 0.0, c.s));
+    PAMD_CHECK(launch_fill(c.T2mb, (size_t)o * o * v2, 0.0, c.s));
+    PAMD_CHECK(launch_fill(c.part_mo, (size_t)kGmax * (o + v), 0.0, c.s));
+    PAMD_CHECK(launch_fill(c.t1_bar, (size_t)o * v, 0.0, c.s));
+    PAMD_CHECK(launch_fill(c.t2_bar, (size_t)o * o * v2, 0.0, c.s));
+    PAMD_CHECK(launch_fill(c.ovoo_bar, (size_t)o * v * o * o, 0.0, c.s));
+    PAMD_CHECK(launch_fill(c.ovov_bar, (size_t)o * v * o * v, 0.0, c.s));
+    PAMD_CHECK(launch_fill(c.fvo_bar, (size_t)v * o, 0.0, c.s));
+  }
+  for (int i = 0; i < o; ++i)
+    for (int j = i; j < o; ++j)
+      for (int k = j; k < o; ++k) {
+        if (i == j && j == k) continue;
+        int st = build_w(c, i, j, k);
+        if (st != PAMD_OK) return st;
+        OrbitArgs oa;
+        oa.W = c.W;
+        oa.Vb = c.Vb;
+        oa.t1 = c.t1;
+        oa.t2 = c.t2;
+        oa.ovov = c.ovov;
+        oa.fvo = c.fvo;
+        oa.mo = c.mo;
+        oa.o = o;
+        oa.v = v;
+        oa.i = i;
+        oa.j = j;
+        oa.k = k;
+        oa.has_fvo = c.has_fvo;
+        oa.vjp = c.vjp;
+        oa.nt = nt;
+        oa.norb = norb;
+        oa.omega = (i == j || j == k) ? 1.0 : 2.0;
+        oa.ebar = c.ebar;
+        oa.part_t = c.part_t;
+        oa.part_mo = c.part_mo;
+        PAMD_CHECK(launch_orbit(oa, c.G, c.s));
+        PAMD_CHECK(launch_triple_energy(c.part_t, c.G, c.tri + ((size_t)i * o + j) * o + k, c.s));
+        if (c.vjp) {
+          st = adjoint(c, i, j, k);
+          if (st != PAMD_OK) return st;
+        }
+      }
+  return PAMD_OK;
+}
+
+static int check_common(const pamd_ccsdt_config *cfg, const double *mo, const double *t1,
+                        const double *t2, const double *ovvv, const double *ovoo,
+                        const double *ovov, const double *fvo) {
+  if (!cfg || cfg->nocc < 1 || cfg->nvir < 1) return PAMD_ERR_ARG;
+  if (!mo || !t1 || !t2 || !ovvv || !ovoo || !ovov) return PAMD_ERR_ARG;
+  if (cfg->has_fvo && !fvo) return PAMD_ERR_ARG;
+  if (cfg->precision != PAMD_PREC_FP64) return PAMD_ERR_UNSUPPORTED;
+  if (cfg->nvir > 4096 || cfg->nocc > 1024) return PAMD_ERR_ARG;
+  return PAMD_OK;
+}
+
+static void bind_workspace(Ctx &c, const Layout &L, void *ws) {
+  char *b = (char *)ws;
+  c.Vp = (double *)(b + L.off_Vp);
+  c.T2m = (double *)(b + L.off_T2m);
+  c.OOt = (double *)(b + L.off_OOt);
+  c.W = (double *)(b + L.off_W);
+  c.tri = (double *)(b + L.off_tri);
+  c.part_t = (double *)(b + L.off_part_t);
+  if (c.vjp) {
+    c.Vb = (double *)(b + L.off_Vb);
+    c.Vpb = (double *)(b + L.off_Vpb);
+    c.T2mb = (double *)(b + L.off_T2mb);
+    c.part_mo = (double *)(b + L.off_part_mo);
+    c.R = (double *)(b + L.off_R);
+    c.part_p = (double *)(b + L.off_part_p);
+    c.part_h = (double *)(b + L.off_part_h);
+  } else {
+    c.Vb = c.Vpb = c.T2mb = c.part_mo = c.R = c.part_p = c.part_h = nullptr;
+  }
+  c.S_p = L.S_p;
+  c.S_h = L.S_h;
+}
+
+}  // namespace pamd
+
+using namespace pamd;
+
+extern "C" {
+
+const char *pamd_version(void) { return "pyscfad_amd 0.1.0 (gfx950)"; }
+
+int pamd_ccsdt_workspace_size(const pamd_ccsdt_config *cfg, int with_vjp, size_t *bytes) {
+  if (!cfg || !bytes || cfg->nocc < 1 || cfg->nvir < 1) return PAMD_ERR_ARG;
+  *bytes = make_layout(cfg->nocc, cfg->nvir, with_vjp ? 1 : 0).total;
+  return PAMD_OK;
+}
+
+static int energy_impl(const pamd_ccsdt_config *cfg, const double *mo, const double *t1,
+                       const double *t2, const double *ovvv, const double *ovoo,
+                       const double *ovov, const double *fvo, void *ws, size_t ws_bytes,
+                       hipStream_t s, Ctx &c) {
+  int st = check_common(cfg, mo, t1, t2, ovvv, ovoo, ovov, fvo);
+  if (st != PAMD_OK) return st;
+  Layout L = make_layout(cfg->nocc, cfg->nvir, 0);
+  if (!ws || ws_bytes < L.total) return PAMD_ERR_WORKSPACE;
+  memset(&c, 0, sizeof(c));
+  c.o = cfg->nocc;
+  c.v = cfg->nvir;
+  c.packed = cfg->ovvv_packed ? 1 : 0;
+  c.has_fvo = cfg->has_fvo ? 1 : 0;
+  c.vjp = 0;
+  c.s = s;
+  c.mo = mo;
+  c.t1 = t1;
+  c.t2 = t2;
+  c.ovvv = ovvv;
+  c.ovoo = ovoo;
+  c.ovov = ovov;
+  c.fvo = fvo;
+  bind_workspace(c, L, ws);
+  return run(c);
+}
+
+int pamd_ccsdt_energy(const pamd_ccsdt_config *cfg, const double *mo_energy, const double *t1,
+                      const double *t2, const double *ovvv, const double *ovoo, const double *ovov,
+                      const double *fvo, void *workspace, size_t workspace_bytes, double *e_out,
+                      void *stream) {
+  if (!e_out) return PAMD_ERR_ARG;
+  Ctx c;
+  hipStream_t s = (hipStream_t)stream;
+  int st = energy_impl(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, workspace, workspace_bytes,
+                       s, c);
+  if (st != PAMD_OK) return st;
+  PAMD_CHECK(launch_total_energy(c.tri, c.o, e_out, s));
+  return PAMD_OK;
+}
+
+int pamd_ccsdt_pair_energy(const pamd_ccsdt_config *cfg, const double *mo_energy, const double *t1,
+                           const double *t2, const double *ovvv, const double *ovoo,
+                           const double *ovov, const double *fvo, void *workspace,
+                           size_t workspace_bytes, double *e_pair, void *stream) {
+  if (!e_pair) return PAMD_ERR_ARG;
+  Ctx c;
+  hipStream_t s = (hipStream_t)stream;
+  int st = energy_impl(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, workspace, workspace_bytes,
+                       s, c);
+  if (st != PAMD_OK) return st;
+  PAMD_CHECK(launch_pair_energy(c.tri, c.o, e_pair, s));
+  return PAMD_OK;
+}
+
+int pamd_ccsdt_energy_vjp(const pamd_ccsdt_config *cfg, const double *mo_energy, const double *t1,
+                          const double *t2, const double *ovvv, const double *ovoo,
+                          const double *ovov, const double *fvo, double et_bar, void *workspace,
+                          size_t workspace_bytes, double *e_out, double *mo_energy_bar,
+                          double *t1_bar, double *t2_bar, double *ovvv_bar, double *ovoo_bar,
+                          double *ovov_bar, double *fvo_bar, void *stream) {
+  int st = check_common(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo);
+  if (st != PAMD_OK) return st;
+  if (!e_out || !mo_energy_bar || !t1_bar || !t2_bar || !ovvv_bar || !ovoo_bar || !ovov_bar ||
+      !fvo_bar)
+    return PAMD_ERR_ARG;
+  if (cfg->nvir > 512) return PAMD_ERR_UNSUPPORTED;
+  Layout L = make_layout(cfg->nocc, cfg->nvir, 1);
+  if (!workspace || workspace_bytes < L.total) return PAMD_ERR_WORKSPACE;
+  Ctx c;
+  memset(&c, 0, sizeof(c));
+  c.o = cfg->nocc;
+  c.v = cfg->nvir;
+  c.packed = cfg->ovvv_packed ? 1 : 0;
+  c.has_fvo = cfg->has_fvo ? 1 : 0;
+  c.vjp = 1;
+  c.s = (hipStream_t)stream;
+  c.mo = mo_energy;
+  c.t1 = t1;
+  c.t2 = t2;
+  c.ovvv = ovvv;
+  c.ovoo = ovoo;
+  c.ovov = ovov;
+  c.fvo = fvo;
+  c.ebar = et_bar;
+  c.t1_bar = t1_bar;
+  c.t2_bar = t2_bar;
+  c.ovoo_bar = ovoo_bar;
+  c.ovov_bar = ovov_bar;
+  c.fvo_bar = fvo_bar;
+  bind_workspace(c, L, workspace);
+  st = run(c);
+  if (st != PAMD_OK) return st;
+  const int o = c.o, v = c.v;
+  PAMD_CHECK(launch_total_energy(c.tri, o, e_out, c.s));
+  PAMD_CHECK(launch_reduce_rows(c.part_mo, c.G, o + v, mo_energy_bar, 0, c.s));
+  PAMD_CHECK(launch_fold_ovvv(c.Vpb, o, v, c.packed, ovvv_bar, c.s));
+  PAMD_CHECK(launch_fold_t2m(c.T2mb, o, v, t2_bar, c.s));
+  return PAMD_OK;
+}
+
+int pamd_debug_gemm_descriptors(int nocc, int nvir, int i, int j, int k, int with_vjp,
+                                const unsigned long long *bases, void *out, int max_desc,
+                                int *n_desc, int *desc_bytes) {
+  if (nocc < 1 || nvir < 1 || !bases || !n_desc || !desc_bytes) return PAMD_ERR_ARG;
+  if (i < 0 || j < i || k < j || k >= nocc) return PAMD_ERR_ARG;
+  Layout L = make_layout(nocc, nvir, with_vjp ? 1 : 0);
+  Ctx c;
+  memset(&c, 0, sizeof(c));
+  c.o = nocc;
+  c.v = nvir;
+  c.vjp = with_vjp ? 1 : 0;
+  c.Vp = (double *)bases[0];
+  c.T2m = (double *)bases[1];
+  c.OOt = (double *)bases[2];
+  c.W = (double *)bases[3];
+  c.t2 = (const double *)bases[4];
+  c.Vb = (double *)bases[5];
+  c.Vpb = (double *)bases[6];
+  c.T2mb = (double *)bases[7];
+  c.part_p = (double *)bases[8];
+  c.part_h = (double *)bases[9];
+  c.S_p = L.S_p;
+  c.S_h = L.S_h;
+  std::vector<GemmDesc> rec;
+  g_record = &rec;
+  int st = build_w(c, i, j, k);
+  if (st == PAMD_OK && with_vjp) st = adjoint(c, i, j, k);
+  g_record = nullptr;
+  if (st != PAMD_OK) return st;
+  *desc_bytes = (int)sizeof(GemmDesc);
+  *n_desc = (int)rec.size();
+  if (out) {
+    const int n = std::min((int)rec.size(), max_desc);
+    memcpy(out, rec.data(), sizeof(GemmDesc) * (size_t)n);
+  }
+  return PAMD_OK;
+}
+
+int pamd_debug_split_factors(int nocc, int nvir, int *S_p, int *S_h) {
+  if (nocc < 1 || nvir < 1 || !S_p || !S_h) return PAMD_ERR_ARG;
+  Layout L = make_layout(nocc, nvir, 1);
+  *S_p = L.S_p;
+  *S_h = L.S_h;
+  return PAMD_OK;
+}
+
+}  // extern "C"

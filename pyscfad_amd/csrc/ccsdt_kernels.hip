@@ -1,0 +1,804 @@
+// HIP kernels for the closed-shell (T) energy and its adjoint on gfx950.
+//
+// * k_gemm_f64   -- the FP64 MFMA contraction engine (v_mfma_f64_16x16x4f64):
+//                   64x64 output tile per 256-thread workgroup, K staged through
+//                   double-buffered LDS in chunks of 16, 2x2 MFMA tiles per wave.
+// * k_orbit      -- fused epilogue for one occupied triple: V = t1*ovov + t2*fvo
+//                   terms, Z = r3(W + V), denominators, the energy reduction and,
+//                   for the adjoint, Wbar / Vbar / dE/d(mo_energy) seeds.  Virtual
+//                   indices are tiled 8^3 and a workgroup owns one orbit of tiles
+//                   {(A,B,C) and its permutations}, so the six permuted reads that
+//                   r3 needs stay in LDS.
+// * k_vr_a/k_vr_b -- contractions of Vbar with t1/fvo/ovov/t2 (the V-term adjoint).
+// * k_combine    -- per-triple gather of the adjoint pieces into the cotangents.
+// * prepare/fold -- layout changes at entry/exit.
+#include "ccsdt_common.h"
+
+namespace pamd {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// generic helpers
+// ---------------------------------------------------------------------------
+__global__ void k_fill(double *p, size_t n, double val) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t st = (size_t)gridDim.x * blockDim.x;
+  for (; t < n; t += st) p[t] = val;
+}
+
+hipError_t launch_fill(double *p, size_t n, double val, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, s, p, n, val);
+  return hipGetLastError();
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+// deterministic block sum for 256 threads; scratch >= 4 doubles
+__device__ double block_sum256(double x, double *scratch) {
+  x = wave_sum(x);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) scratch[w] = x;
+  __syncthreads();
+  double r = scratch[0] + scratch[1] + scratch[2] + scratch[3];
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// layout preparation
+//   Vp[x][p][q][f]  = ovvv[x,p,f,q]   (f contiguous: the contraction index)
+//   T2m[x][p][q][m] = t2[x,m,p,q]
+//   OOt[x][y][p][m] = ovoo[x,p,m,y]
+// ---------------------------------------------------------------------------
+__global__ void k_prep_vp(const double *__restrict__ ovvv, int packed, int o, int v,
+                          double *__restrict__ Vp) {
+  const size_t total = (size_t)o * v * v * v;
+  const size_t vv = (size_t)v * v;
+  const size_t npair = (size_t)v * (v + 1) / 2;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    size_t f = t % v;
+    size_t q = (t / v) % v;
+    size_t xp = t / vv;  // x * v + p
+    double val;
+    if (packed) {
+      size_t hi = f > q ? f : q, lo = f > q ? q : f;
+      val = ovvv[xp * npair + hi * (hi + 1) / 2 + lo];
+    } else {
+      val = ovvv[xp * vv + f * v + q];
+    }
+    Vp[t] = val;
+  }
+}
+
+__global__ void k_prep_t2m(const double *__restrict__ t2, int o, int v, double *__restrict__ T2m) {
+  const size_t total = (size_t)o * o * v * v;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    size_t m = t % o;
+    size_t q = (t / o) % v;
+    size_t p = (t / ((size_t)o * v)) % v;
+    size_t x = t / ((size_t)o * v * v);
+    T2m[t] = t2[((x * o + m) * v + p) * v + q];
+  }
+}
+
+__global__ void k_prep_oot(const double *__restrict__ ovoo, int o, int v, double *__restrict__ OOt) {
+  const size_t total = (size_t)o * o * v * o;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    size_t m = t % o;
+    size_t p = (t / o) % v;
+    size_t y = (t / ((size_t)o * v)) % o;
+    size_t x = t / ((size_t)o * v * o);
+    OOt[t] = ovoo[((x * v + p) * o + m) * o + y];
+  }
+}
+
+static unsigned grid_for(size_t n) {
+  size_t b = (n + 255) / 256;
+  if (b > 262144) b = 262144;
+  if (b == 0) b = 1;
+  return (unsigned)b;
+}
+
+hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, const double *ovoo,
+                          int o, int v, double *Vp, double *T2m, double *OOt, hipStream_t s) {
+  hipLaunchKernelGGL(k_prep_vp, dim3(grid_for((size_t)o * v * v * v)), dim3(256), 0, s, ovvv,
+                     packed, o, v, Vp);
+  hipLaunchKernelGGL(k_prep_t2m, dim3(grid_for((size_t)o * o * v * v)), dim3(256), 0, s, t2, o, v,
+                     T2m);
+  hipLaunchKernelGGL(k_prep_oot, dim3(grid_for((size_t)o * o * v * o)), dim3(256), 0, s, ovoo, o,
+                     v, OOt);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// FP64 MFMA contraction engine
+// ---------------------------------------------------------------------------
+constexpr int GMT = 64, GNT = 64, GKC = 16, GLD = 80;  // LDS row stride 80 doubles (640 B)
+
+__device__ __forceinline__ long long op_roff(const OpDesc &d, int r) {
+  int q = r / d.rdiv;
+  int rem = r - q * d.rdiv;
+  return (long long)q * d.rx + (long long)rem * d.ry;
+}
+
+__device__ __forceinline__ long long op_koff(const OpDesc &d, int kk) {
+  int q = kk / d.kdiv;
+  int rem = kk - q * d.kdiv;
+  return (long long)q * d.kx + (long long)rem * d.ky;
+}
+
+// Fetch the 4 elements this thread stages for one operand of chunk g.
+__device__ __forceinline__ void fetch_op(const OpDesc &d, int L, double alpha, int R, int r0,
+                                         int kappa0, int tid, double (&v)[4]) {
+  if (d.rfast) {
+    const int rl = tid & 63;
+    const int r = r0 + rl;
+    const bool rok = r < R;
+    const long long ro = rok ? op_roff(d, r) : 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int kk = kappa0 + (tid >> 6) + 4 * e;
+      v[e] = (rok && kk < L) ? d.p[ro + op_koff(d, kk)] * alpha : 0.0;
+    }
+  } else {
+    const int kk = kappa0 + (tid & 15);
+    const bool kok = kk < L;
+    const long long ko = kok ? op_koff(d, kk) : 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int r = r0 + (tid >> 4) + 16 * e;
+      v[e] = (kok && r < R) ? d.p[op_roff(d, r) + ko] * alpha : 0.0;
+    }
+  }
+}
+
+__device__ __forceinline__ void stash_op(const OpDesc &d, int tid, const double (&v)[4],
+                                         double (*S)[GLD]) {
+  if (d.rfast) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) S[(tid >> 6) + 4 * e][tid & 63] = v[e];
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) S[tid & 15][(tid >> 4) + 16 * e] = v[e];
+  }
+}
+
+__device__ __forceinline__ int seg_of_chunk(const GemmDesc &d, int g) {
+  int s = 0;
+#pragma unroll
+  for (int q = 1; q < kMaxSeg; ++q)
+    if (q < d.nseg && g >= d.chunk_start[q]) s = q;
+  return s;
+}
+
+__global__ __launch_bounds__(256, 2) void k_gemm_f64(GemmDesc d) {
+  __shared__ double As[2][GKC][GLD];
+  __shared__ double Bs[2][GKC][GLD];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int m0 = blockIdx.x * GMT, n0 = blockIdx.y * GNT;
+  const int total = d.chunk_start[d.nseg];
+  const int cps = (total + d.splits - 1) / d.splits;
+  const int g0 = blockIdx.z * cps;
+  const int g1 = min(total, g0 + cps);
+
+  d4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+
+  double ra[4], rb[4];
+  int buf = 0;
+  if (g0 < g1) {
+    const int s = seg_of_chunk(d, g0);
+    const int k0 = (g0 - d.chunk_start[s]) * GKC;
+    fetch_op(d.seg[s].A, d.seg[s].L, d.seg[s].alpha, d.M, m0, k0, tid, ra);
+    fetch_op(d.seg[s].B, d.seg[s].L, 1.0, d.N, n0, k0, tid, rb);
+  }
+  for (int g = g0; g < g1; ++g) {
+    const int s = seg_of_chunk(d, g);
+    stash_op(d.seg[s].A, tid, ra, As[buf]);
+    stash_op(d.seg[s].B, tid, rb, Bs[buf]);
+    __syncthreads();
+    if (g + 1 < g1) {
+      const int s1 = seg_of_chunk(d, g + 1);
+      const int k1 = (g + 1 - d.chunk_start[s1]) * GKC;
+      fetch_op(d.seg[s1].A, d.seg[s1].L, d.seg[s1].alpha, d.M, m0, k1, tid, ra);
+      fetch_op(d.seg[s1].B, d.seg[s1].L, 1.0, d.N, n0, k1, tid, rb);
+    }
+#pragma unroll
+    for (int ks = 0; ks < GKC / 4; ++ks) {
+      const int kr = ks * 4 + (lane >> 4);
+      const double a0 = As[buf][kr][wm * 32 + (lane & 15)];
+      const double a1 = As[buf][kr][wm * 32 + 16 + (lane & 15)];
+      const double b0 = Bs[buf][kr][wn * 32 + (lane & 15)];
+      const double b1 = Bs[buf][kr][wn * 32 + 16 + (lane & 15)];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    buf ^= 1;
+  }
+
+  // D layout of v_mfma_f64_16x16x4f64 on gfx950: lane l, element r holds
+  // row 4*r + l/16, column l%16 of the 16x16 tile.
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm * 32 + a * 16 + 4 * r + (lane >> 4);
+        const int n = n0 + wn * 32 + b * 16 + (lane & 15);
+        if (m >= d.M || n >= d.N) continue;
+        const double val = acc[a][b][r];
+        if (d.mode == kPartial) {
+          d.partial[((size_t)blockIdx.z * d.M + m) * d.N + n] = val;
+        } else {
+          const int qm = m / d.C.mdiv, rm = m - qm * d.C.mdiv;
+          const int qn = n / d.C.ndiv, rn = n - qn * d.C.ndiv;
+          double *c = d.C.p + (long long)qm * d.C.mx + (long long)rm * d.C.my +
+                      (long long)qn * d.C.nx + (long long)rn * d.C.ny;
+          if (d.mode == kAccumulate)
+            *c += val;
+          else
+            *c = val;
+        }
+      }
+}
+
+hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s) {
+  if (d.M <= 0 || d.N <= 0) return hipSuccess;
+  dim3 grid((d.M + GMT - 1) / GMT, (d.N + GNT - 1) / GNT, d.splits);
+  hipLaunchKernelGGL(k_gemm_f64, grid, dim3(256), 0, s, d);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Orbit epilogue
+// ---------------------------------------------------------------------------
+constexpr int OT = 8, OT3 = OT * OT * OT;
+
+__device__ __forceinline__ void decode_orbit(long long q, int &A, int &B, int &C) {
+  int a = 0;
+  while ((long long)(a + 1) * (a + 2) * (a + 3) / 6 <= q) ++a;
+  long long rem = q - (long long)a * (a + 1) * (a + 2) / 6;
+  int b = 0;
+  while ((long long)(b + 1) * (b + 2) / 2 <= rem) ++b;
+  A = a;
+  B = b;
+  C = (int)(rem - (long long)b * (b + 1) / 2);
+}
+
+int orbit_grid(int v) {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+  }
+  long long nt = (v + OT - 1) / OT;
+  long long norb = nt * (nt + 1) * (nt + 2) / 6;
+  long long g = 2LL * cus;
+  if (g > norb) g = norb;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+__global__ __launch_bounds__(256) void k_orbit(OrbitArgs A) {
+  __shared__ double Ws[6][OT3];
+  __shared__ double Vs[6][OT3];
+  __shared__ double Ds[6][OT3];
+  __shared__ int pl[6][3];
+  __shared__ int pmap[5][6];
+  __shared__ int npl_s;
+  __shared__ double red[4];
+  extern __shared__ double vir[];  // v doubles: dE/de_virtual accumulated by this workgroup
+
+  const int tid = threadIdx.x;
+  const int o = A.o, v = A.v, n = o + v;
+  const int i = A.i, j = A.j, k = A.k;
+  const double eo = A.mo[i] + A.mo[j] + A.mo[k];
+  const double *ev = A.mo + o;
+  const double g = A.ebar * A.omega;
+  for (int x = tid; x < v; x += 256) vir[x] = 0.0;
+
+  double eacc = 0.0, occ = 0.0;
+  const size_t vv = (size_t)v * v;
+  // slab bases
+  const double *t1 = A.t1, *ovov = A.ovov, *t2 = A.t2, *fvo = A.fvo;
+#define OVOV(x, p, y, q) ovov[(((size_t)(x) * v + (p)) * o + (y)) * v + (q)]
+#define T2(x, y, p, q) t2[(((size_t)(x) * o + (y)) * v + (p)) * v + (q)]
+
+  for (long long q = blockIdx.x; q < A.norb; q += gridDim.x) {
+    if (tid == 0) {
+      int a, b, c;
+      decode_orbit(q, a, b, c);
+      const int cand[6][3] = {{a, b, c}, {a, c, b}, {b, a, c}, {b, c, a}, {c, a, b}, {c, b, a}};
+      int np = 0;
+      for (int s = 0; s < 6; ++s) {
+        bool dup = false;
+        for (int t = 0; t < np; ++t)
+          if (pl[t][0] == cand[s][0] && pl[t][1] == cand[s][1] && pl[t][2] == cand[s][2]) dup = true;
+        if (!dup) {
+          pl[np][0] = cand[s][0];
+          pl[np][1] = cand[s][1];
+          pl[np][2] = cand[s][2];
+          ++np;
+        }
+      }
+      npl_s = np;
+      for (int p = 0; p < np; ++p) {
+        const int P1 = pl[p][0], P2 = pl[p][1], P3 = pl[p][2];
+        const int tgt[5][3] = {{P2, P3, P1}, {P3, P1, P2}, {P3, P2, P1}, {P1, P3, P2}, {P2, P1, P3}};
+        for (int s = 0; s < 5; ++s)
+          for (int t = 0; t < np; ++t)
+            if (pl[t][0] == tgt[s][0] && pl[t][1] == tgt[s][1] && pl[t][2] == tgt[s][2])
+              pmap[s][p] = t;
+      }
+    }
+    __syncthreads();
+    const int np = npl_s;
+    for (int e = tid; e < np * OT3; e += 256) {
+      const int p = e / OT3, loc = e - p * OT3;
+      const int a = pl[p][0] * OT + (loc >> 6);
+      const int b = pl[p][1] * OT + ((loc >> 3) & 7);
+      const int c = pl[p][2] * OT + (loc & 7);
+      double wv = 0.0, vv_ = 0.0;
+      if (a < v && b < v && c < v) {
+        wv = A.W[(size_t)a * vv + (size_t)b * v + c];
+        const double t1a = t1[i * v + a], t1b = t1[j * v + b], t1c = t1[k * v + c];
+        vv_ = (OVOV(i, a, j, b) + OVOV(j, b, i, a)) * t1c + (OVOV(i, a, k, c) + OVOV(k, c, i, a)) * t1b +
+              (OVOV(j, b, k, c) + OVOV(k, c, j, b)) * t1a;
+        if (A.has_fvo) {
+          const double fck = fvo[c * o + k], fbj = fvo[b * o + j], fai = fvo[a * o + i];
+          vv_ += (T2(j, i, b, a) + T2(i, j, a, b)) * fck + (T2(k, i, c, a) + T2(i, k, a, c)) * fbj +
+                 (T2(k, j, c, b) + T2(j, k, b, c)) * fai;
+        }
+        vv_ *= 0.5;
+      }
+      Ws[p][loc] = wv;
+      Vs[p][loc] = vv_;
+    }
+    __syncthreads();
+    for (int e = tid; e < np * OT3; e += 256) {
+      const int p = e / OT3, loc = e - p * OT3;
+      const int al = loc >> 6, be = (loc >> 3) & 7, ga = loc & 7;
+      const int a = pl[p][0] * OT + al, b = pl[p][1] * OT + be, c = pl[p][2] * OT + ga;
+      double dbar = 0.0;
+      if (a < v && b < v && c < v) {
+        // permuted element positions: bca, cab, cba, acb, bac
+        const int l_bca = (be << 6) | (ga << 3) | al;
+        const int l_cab = (ga << 6) | (al << 3) | be;
+        const int l_cba = (ga << 6) | (be << 3) | al;
+        const int l_acb = (al << 6) | (ga << 3) | be;
+        const int l_bac = (be << 6) | (al << 3) | ga;
+        const int p0 = pmap[0][p], p1 = pmap[1][p], p2 = pmap[2][p], p3 = pmap[3][p],
+                  p4 = pmap[4][p];
+        const double w0 = Ws[p][loc];
+        const double x0 = w0 + Vs[p][loc];
+        const double w1 = Ws[p0][l_bca], w2 = Ws[p1][l_cab], w3 = Ws[p2][l_cba],
+                     w4 = Ws[p3][l_acb], w5 = Ws[p4][l_bac];
+        const double x1 = w1 + Vs[p0][l_bca], x2 = w2 + Vs[p1][l_cab], x3 = w3 + Vs[p2][l_cba],
+                     x4 = w4 + Vs[p3][l_acb], x5 = w5 + Vs[p4][l_bac];
+        const double Z = 4.0 * x0 + x1 + x2 - 2.0 * (x3 + x4 + x5);
+        const double D = eo - ev[a] - ev[b] - ev[c];
+        const double rD = 1.0 / D;
+        eacc += w0 * Z * rD;
+        if (A.vjp) {
+          const double r3W = 4.0 * w0 + w1 + w2 - 2.0 * (w3 + w4 + w5);
+          const size_t idx = (size_t)a * vv + (size_t)b * v + c;
+          const double vbar = g * r3W * rD;
+          A.W[idx] = g * Z * rD + vbar;
+          A.Vb[idx] = vbar;
+          dbar = -g * w0 * Z * rD * rD;
+          occ += dbar;
+        }
+      }
+      Ds[p][loc] = dbar;
+    }
+    __syncthreads();
+    if (A.vjp) {
+      for (int t = tid; t < np * 24; t += 256) {
+        const int p = t / 24, s = (t >> 3) % 3, x = t & 7;
+        double sum = 0.0;
+        for (int y = 0; y < 8; ++y)
+          for (int z = 0; z < 8; ++z) {
+            const int loc = s == 0 ? (x << 6) | (y << 3) | z
+                                   : (s == 1 ? (y << 6) | (x << 3) | z : (y << 6) | (z << 3) | x);
+            sum += Ds[p][loc];
+          }
+        const int vi = pl[p][s] * OT + x;
+        if (vi < v) atomicAdd(&vir[vi], -sum);
+      }
+      __syncthreads();
+    }
+  }
+  const double etot = block_sum256(eacc, red);
+  const double otot = block_sum256(occ, red);
+  if (tid == 0) A.part_t[blockIdx.x] = etot;
+  if (A.vjp) {
+    double *pm = A.part_mo + (size_t)blockIdx.x * n;
+    for (int x = tid; x < v; x += 256) pm[o + x] += vir[x];
+    if (tid == 0) {
+      pm[i] += otot;
+      pm[j] += otot;
+      pm[k] += otot;
+    }
+  }
+#undef OVOV
+#undef T2
+}
+
+hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_orbit, dim3(grid), dim3(256), sizeof(double) * a.v, s, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// V-bar contractions for one triple.  Scratch R (v x v blocks):
+//   0 R1[a,b] = sum_c Vb t1[k,c]        1 R2[a,b] = sum_c Vb fvo[c,k]
+//   2 R3[a,c] = sum_b Vb t1[j,b]        3 R4[a,c] = sum_b Vb fvo[b,j]
+//   4 R5[b,c] = sum_a Vb t1[i,a]        5 R6[b,c] = sum_a Vb fvo[a,i]
+//   6 Pk[a,c] = sum_b Vb (ovov[i,a,j,b] + ovov[j,b,i,a])
+//   7 Fk[a,c] = sum_b Vb (t2[j,i,b,a] + t2[i,j,a,b])
+//   8 Pj[a,b] = sum_c Vb (ovov[i,a,k,c] + ovov[k,c,i,a])
+//   9 Fj[a,b] = sum_c Vb (t2[k,i,c,a] + t2[i,k,a,c])
+//  10*v^2 + a : t1i[a] = sum_bc Vb (ovov[j,b,k,c] + ovov[k,c,j,b])
+//  10*v^2 + v + a : fvoi[a] = sum_bc Vb (t2[k,j,c,b] + t2[j,k,b,c])
+// ---------------------------------------------------------------------------
+constexpr int kMaxVQ = 8;  // v <= 512 for the adjoint
+
+// write the 4-wave sum of a per-lane array (element c = lane + 64 qq) to dst[c]
+#define PAMD_WAVE_COMBINE(arr, dst)                                          \
+  do {                                                                       \
+    _Pragma("unroll") for (int qq = 0; qq < kMaxVQ; ++qq) {                  \
+      const int c_ = lane + 64 * qq;                                         \
+      if (qq < nq && c_ < v) red[wave * 512 + c_] = (arr)[qq];               \
+    }                                                                        \
+    __syncthreads();                                                         \
+    for (int c_ = tid; c_ < v; c_ += 256)                                    \
+      (dst)[c_] = red[c_] + red[512 + c_] + red[1024 + c_] + red[1536 + c_]; \
+    __syncthreads();                                                         \
+  } while (0)
+
+__global__ __launch_bounds__(256) void k_vr_a(VRArgs A) {
+  __shared__ double red[4 * 512];
+  __shared__ double cst[4][512];  // per-c constants: t1[k,c], fvo[c,k], S2[a,c], T2[a,c]
+  __shared__ double sred[4];
+  const int a = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int o = A.o, v = A.v, i = A.i, j = A.j, k = A.k;
+  const size_t vv = (size_t)v * v;
+  const double *ovov = A.ovov, *t2 = A.t2, *t1 = A.t1, *fvo = A.fvo;
+#define OVOV(x, p, y, q) ovov[(((size_t)(x) * v + (p)) * o + (y)) * v + (q)]
+#define T2(x, y, p, q) t2[(((size_t)(x) * o + (y)) * v + (p)) * v + (q)]
+  const int nq = (v + 63) / 64;
+  for (int c = tid; c < v; c += 256) {
+    cst[0][c] = t1[k * v + c];
+    cst[2][c] = OVOV(i, a, k, c) + OVOV(k, c, i, a);
+    cst[1][c] = A.has_fvo ? fvo[c * o + k] : 0.0;
+    cst[3][c] = T2(k, i, c, a) + T2(i, k, a, c);
+  }
+  __syncthreads();
+  double r3[kMaxVQ], r4[kMaxVQ], pk[kMaxVQ], fk[kMaxVQ];
+#pragma unroll
+  for (int qq = 0; qq < kMaxVQ; ++qq) r3[qq] = r4[qq] = pk[qq] = fk[qq] = 0.0;
+  double t1i = 0.0, fvoi = 0.0;
+  double *R = A.R;
+  for (int b = wave; b < v; b += 4) {
+    const double t1j = t1[j * v + b];
+    const double S1 = OVOV(i, a, j, b) + OVOV(j, b, i, a);
+    const double fvj = A.has_fvo ? fvo[b * o + j] : 0.0;
+    const double T1 = T2(j, i, b, a) + T2(i, j, a, b);
+    const double *row = A.Vb + (size_t)a * vv + (size_t)b * v;
+    double sr1 = 0.0, sr2 = 0.0, spj = 0.0, sfj = 0.0;
+#pragma unroll
+    for (int qq = 0; qq < kMaxVQ; ++qq) {
+      const int c = lane + 64 * qq;
+      if (qq < nq && c < v) {
+        const double x = row[c];
+        sr1 += x * cst[0][c];
+        spj += x * cst[2][c];
+        r3[qq] += x * t1j;
+        pk[qq] += x * S1;
+        t1i += x * (OVOV(j, b, k, c) + OVOV(k, c, j, b));
+        // fvo_bar pieces depend on t2 only; the fvo-valued pieces only matter when fvo != 0
+        sfj += x * cst[3][c];
+        fk[qq] += x * T1;
+        fvoi += x * (T2(k, j, c, b) + T2(j, k, b, c));
+        if (A.has_fvo) {
+          sr2 += x * cst[1][c];
+          r4[qq] += x * fvj;
+        }
+      }
+    }
+    sr1 = wave_sum(sr1);
+    spj = wave_sum(spj);
+    sr2 = wave_sum(sr2);
+    sfj = wave_sum(sfj);
+    if (lane == 0) {
+      R[0 * vv + (size_t)a * v + b] = sr1;
+      R[1 * vv + (size_t)a * v + b] = sr2;
+      R[8 * vv + (size_t)a * v + b] = spj;
+      R[9 * vv + (size_t)a * v + b] = sfj;
+    }
+  }
+  __syncthreads();
+  PAMD_WAVE_COMBINE(r3, R + 2 * vv + (size_t)a * v);
+  PAMD_WAVE_COMBINE(r4, R + 3 * vv + (size_t)a * v);
+  PAMD_WAVE_COMBINE(pk, R + 6 * vv + (size_t)a * v);
+  PAMD_WAVE_COMBINE(fk, R + 7 * vv + (size_t)a * v);
+  const double t1i_s = block_sum256(t1i, sred);
+  const double fvoi_s = block_sum256(fvoi, sred);
+  if (tid == 0) {
+    R[10 * vv + a] = t1i_s;
+    R[10 * vv + v + a] = fvoi_s;
+  }
+#undef OVOV
+#undef T2
+}
+
+__global__ __launch_bounds__(256) void k_vr_b(VRArgs A) {
+  __shared__ double red[4 * 512];
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int o = A.o, v = A.v, i = A.i;
+  const size_t vv = (size_t)v * v;
+  const int nq = (v + 63) / 64;
+  double r5[kMaxVQ], r6[kMaxVQ];
+#pragma unroll
+  for (int qq = 0; qq < kMaxVQ; ++qq) r5[qq] = r6[qq] = 0.0;
+  for (int a = wave; a < v; a += 4) {
+    const double t1a = A.t1[i * v + a];
+    const double fa = A.has_fvo ? A.fvo[a * o + i] : 0.0;
+    const double *row = A.Vb + (size_t)a * vv + (size_t)b * v;
+#pragma unroll
+    for (int qq = 0; qq < kMaxVQ; ++qq) {
+      const int c = lane + 64 * qq;
+      if (qq < nq && c < v) {
+        const double x = row[c];
+        r5[qq] += x * t1a;
+        r6[qq] += x * fa;
+      }
+    }
+  }
+  PAMD_WAVE_COMBINE(r5, A.R + 4 * vv + (size_t)b * v);
+  PAMD_WAVE_COMBINE(r6, A.R + 5 * vv + (size_t)b * v);
+}
+#undef PAMD_WAVE_COMBINE
+
+hipError_t launch_vr(const VRArgs &a, hipStream_t s) {
+  if (a.v > 64 * kMaxVQ) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_vr_a, dim3(a.v), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_vr_b, dim3(a.v), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Per-triple gather into the cotangents.  One thread owns one (p,q) position
+// of every slab it touches, so coinciding occupied indices (i == j etc.) are
+// summed sequentially in the same thread: no atomics, deterministic.
+// ---------------------------------------------------------------------------
+__global__ void k_combine(CombineArgs A) {
+  const int o = A.o, v = A.v, i = A.i, j = A.j, k = A.k;
+  const size_t vv = (size_t)v * v;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double *R = A.R;
+  if (t < vv) {
+    const int p = (int)(t / v), q = (int)(t % v);
+    const size_t pq = (size_t)p * v + q, qp = (size_t)q * v + p;
+    // ovov_bar[x,p,y,q]
+    auto ob = [&](int x, int y) -> double & {
+      return A.ovov_bar[(((size_t)x * v + p) * o + y) * v + q];
+    };
+    ob(i, j) += 0.5 * R[0 * vv + pq];
+    ob(j, i) += 0.5 * R[0 * vv + qp];
+    ob(i, k) += 0.5 * R[2 * vv + pq];
+    ob(k, i) += 0.5 * R[2 * vv + qp];
+    ob(j, k) += 0.5 * R[4 * vv + pq];
+    ob(k, j) += 0.5 * R[4 * vv + qp];
+    // t2_bar[x,y,p,q]: particle split-K partials (problem order P1..P6) + V terms
+    auto tb = [&](int x, int y) -> double & {
+      return A.t2_bar[(((size_t)x * o + y) * v + p) * v + q];
+    };
+    const int xs[6] = {k, j, k, i, j, i};
+    const int ys[6] = {j, k, i, k, i, j};
+    for (int pr = 0; pr < 6; ++pr) {
+      double s = 0.0;
+      const double *pp = A.part_p + (size_t)pr * A.S_p * vv + pq;
+      for (int sp = 0; sp < A.S_p; ++sp) s += pp[(size_t)sp * vv];
+      if (A.has_fvo) {
+        switch (pr) {
+          case 0: s += 0.5 * R[5 * vv + qp]; break;  // t2_bar[k,j,c,b] += R6[b,c]/2
+          case 1: s += 0.5 * R[5 * vv + pq]; break;  // t2_bar[j,k,b,c] += R6[b,c]/2
+          case 2: s += 0.5 * R[3 * vv + qp]; break;  // t2_bar[k,i,c,a] += R4[a,c]/2
+          case 3: s += 0.5 * R[3 * vv + pq]; break;  // t2_bar[i,k,a,c] += R4[a,c]/2
+          case 4: s += 0.5 * R[1 * vv + qp]; break;  // t2_bar[j,i,b,a] += R2[a,b]/2
+          case 5: s += 0.5 * R[1 * vv + pq]; break;  // t2_bar[i,j,a,b] += R2[a,b]/2
+        }
+      }
+      tb(xs[pr], ys[pr]) += s;
+    }
+  }
+  if (t < (size_t)v * o) {
+    // ovoo_bar[x,p,m,y] from the hole split-K partials H1..H6
+    const int p = (int)(t / o), m = (int)(t % o);
+    const int xs[6] = {i, i, j, j, k, k};
+    const int ys[6] = {j, k, i, k, i, j};
+    const size_t vo = (size_t)v * o;
+    for (int pr = 0; pr < 6; ++pr) {
+      double s = 0.0;
+      const double *pp = A.part_h + (size_t)pr * A.S_h * vo + t;
+      for (int sp = 0; sp < A.S_h; ++sp) s += pp[(size_t)sp * vo];
+      A.ovoo_bar[(((size_t)xs[pr] * v + p) * o + m) * o + ys[pr]] += s;
+    }
+  }
+  if (t < (size_t)v) {
+    const int c = (int)t;
+    double sk = 0.0, sj = 0.0, fk = 0.0, fj = 0.0;
+    for (int a = 0; a < v; ++a) {
+      sk += R[6 * vv + (size_t)a * v + c];
+      sj += R[8 * vv + (size_t)a * v + c];
+      fk += R[7 * vv + (size_t)a * v + c];
+      fj += R[9 * vv + (size_t)a * v + c];
+    }
+    A.t1_bar[k * v + c] += 0.5 * sk;
+    A.t1_bar[j * v + c] += 0.5 * sj;
+    A.t1_bar[i * v + c] += 0.5 * R[10 * vv + c];
+    {
+      A.fvo_bar[(size_t)c * o + k] += 0.5 * fk;
+      A.fvo_bar[(size_t)c * o + j] += 0.5 * fj;
+      A.fvo_bar[(size_t)c * o + i] += 0.5 * R[10 * vv + v + c];
+    }
+  }
+}
+
+hipError_t launch_combine(const CombineArgs &a, hipStream_t s) {
+  size_t n = (size_t)a.v * a.v;
+  if ((size_t)a.v * a.o > n) n = (size_t)a.v * a.o;
+  hipLaunchKernelGGL(k_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// exit folds and small reductions
+// ---------------------------------------------------------------------------
+// ovvv_bar from Vpb[x][p][q][f] (= dE/d ovvv[x,p,f,q]):
+//   full:   out[x,p,f,q] = Vpb[x][p][q][f]
+//   packed: out[x,p,f(f+1)/2+q] = g[f,q] + g[q,f] (q < f), g[f,f] (q == f)
+__global__ void k_fold_ovvv(const double *__restrict__ Vpb, int o, int v, int packed,
+                            double *__restrict__ out) {
+  const size_t vv = (size_t)v * v;
+  const size_t npair = (size_t)v * (v + 1) / 2;
+  const size_t total = (size_t)o * v * (packed ? npair : vv);
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    if (packed) {
+      const size_t xp = t / npair, r = t % npair;
+      size_t f = (size_t)((sqrt(8.0 * (double)r + 1.0) - 1.0) * 0.5);
+      while (f * (f + 1) / 2 > r) --f;
+      while ((f + 1) * (f + 2) / 2 <= r) ++f;
+      const size_t q = r - f * (f + 1) / 2;
+      const double *base = Vpb + xp * vv;
+      double val = base[q * v + f];
+      if (q != f) val += base[f * v + q];
+      out[t] = val;
+    } else {
+      const size_t xp = t / vv, r = t % vv;
+      const size_t f = r / v, q = r % v;
+      out[t] = Vpb[xp * vv + q * v + f];
+    }
+  }
+}
+
+hipError_t launch_fold_ovvv(const double *Vpb, int o, int v, int packed, double *out, hipStream_t s) {
+  size_t n = (size_t)o * v * (packed ? (size_t)v * (v + 1) / 2 : (size_t)v * v);
+  hipLaunchKernelGGL(k_fold_ovvv, dim3(grid_for(n)), dim3(256), 0, s, Vpb, o, v, packed, out);
+  return hipGetLastError();
+}
+
+// t2_bar[x,m,p,q] += T2mb[x][p][q][m]
+__global__ void k_fold_t2m(const double *__restrict__ T2mb, int o, int v, double *__restrict__ t2_bar) {
+  const size_t total = (size_t)o * o * v * v;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t q = t % v, p = (t / v) % v, m = (t / ((size_t)v * v)) % o,
+                 x = t / ((size_t)v * v * o);
+    t2_bar[t] += T2mb[((x * v + p) * v + q) * o + m];
+  }
+}
+
+hipError_t launch_fold_t2m(const double *T2mb, int o, int v, double *t2_bar, hipStream_t s) {
+  hipLaunchKernelGGL(k_fold_t2m, dim3(grid_for((size_t)o * o * v * v)), dim3(256), 0, s, T2mb, o,
+                     v, t2_bar);
+  return hipGetLastError();
+}
+
+// out[c] (+)= sum_r part[r][c], fixed order
+__global__ void k_reduce_rows(const double *part, int rows, int cols, double *out, int accumulate) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  double s = 0.0;
+  for (int r = 0; r < rows; ++r) s += part[(size_t)r * cols + c];
+  out[c] = accumulate ? out[c] + s : s;
+}
+
+hipError_t launch_reduce_rows(const double *part, int rows, int cols, double *out, int accumulate,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_rows, dim3((cols + 255) / 256), dim3(256), 0, s, part, rows, cols,
+                     out, accumulate);
+  return hipGetLastError();
+}
+
+__global__ void k_triple_energy(const double *part_t, int G, double *dst) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int r = threadIdx.x; r < G; r += 256) s += part_t[r];
+  s = block_sum256(s, red);
+  if (threadIdx.x == 0) *dst = s;
+}
+
+hipError_t launch_triple_energy(const double *part_t, int G, double *dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_triple_energy, dim3(1), dim3(256), 0, s, part_t, G, dst);
+  return hipGetLastError();
+}
+
+// tri_e[i][j][k] holds sum_abc W Z / D for sorted i <= j <= k (zero elsewhere).
+// E = 1/3 sum over ordered triples = sum_{i<=j<=k} 2 s_ijk tri_e.
+__global__ void k_total_energy(const double *tri_e, int o, double *e_out) {
+  __shared__ double red[4];
+  double s = 0.0;
+  const int n3 = o * o * o;
+  for (int t = threadIdx.x; t < n3; t += 256) {
+    const int i = t / (o * o), j = (t / o) % o, k = t % o;
+    if (i <= j && j <= k) {
+      const double w = (i == j && j == k) ? (1.0 / 3.0) : ((i == j || j == k) ? 1.0 : 2.0);
+      s += w * tri_e[t];
+    }
+  }
+  s = block_sum256(s, red);
+  if (threadIdx.x == 0) *e_out = s;
+}
+
+hipError_t launch_total_energy(const double *tri_e, int o, double *e_out, hipStream_t s) {
+  hipLaunchKernelGGL(k_total_energy, dim3(1), dim3(256), 0, s, tri_e, o, e_out);
+  return hipGetLastError();
+}
+
+// e_pair[x][y] = 1/3 sum_z tri_e[sort(x,y,z)]
+__global__ void k_pair_energy(const double *tri_e, int o, double *e_pair) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= o * o) return;
+  const int x = t / o, y = t % o;
+  double s = 0.0;
+  for (int z = 0; z < o; ++z) {
+    int a = x, b = y, c = z, tmp;
+    if (a > b) { tmp = a; a = b; b = tmp; }
+    if (b > c) { tmp = b; b = c; c = tmp; }
+    if (a > b) { tmp = a; a = b; b = tmp; }
+    s += tri_e[((size_t)a * o + b) * o + c];
+  }
+  e_pair[t] = s / 3.0;
+}
+
+hipError_t launch_pair_energy(const double *tri_e, int o, double *e_pair, hipStream_t s) {
+  hipLaunchKernelGGL(k_pair_energy, dim3((o * o + 255) / 256), dim3(256), 0, s, tri_e, o, e_pair);
+  return hipGetLastError();
+}
+
+}  // namespace pamd
