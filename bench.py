@@ -1,0 +1,182 @@
+#!/usr/bin/env python
+"""Benchmark of the closed-shell (T) energy + VJP on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|bf16x3]
+
+One step = one E(T)+VJP pass (E and every cotangent block, et_bar = 1) over the
+seeded synthetic problem nocc=40, nvir=400, naux=1600 (packed ovvv, the DF layout),
+with all inputs already resident in HBM when the timer starts.  Prints one JSON
+line: seconds per E(T)+VJP, the roofline of the dominant kernel (the FP64 MFMA
+contraction engine; its launches are timed with hipEvents on their own stream
+for one occupied triple in --profile-every), and the reference's own C
+(oracle/_ref, built from the reference sources) timed on this host's cores on a
+bounded sample and scaled by algorithmic flops.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAKS = {'fp64': 78.6, 'bf16': 2500.0, 'fp32': 157.3}  # dense MFMA TFLOP/s, MI355X
+
+
+def alg_flops(o, v):
+    """E + VJP algorithmic flops: 3 x 2 o^3 v^3 (o + v)."""
+    return 3.0 * 2.0 * o ** 3 * v ** 3 * (o + v)
+
+
+def cpu_baseline(o_full, v_full, sample, threads):
+    """Reference C (oracle/_ref) E+VJP on a smaller seeded problem, scaled by flops."""
+    from oracle import ref_harness as R
+    from pyscfad_amd.gen import make_problem
+    so, sv = sample
+    if not R.available():
+        return None
+    os.environ.setdefault('OMP_NUM_THREADS', str(threads))
+    p = make_problem(so, sv, 4 * sv, seed=0)
+    args = (p.mo_energy, p.t1, p.t2, p.ovvv, p.ovoo, p.ovov, p.fvo)
+    t0 = time.perf_counter()
+    R.energy(*args)
+    R.energy_vjp(*args)
+    dt = time.perf_counter() - t0
+    scale = alg_flops(o_full, v_full) / alg_flops(so, sv)
+    return {'value': dt * scale, 'unit': 's', 'cores': threads, 'kind': 'reference',
+            'sample': (f'reference C E(T)+VJP at o{so}/v{sv}/naux{4 * sv} took {dt:.2f} s on '
+                       f'{threads} threads; scaled by algorithmic flops x{scale:.4g} to '
+                       f'o{o_full}/v{v_full}'),
+            'gflops': alg_flops(so, sv) / dt / 1e9}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=1)
+    ap.add_argument('--warmup', type=int, default=0)
+    ap.add_argument('--precision', default='fp64', choices=['fp64', 'bf16x3', 'fp32x2'])
+    ap.add_argument('--nocc', type=int, default=40)
+    ap.add_argument('--nvir', type=int, default=400)
+    ap.add_argument('--naux', type=int, default=None)
+    ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--profile-every', type=int, default=64)
+    ap.add_argument('--cpu-sample', default='16,160', help='nocc,nvir of the CPU baseline sample')
+    ap.add_argument('--cpu-threads', type=int, default=16)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--energy-only', action='store_true', help='time E(T) without the VJP')
+    ap.add_argument('--out', default=None, help='also write the JSON line to this file')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py: this build runs on one GPU only; the (T) path would shard over '
+              'occupied triples across GPUs (see DESIGN.md), which is not part of it. '
+              f'--gpus {args.gpus} is not supported.', file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from pyscfad_amd import _lib
+    from pyscfad_amd.cc import _ops
+    from pyscfad_amd.gen import make_problem
+
+    if not torch.cuda.is_available():
+        print('bench.py: no GPU visible', file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device('cuda', 0)
+    o, v = args.nocc, args.nvir
+    naux = args.naux or 4 * v
+
+    # untimed: library load + a tiny problem through every kernel
+    tiny = make_problem(3, 40, 160, seed=1)
+    tin = _ops.Inputs(tiny.mo_energy, tiny.t1, tiny.t2, tiny.ovvv_packed(), tiny.ovoo,
+                      tiny.ovov, None, precision=args.precision, device=dev)
+    _ops.energy_vjp(tin)
+    torch.cuda.synchronize()
+
+    t_gen = time.perf_counter()
+    prob = make_problem(o, v, naux, seed=args.seed, device=dev)
+    idx, idy = torch.tril_indices(v, v, device=dev)
+    ovvv_packed = prob.ovvv[:, :, idx, idy].contiguous()
+    del prob.ovvv
+    inp = _ops.Inputs(prob.mo_energy, prob.t1, prob.t2, ovvv_packed, prob.ovoo, prob.ovov,
+                      None, precision=args.precision, device=dev)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t_gen
+
+    def step():
+        if args.energy_only:
+            return _ops.energy(inp), None
+        return _ops.energy_vjp(inp, et_bar=1.0)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    _lib.profile_reset()
+    _lib.profile_enable(args.profile_every)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e, grads = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _lib.profile_enable(0)
+    prof = _lib.profile_read()
+    sec = (t1 - t0) / args.steps
+    e_val = float(e.item())
+
+    gemm = [prof[k] for k in ('gemm_f64_wbuild', 'gemm_f64_adjoint') if prof[k]['launches']]
+    g_ms = sum(x['ms'] for x in gemm)
+    g_fl = sum(x['flops'] for x in gemm)
+    g_n = sum(x['launches'] for x in gemm)
+    achieved = g_fl / (g_ms * 1e-3) / 1e12 if g_ms > 0 else None
+    peak = PEAKS['fp64']
+    roofline = {'bound': 'mfma', 'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s',
+                'frac': (achieved / peak) if achieved else None, 'traffic': None,
+                'kernel': 'k_gemm_f64', 'launches_timed': g_n,
+                'avg_launch_ms': g_ms / g_n if g_n else None,
+                'flops_per_launch': g_fl / g_n if g_n else None}
+
+    F = alg_flops(o, v) if not args.energy_only else alg_flops(o, v) / 3.0
+    cpu = None
+    if not args.no_cpu_baseline:
+        so, sv = (int(x) for x in args.cpu_sample.split(','))
+        cpu = cpu_baseline(o, v, (so, sv), args.cpu_threads)
+        if cpu is not None and args.energy_only:
+            cpu['value'] /= 3.0
+    line = {
+        'metric': 'E(T)+VJP sec @o40/v400: FP64 MFMA vs split-BF16x3, |dE|<1e-7 Eh',
+        'value': sec,
+        'unit': 's',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': sec * 1e3,
+        'higher_is_better': False,
+        'scaling': 'strong',
+        'vs_baseline': None,
+        'dtype': 'f64' if args.precision == 'fp64' else args.precision,
+        'data': 'synthetic (seeded DF-like generator; see pyscfad_amd/gen.py)',
+        'config': {'workload': f'RHF (T) E+VJP nocc={o} nvir={v} naux={naux} seed={args.seed}'
+                   + (' (energy only)' if args.energy_only else ''),
+                   'precision': args.precision, 'ovvv': 'packed'},
+        'e_t': e_val,
+        'tflops_alg': F / sec / 1e12,
+        'frac_fp64_peak_alg': F / sec / 1e12 / PEAKS['fp64'],
+        'input_gen_s': t_gen,
+        'roofline': roofline,
+        'cpu_baseline': cpu,
+        'kernels': {k: dict(x, avg_ms=(x['ms'] / x['launches'] if x['launches'] else None))
+                    for k, x in prof.items()},
+    }
+    s = json.dumps(line)
+    print(s, flush=True)
+    if args.out:
+        with open(args.out, 'w') as f:
+            f.write(s + '\n')
+
+
+if __name__ == '__main__':
+    main()

@@ -23,7 +23,9 @@ PRECISIONS = {'fp64': 0, 'bf16x3': 1, 'fp32x2': 2}
 
 # exported symbols of include/pyscfad_amd_ccsdt.h and include/pyscfad_amd_mlp.h
 EXPORTS = ('pamd_version', 'pamd_ccsdt_workspace_size', 'pamd_ccsdt_energy',
-           'pamd_ccsdt_energy_vjp', 'pamd_ccsdt_pair_energy')
+           'pamd_ccsdt_energy_vjp', 'pamd_ccsdt_pair_energy', 'pamd_profile_enable',
+           'pamd_profile_reset', 'pamd_profile_num_kinds', 'pamd_profile_kind_name',
+           'pamd_profile_read', 'pamd_debug_gemm_descriptors', 'pamd_debug_split_factors')
 
 
 class Config(ctypes.Structure):
@@ -56,6 +58,17 @@ def load():
     lib.pamd_ccsdt_pair_energy.argtypes = [cfgp] + [vp] * 7 + [vp, sz, vp, vp]
     lib.pamd_ccsdt_energy_vjp.restype = i
     lib.pamd_ccsdt_energy_vjp.argtypes = [cfgp] + [vp] * 7 + [d, vp, sz] + [vp] * 8 + [vp]
+    lib.pamd_profile_enable.argtypes = [i]
+    lib.pamd_profile_enable.restype = i
+    lib.pamd_profile_reset.argtypes = []
+    lib.pamd_profile_reset.restype = i
+    lib.pamd_profile_num_kinds.argtypes = []
+    lib.pamd_profile_num_kinds.restype = i
+    lib.pamd_profile_kind_name.argtypes = [i]
+    lib.pamd_profile_kind_name.restype = ctypes.c_char_p
+    lib.pamd_profile_read.argtypes = [i, ctypes.POINTER(ctypes.c_longlong)] + \
+        [ctypes.POINTER(ctypes.c_double)] * 3
+    lib.pamd_profile_read.restype = i
     _lib = lib
     return lib
 
@@ -81,3 +94,26 @@ def workspace_size(cfg, with_vjp):
     check(lib.pamd_ccsdt_workspace_size(ctypes.byref(cfg), int(bool(with_vjp)), ctypes.byref(out)),
           'pamd_ccsdt_workspace_size')
     return out.value
+
+
+def profile_enable(every):
+    """Time the launches of one occupied triple in ``every`` (0 disables)."""
+    check(load().pamd_profile_enable(int(every)), 'pamd_profile_enable')
+
+
+def profile_reset():
+    check(load().pamd_profile_reset(), 'pamd_profile_reset')
+
+
+def profile_read():
+    """{kernel: dict(launches, ms, flops, bytes)} accumulated since the last reset."""
+    lib = load()
+    out = {}
+    for kind in range(lib.pamd_profile_num_kinds()):
+        n = ctypes.c_longlong()
+        ms, fl, by = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(lib.pamd_profile_read(kind, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(fl),
+                                    ctypes.byref(by)), 'pamd_profile_read')
+        out[lib.pamd_profile_kind_name(kind).decode()] = dict(
+            launches=n.value, ms=ms.value, flops=fl.value, bytes=by.value)
+    return out

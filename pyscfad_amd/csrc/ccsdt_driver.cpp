@@ -34,12 +34,89 @@ static const int kGmax = 1024;
 // kernels are skipped (host-only descriptor inspection, no GPU work).
 static std::vector<GemmDesc> *g_record = nullptr;
 
-static hipError_t run_gemm(const GemmDesc &d, hipStream_t s) {
+// Sampled launch timing (pamd_profile_*): for one triple in g_prof_every, every
+// launch is bracketed by hipEvents on its own stream; the events are read after
+// the call has been queued completely.
+enum ProfKind { kProfWBuild = 0, kProfAdjGemm, kProfOrbit, kProfVR, kProfCombine, kProfNum };
+static const char *kProfNames[kProfNum] = {"gemm_f64_wbuild", "gemm_f64_adjoint", "orbit_epilogue",
+                                           "vbar_reduce", "combine"};
+struct ProfRec {
+  int kind;
+  hipEvent_t a, b;
+  double flops, bytes;
+};
+struct ProfStat {
+  long long launches;
+  double ms, flops, bytes;
+};
+static int g_prof_every = 0;
+static bool g_prof_on = false;  // sampling the current triple
+static std::vector<ProfRec> g_prof_recs;
+static ProfStat g_prof_stat[kProfNum];
+
+static hipError_t prof_begin(hipStream_t s, ProfRec &r, int kind, double flops, double bytes) {
+  r.kind = kind;
+  r.flops = flops;
+  r.bytes = bytes;
+  hipError_t e = hipEventCreate(&r.a);
+  if (e != hipSuccess) return e;
+  e = hipEventCreate(&r.b);
+  if (e != hipSuccess) return e;
+  return hipEventRecord(r.a, s);
+}
+
+static hipError_t prof_end(hipStream_t s, ProfRec &r) {
+  hipError_t e = hipEventRecord(r.b, s);
+  g_prof_recs.push_back(r);
+  return e;
+}
+
+static hipError_t prof_collect() {
+  hipError_t first = hipSuccess;
+  for (ProfRec &r : g_prof_recs) {
+    float ms = 0.f;
+    hipError_t e = hipEventSynchronize(r.b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, r.a, r.b);
+    if (e == hipSuccess) {
+      ProfStat &st = g_prof_stat[r.kind];
+      st.launches += 1;
+      st.ms += ms;
+      st.flops += r.flops;
+      st.bytes += r.bytes;
+    } else if (first == hipSuccess) {
+      first = e;
+    }
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
+  g_prof_recs.clear();
+  return first;
+}
+
+static double gemm_flops(const GemmDesc &d) {
+  double k = 0;
+  for (int q = 0; q < d.nseg; ++q) k += d.seg[q].L;
+  return 2.0 * d.M * (double)d.N * k;
+}
+
+// timed launch wrapper for any kernel
+template <class F>
+static hipError_t timed(hipStream_t s, int kind, double flops, double bytes, F &&launch) {
+  if (!g_prof_on) return launch();
+  ProfRec r;
+  hipError_t e = prof_begin(s, r, kind, flops, bytes);
+  if (e != hipSuccess) return e;
+  e = launch();
+  if (e != hipSuccess) return e;
+  return prof_end(s, r);
+}
+
+static hipError_t run_gemm(const GemmDesc &d, hipStream_t s, int kind) {
   if (g_record) {
     g_record->push_back(d);
     return hipSuccess;
   }
-  return launch_gemm_f64(d, s);
+  return timed(s, kind, gemm_flops(d), 0.0, [&] { return launch_gemm_f64(d, s); });
 }
 
 static OpDesc op(const double *p, long long rx, long long ry, int rdiv, long long kx, long long ky,
@@ -205,7 +282,7 @@ static int build_w(const Ctx &c, int i, int j, int k) {
                        seg(opR(T2m(j), o, vo, V, 1, 0), op1(OOt(k, i), o, 1, 0), O, -1.0),   // H5
                        seg(opR(T2m(i), vo, o, V, 1, 0), op1(OOt(k, j), o, 1, 0), O, -1.0)},  // H6
                       out(c.W, v2, v, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s));
+    PAMD_CHECK(run_gemm(d, c.s, kProfWBuild));
   }
   // pass B: rows b, cols (a,c)
   {
@@ -215,7 +292,7 @@ static int build_w(const Ctx &c, int i, int j, int k) {
                        seg(op1(OOt(j, i), o, 1, 0), opR(T2m(k), o, vo, V, 1, 0), O, -1.0),   // H3
                        seg(op1(OOt(j, k), o, 1, 0), opR(T2m(i), vo, o, V, 1, 0), O, -1.0)},  // H4
                       out(c.W, 0, v, kBig, v2, 1, V));
-    PAMD_CHECK(run_gemm(d, c.s));
+    PAMD_CHECK(run_gemm(d, c.s, kProfWBuild));
   }
   // pass A: rows a, cols (b,c)
   {
@@ -225,7 +302,7 @@ static int build_w(const Ctx &c, int i, int j, int k) {
                        seg(op1(OOt(i, j), o, 1, 0), opR(T2m(k), o, vo, V, 1, 0), O, -1.0),   // H1
                        seg(op1(OOt(i, k), o, 1, 0), opR(T2m(j), vo, o, V, 1, 0), O, -1.0)},  // H2
                       out(c.W, 0, v2, kBig, v, 1, V));
-    PAMD_CHECK(run_gemm(d, c.s));
+    PAMD_CHECK(run_gemm(d, c.s, kProfWBuild));
   }
   return PAMD_OK;
 }
@@ -259,7 +336,10 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
   vr.k = k;
   vr.has_fvo = c.has_fvo;
   vr.R = c.R;
-  if (!g_record) PAMD_CHECK(launch_vr(vr, c.s));
+  if (!g_record) {
+    const double vb = 2.0 * 8.0 * (double)v3;  // Vbar read twice
+    PAMD_CHECK(timed(c.s, kProfVR, 0.0, vb, [&] { return launch_vr(vr, c.s); }));
+  }
 
   // ovvv_bar, rows = virtual pairs, cols = f
   {  // Vpb[i][a][b|c][f]: P1 (K=c), P2 (K=b)
@@ -267,21 +347,21 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
                       {seg(opR(Wb, v2, v, V, 1, 0), op1(T2(k, j), 1, v, 1), V, 1.0),
                        seg(opR(Wb, v2, 1, V, v, 1), op1(T2(j, k), 1, v, 1), V, 1.0)},
                       out(Vpb(i), v2, v, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s));
+    PAMD_CHECK(run_gemm(d, c.s, kProfAdjGemm));
   }
   {  // Vpb[j][b][a|c][f]: P3 (K=c), P4 (K=a)
     GemmDesc d = gemm(V * V, V, kAccumulate,
                       {seg(opR(Wb, v, v2, V, 1, 0), op1(T2(k, i), 1, v, 1), V, 1.0),
                        seg(opR(Wb, v, 1, V, v2, 1), op1(T2(i, k), 1, v, 1), V, 1.0)},
                       out(Vpb(j), v2, v, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s));
+    PAMD_CHECK(run_gemm(d, c.s, kProfAdjGemm));
   }
   {  // Vpb[k][c][a|b][f]: rows m = y*v + x with x = c: P5 (K=b), P6 (K=a)
     GemmDesc d = gemm(V * V, V, kAccumulate,
                       {seg(opR(Wb, v2, 1, V, v, 1), op1(T2(j, i), 1, v, 1), V, 1.0),
                        seg(opR(Wb, v, 1, V, v2, 1), op1(T2(i, j), 1, v, 1), V, 1.0)},
                       out(Vpb(k), v, v2, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s));
+    PAMD_CHECK(run_gemm(d, c.s, kProfAdjGemm));
   }
 
   // particle t2_bar pieces: split-K over the virtual pair, partials per problem
@@ -307,7 +387,7 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
         gemm(V, V, kPartial, {seg(opK(Wb, v2, v, 1, V, 0), opK(Vp(k), 1, v, v2, V, 1), V * V, 1.0)},
              out(nullptr, 0, 0, 1, 0, 0, 1), c.S_p, c.part_p + 5 * pp),
     };
-    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(ds[q], c.s));
+    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(ds[q], c.s, kProfAdjGemm));
   }
   // hole ovoo_bar pieces: split-K over the virtual pair
   const size_t ph = (size_t)c.S_h * v * o;
@@ -332,7 +412,7 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
         gemm(V, O, kPartial, {seg(opK(Wb, 1, v2, v, V, 1), opK(T2m(i), 1, vo, o, V, 1), V * V, -1.0)},
              out(nullptr, 0, 0, 1, 0, 0, 1), c.S_h, c.part_h + 5 * ph),
     };
-    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(ds[q], c.s));
+    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(ds[q], c.s, kProfAdjGemm));
   }
   // hole-term t2 reads: T2mb[x][p][q][m] (= dE/d t2[x,m,p,q])
   {  // T2mb[i]: H4 (p=a,q=c,K=b), H6 (p=a,q=b,K=c)
@@ -340,21 +420,21 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
                       {seg(opR(Wb, v2, 1, V, v, 1), op1(OOt(j, k), 1, o, 1), V, -1.0),
                        seg(opR(Wb, v2, v, V, 1, 0), op1(OOt(k, j), 1, o, 1), V, -1.0)},
                       out(T2mb(i), vo, o, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s));
+    PAMD_CHECK(run_gemm(d, c.s, kProfAdjGemm));
   }
   {  // T2mb[j]: H2 (p=b,q=c,K=a), H5 (p=b,q=a,K=c)
     GemmDesc d = gemm(V * V, O, kAccumulate,
                       {seg(opR(Wb, v, 1, V, v2, 1), op1(OOt(i, k), 1, o, 1), V, -1.0),
                        seg(opR(Wb, v, v2, V, 1, 0), op1(OOt(k, i), 1, o, 1), V, -1.0)},
                       out(T2mb(j), vo, o, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s));
+    PAMD_CHECK(run_gemm(d, c.s, kProfAdjGemm));
   }
   {  // T2mb[k]: rows m = q*v + p with p = c: H1 (q=b,K=a), H3 (q=a,K=b)
     GemmDesc d = gemm(V * V, O, kAccumulate,
                       {seg(opR(Wb, v, 1, V, v2, 1), op1(OOt(i, j), 1, o, 1), V, -1.0),
                        seg(opR(Wb, v2, 1, V, v, 1), op1(OOt(j, i), 1, o, 1), V, -1.0)},
                       out(T2mb(k), o, vo, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s));
+    PAMD_CHECK(run_gemm(d, c.s, kProfAdjGemm));
   }
   // gather into the cotangents
   CombineArgs ca;
@@ -374,7 +454,10 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
   ca.ovoo_bar = c.ovoo_bar;
   ca.t1_bar = c.t1_bar;
   ca.fvo_bar = c.fvo_bar;
-  if (!g_record) PAMD_CHECK(launch_combine(ca, c.s));
+  if (!g_record) {
+    const double cb = 8.0 * (6.0 * c.S_p * v2 + 6.0 * c.S_h * v * o + 12.0 * v2);
+    PAMD_CHECK(timed(c.s, kProfCombine, 0.0, cb, [&] { return launch_combine(ca, c.s); }));
+  }
   return PAMD_OK;
 }
 
@@ -396,10 +479,13 @@ static int run(Ctx &c) {
     PAMD_CHECK(launch_fill(c.ovov_bar, (size_t)o * v * o * v, 0.0, c.s));
     PAMD_CHECK(launch_fill(c.fvo_bar, (size_t)v * o, 0.0, c.s));
   }
+  long long tcount = 0;
   for (int i = 0; i < o; ++i)
     for (int j = i; j < o; ++j)
       for (int k = j; k < o; ++k) {
         if (i == j && j == k) continue;
+        g_prof_on = g_prof_every > 0 && (tcount % g_prof_every) == 0;
+        ++tcount;
         int st = build_w(c, i, j, k);
         if (st != PAMD_OK) return st;
         OrbitArgs oa;
@@ -423,13 +509,19 @@ static int run(Ctx &c) {
         oa.ebar = c.ebar;
         oa.part_t = c.part_t;
         oa.part_mo = c.part_mo;
-        PAMD_CHECK(launch_orbit(oa, c.G, c.s));
+        {
+          const double v3 = (double)v * v * v;
+          const double ob = 8.0 * v3 * (c.vjp ? 3.0 : 1.0);  // W read (+ Wbar, Vbar written)
+          PAMD_CHECK(timed(c.s, kProfOrbit, 0.0, ob, [&] { return launch_orbit(oa, c.G, c.s); }));
+        }
         PAMD_CHECK(launch_triple_energy(c.part_t, c.G, c.tri + ((size_t)i * o + j) * o + k, c.s));
         if (c.vjp) {
           st = adjoint(c, i, j, k);
           if (st != PAMD_OK) return st;
         }
       }
+  g_prof_on = false;
+  if (!g_prof_recs.empty()) PAMD_CHECK(prof_collect());
   return PAMD_OK;
 }
 
@@ -616,6 +708,32 @@ int pamd_debug_gemm_descriptors(int nocc, int nvir, int i, int j, int k, int wit
     const int n = std::min((int)rec.size(), max_desc);
     memcpy(out, rec.data(), sizeof(GemmDesc) * (size_t)n);
   }
+  return PAMD_OK;
+}
+
+int pamd_profile_enable(int every) {
+  if (every < 0) return PAMD_ERR_ARG;
+  g_prof_every = every;
+  return PAMD_OK;
+}
+
+int pamd_profile_reset(void) {
+  memset(g_prof_stat, 0, sizeof(g_prof_stat));
+  return PAMD_OK;
+}
+
+int pamd_profile_num_kinds(void) { return kProfNum; }
+
+const char *pamd_profile_kind_name(int kind) {
+  return (kind >= 0 && kind < kProfNum) ? kProfNames[kind] : "";
+}
+
+int pamd_profile_read(int kind, long long *launches, double *ms, double *flops, double *bytes) {
+  if (kind < 0 || kind >= kProfNum || !launches || !ms || !flops || !bytes) return PAMD_ERR_ARG;
+  *launches = g_prof_stat[kind].launches;
+  *ms = g_prof_stat[kind].ms;
+  *flops = g_prof_stat[kind].flops;
+  *bytes = g_prof_stat[kind].bytes;
   return PAMD_OK;
 }
 

@@ -1,8 +1,9 @@
 // HIP kernels for the closed-shell (T) energy and its adjoint on gfx950.
 //
 // * k_gemm_f64   -- the FP64 MFMA contraction engine (v_mfma_f64_16x16x4f64):
-//                   64x64 output tile per 256-thread workgroup, K staged through
-//                   double-buffered LDS in chunks of 16, 2x2 MFMA tiles per wave.
+//                   128x128 (or 128x64) output tile per 512-thread workgroup, K
+//                   staged through double-buffered LDS in chunks of 16, 4x2 (2x2)
+//                   MFMA tiles per wave, XCD-aware tile order.
 // * k_orbit      -- fused epilogue for one occupied triple: V = t1*ovov + t2*fvo
 //                   terms, Z = r3(W + V), denominators, the energy reduction and,
 //                   for the adjoint, Wbar / Vbar / dE/d(mo_energy) seeds.  Virtual
@@ -124,113 +125,244 @@ hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, cons
 
 // ---------------------------------------------------------------------------
 // FP64 MFMA contraction engine
+//
+// Workgroup tile MT x NT: 256 threads = 4 waves in a 2 x 2 grid, each wave
+// computing (MT/2) x (NT/2) as (MT/32) x (NT/32) tiles of v_mfma_f64_16x16x4f64.
+// K is staged through double-buffered LDS in chunks of 16.  Each operand of
+// each segment is staged in the LDS layout that keeps its global reads
+// coalesced and its LDS traffic free of bank conflicts:
+//   row-contiguous operand ("rfast") -> [k][r], row stride R + 16 doubles
+//   K-contiguous operand   ("kfast") -> [r][k], row stride 17 doubles
+// Offsets are 32-bit, relative to the segment's base pointer (the host folds
+// the occupied-index slab into the pointer and checks the range); row offsets
+// are computed once per segment and K offsets advance without divisions.
 // ---------------------------------------------------------------------------
-constexpr int GMT = 64, GNT = 64, GKC = 16, GLD = 80;  // LDS row stride 80 doubles (640 B)
+constexpr int GKC = 16;
 
-__device__ __forceinline__ long long op_roff(const OpDesc &d, int r) {
-  int q = r / d.rdiv;
-  int rem = r - q * d.rdiv;
-  return (long long)q * d.rx + (long long)rem * d.ry;
-}
+template <int R, int NTHR>
+struct OpStage {
+  static constexpr int E = R * GKC / NTHR;  // staged elements per thread
+  static constexpr int RSTEP = NTHR / R;    // rfast: k step between a thread's elements
+  static constexpr int KSTEP = NTHR / 16;   // kfast: row step between a thread's elements
+  static constexpr int LR = R + 16;         // [k][r] row stride (doubles)
+  static constexpr int LK = GKC + 1;        // [r][k] row stride (doubles)
+  static constexpr int LDS = (GKC * LR > R * LK) ? GKC * LR : R * LK;
+  // every (row, k) of the R x 16 panel is staged exactly once by both mappings
+  static_assert((R * GKC) % NTHR == 0 && NTHR % R == 0 && NTHR % 16 == 0, "staging map");
+  static_assert(KSTEP * E == R && RSTEP * E == GKC, "staging map covers the panel");
+};
 
-__device__ __forceinline__ long long op_koff(const OpDesc &d, int kk) {
-  int q = kk / d.kdiv;
-  int rem = kk - q * d.kdiv;
-  return (long long)q * d.kx + (long long)rem * d.ky;
-}
-
-// Fetch the 4 elements this thread stages for one operand of chunk g.
-__device__ __forceinline__ void fetch_op(const OpDesc &d, int L, double alpha, int R, int r0,
-                                         int kappa0, int tid, double (&v)[4]) {
+template <int R, int NTHR>
+__device__ __forceinline__ void op_rows(const OpDesc &d, int R_total, int r0, int tid,
+                                        int (&roff)[OpStage<R, NTHR>::E],
+                                        bool (&rok)[OpStage<R, NTHR>::E]) {
+  using S = OpStage<R, NTHR>;
   if (d.rfast) {
-    const int rl = tid & 63;
-    const int r = r0 + rl;
-    const bool rok = r < R;
-    const long long ro = rok ? op_roff(d, r) : 0;
+    const int r = r0 + tid % R;
+    rok[0] = r < R_total;
+    const int q = r / d.rdiv;
+    roff[0] = rok[0] ? (int)(q * d.rx + (r - q * d.rdiv) * d.ry) : 0;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int kk = kappa0 + (tid >> 6) + 4 * e;
-      v[e] = (rok && kk < L) ? d.p[ro + op_koff(d, kk)] * alpha : 0.0;
+    for (int e = 1; e < S::E; ++e) {
+      rok[e] = false;
+      roff[e] = 0;
     }
   } else {
-    const int kk = kappa0 + (tid & 15);
-    const bool kok = kk < L;
-    const long long ko = kok ? op_koff(d, kk) : 0;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int r = r0 + (tid >> 4) + 16 * e;
-      v[e] = (kok && r < R) ? d.p[op_roff(d, r) + ko] * alpha : 0.0;
+    for (int e = 0; e < S::E; ++e) {
+      const int r = r0 + (tid >> 4) + S::KSTEP * e;
+      rok[e] = r < R_total;
+      const int q = r / d.rdiv;
+      roff[e] = rok[e] ? (int)(q * d.rx + (r - q * d.rdiv) * d.ry) : 0;
     }
   }
 }
 
-__device__ __forceinline__ void stash_op(const OpDesc &d, int tid, const double (&v)[4],
-                                         double (*S)[GLD]) {
+// K position of a chunk as (q, r) = divmod(kappa0, kdiv), advanced without division
+struct KPos {
+  int q, r;
+};
+
+__device__ __forceinline__ int k_offset(const OpDesc &d, KPos kp, int add) {
+  int r = kp.r + add, q = kp.q;
+  if (d.kdiv >= GKC) {
+    if (r >= d.kdiv) {
+      r -= d.kdiv;
+      ++q;
+    }
+  } else {
+    while (r >= d.kdiv) {
+      r -= d.kdiv;
+      ++q;
+    }
+  }
+  return (int)(q * d.kx + r * d.ky);
+}
+
+// Loads are issued unconditionally from a clamped (in-range) address; the raw
+// values stay in registers and the mask and alpha are applied when they are
+// written to LDS, so nothing waits on the loads before the MFMA phase.
+template <int R, int NTHR>
+__device__ __forceinline__ void op_fetch(const OpDesc &d, int L, int kappa0, KPos kp, int tid,
+                                         const int (&roff)[OpStage<R, NTHR>::E],
+                                         const bool (&rok)[OpStage<R, NTHR>::E],
+                                         double (&v)[OpStage<R, NTHR>::E],
+                                         bool (&ok)[OpStage<R, NTHR>::E]) {
+  using S = OpStage<R, NTHR>;
   if (d.rfast) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) S[(tid >> 6) + 4 * e][tid & 63] = v[e];
+    for (int e = 0; e < S::E; ++e) {
+      const int kl = tid / R + S::RSTEP * e;
+      ok[e] = rok[0] && (kappa0 + kl) < L;
+      const int off = ok[e] ? roff[0] + k_offset(d, kp, kl) : 0;
+      v[e] = d.p[off];
+    }
   } else {
+    const int kl = tid & 15;
+    const bool kok = (kappa0 + kl) < L;
+    const int ko = kok ? k_offset(d, kp, kl) : 0;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) S[tid & 15][(tid >> 4) + 16 * e] = v[e];
+    for (int e = 0; e < S::E; ++e) {
+      ok[e] = kok && rok[e];
+      v[e] = d.p[ok[e] ? roff[e] + ko : 0];
+    }
   }
 }
 
-__device__ __forceinline__ int seg_of_chunk(const GemmDesc &d, int g) {
-  int s = 0;
+template <int R, int NTHR>
+__device__ __forceinline__ void op_stash(int rfast, double alpha, int tid,
+                                         const double (&v)[OpStage<R, NTHR>::E],
+                                         const bool (&ok)[OpStage<R, NTHR>::E], double *Sm) {
+  using S = OpStage<R, NTHR>;
+  if (rfast) {
+    const int r = tid % R;
 #pragma unroll
-  for (int q = 1; q < kMaxSeg; ++q)
-    if (q < d.nseg && g >= d.chunk_start[q]) s = q;
-  return s;
+    for (int e = 0; e < S::E; ++e)
+      Sm[(tid / R + S::RSTEP * e) * S::LR + r] = ok[e] ? v[e] * alpha : 0.0;
+  } else {
+    const int k = tid & 15;
+#pragma unroll
+    for (int e = 0; e < S::E; ++e)
+      Sm[((tid >> 4) + S::KSTEP * e) * S::LK + k] = ok[e] ? v[e] * alpha : 0.0;
+  }
 }
 
-__global__ __launch_bounds__(256, 2) void k_gemm_f64(GemmDesc d) {
-  __shared__ double As[2][GKC][GLD];
-  __shared__ double Bs[2][GKC][GLD];
+// Workgroup w runs on XCD w % 8; give each XCD a contiguous range of logical
+// tiles so that tiles sharing an operand panel share that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int w, int n) {
+  const int q = n >> 3, r = n & 7, x = w & 7, k = w >> 3;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+// Tile MT x NT with WM x WN waves; each wave (MT/WM) x (NT/WN).  The grid is
+// 1-D over (split, tile) with the N tiles innermost when n_fast (the A panel
+// is then read once and reused from L2), else the M tiles innermost.
+template <int MT, int NT, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, int n_fast) {
+  constexpr int NTHR = 64 * WM * WN;
+  using SA = OpStage<MT, NTHR>;
+  using SB = OpStage<NT, NTHR>;
+  constexpr int WTM = MT / WM, WTN = NT / WN;
+  constexpr int TM = WTM / 16, TN = WTN / 16;
+  __shared__ double As[2][SA::LDS];
+  __shared__ double Bs[2][SB::LDS];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const int m0 = blockIdx.x * GMT, n0 = blockIdx.y * GNT;
+  const int wm = wave / WN, wn = wave % WN;
+  const int mt = (d.M + MT - 1) / MT, nt = (d.N + NT - 1) / NT;
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const int split = logical / (mt * nt);
+  const int tile = logical - split * (mt * nt);
+  const int tm = n_fast ? tile / nt : tile % mt;
+  const int tn = n_fast ? tile % nt : tile / mt;
+  const int m0 = tm * MT, n0 = tn * NT;
   const int total = d.chunk_start[d.nseg];
   const int cps = (total + d.splits - 1) / d.splits;
-  const int g0 = blockIdx.z * cps;
+  const int g0 = split * cps;
   const int g1 = min(total, g0 + cps);
 
-  d4 acc[2][2];
+  d4 acc[TM][TN];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < TM; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < TN; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 
-  double ra[4], rb[4];
-  int buf = 0;
+  // fetch state: the segment of the next chunk to load, held in registers
+  int fs = 0;
+  while (fs + 1 < d.nseg && g0 >= d.chunk_start[fs + 1]) ++fs;
+  OpDesc FA = d.seg[fs].A, FB = d.seg[fs].B;
+  int FL = d.seg[fs].L, Fend = d.chunk_start[fs + 1];
+  double Falpha = d.seg[fs].alpha;
+  int kappa = (g0 - d.chunk_start[fs]) * GKC;
+  KPos ka{kappa / FA.kdiv, kappa - (kappa / FA.kdiv) * FA.kdiv};
+  KPos kb{kappa / FB.kdiv, kappa - (kappa / FB.kdiv) * FB.kdiv};
+  int ra_off[SA::E], rb_off[SB::E];
+  bool ra_ok[SA::E], rb_ok[SB::E];
+  double va[SA::E], vb[SB::E];
+  bool oka[SA::E], okb[SB::E];
+  op_rows<MT, NTHR>(FA, d.M, m0, tid, ra_off, ra_ok);
+  op_rows<NT, NTHR>(FB, d.N, n0, tid, rb_off, rb_ok);
   if (g0 < g1) {
-    const int s = seg_of_chunk(d, g0);
-    const int k0 = (g0 - d.chunk_start[s]) * GKC;
-    fetch_op(d.seg[s].A, d.seg[s].L, d.seg[s].alpha, d.M, m0, k0, tid, ra);
-    fetch_op(d.seg[s].B, d.seg[s].L, 1.0, d.N, n0, k0, tid, rb);
+    op_fetch<MT, NTHR>(FA, FL, kappa, ka, tid, ra_off, ra_ok, va, oka);
+    op_fetch<NT, NTHR>(FB, FL, kappa, kb, tid, rb_off, rb_ok, vb, okb);
   }
+  int la = FA.rfast, lb = FB.rfast;  // LDS layouts of the chunk held in va/vb
+  double alpha = Falpha;             // and its segment's scale
+  int buf = 0;
   for (int g = g0; g < g1; ++g) {
-    const int s = seg_of_chunk(d, g);
-    stash_op(d.seg[s].A, tid, ra, As[buf]);
-    stash_op(d.seg[s].B, tid, rb, Bs[buf]);
+    op_stash<MT, NTHR>(la, alpha, tid, va, oka, As[buf]);
+    op_stash<NT, NTHR>(lb, 1.0, tid, vb, okb, Bs[buf]);
     __syncthreads();
+    const int cla = la, clb = lb;
     if (g + 1 < g1) {
-      const int s1 = seg_of_chunk(d, g + 1);
-      const int k1 = (g + 1 - d.chunk_start[s1]) * GKC;
-      fetch_op(d.seg[s1].A, d.seg[s1].L, d.seg[s1].alpha, d.M, m0, k1, tid, ra);
-      fetch_op(d.seg[s1].B, d.seg[s1].L, 1.0, d.N, n0, k1, tid, rb);
+      if (g + 1 >= Fend) {  // next segment: reload its descriptors once
+        ++fs;
+        FA = d.seg[fs].A;
+        FB = d.seg[fs].B;
+        FL = d.seg[fs].L;
+        Fend = d.chunk_start[fs + 1];
+        Falpha = d.seg[fs].alpha;
+        kappa = 0;
+        ka = KPos{0, 0};
+        kb = KPos{0, 0};
+        op_rows<MT, NTHR>(FA, d.M, m0, tid, ra_off, ra_ok);
+        op_rows<NT, NTHR>(FB, d.N, n0, tid, rb_off, rb_ok);
+      } else {
+        kappa += GKC;
+        ka.r += GKC;
+        while (ka.r >= FA.kdiv) {
+          ka.r -= FA.kdiv;
+          ++ka.q;
+        }
+        kb.r += GKC;
+        while (kb.r >= FB.kdiv) {
+          kb.r -= FB.kdiv;
+          ++kb.q;
+        }
+      }
+      op_fetch<MT, NTHR>(FA, FL, kappa, ka, tid, ra_off, ra_ok, va, oka);
+      op_fetch<NT, NTHR>(FB, FL, kappa, kb, tid, rb_off, rb_ok, vb, okb);
+      la = FA.rfast;
+      lb = FB.rfast;
+      alpha = Falpha;
     }
+    const int ska = cla ? SA::LR : 1, sra = cla ? 1 : SA::LK;
+    const int skb = clb ? SB::LR : 1, srb = clb ? 1 : SB::LK;
+    const double *Ab = As[buf];
+    const double *Bb = Bs[buf];
 #pragma unroll
     for (int ks = 0; ks < GKC / 4; ++ks) {
       const int kr = ks * 4 + (lane >> 4);
-      const double a0 = As[buf][kr][wm * 32 + (lane & 15)];
-      const double a1 = As[buf][kr][wm * 32 + 16 + (lane & 15)];
-      const double b0 = Bs[buf][kr][wn * 32 + (lane & 15)];
-      const double b1 = Bs[buf][kr][wn * 32 + 16 + (lane & 15)];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      double a[TM], b[TN];
+#pragma unroll
+      for (int t = 0; t < TM; ++t) a[t] = Ab[kr * ska + (wm * WTM + t * 16 + (lane & 15)) * sra];
+#pragma unroll
+      for (int t = 0; t < TN; ++t) b[t] = Bb[kr * skb + (wn * WTN + t * 16 + (lane & 15)) * srb];
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TN; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
     }
     buf ^= 1;
   }
@@ -238,17 +370,17 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(GemmDesc d) {
   // D layout of v_mfma_f64_16x16x4f64 on gfx950: lane l, element r holds
   // row 4*r + l/16, column l%16 of the 16x16 tile.
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int x = 0; x < TM; ++x)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int y = 0; y < TN; ++y)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm * 32 + a * 16 + 4 * r + (lane >> 4);
-        const int n = n0 + wn * 32 + b * 16 + (lane & 15);
+        const int m = m0 + wm * WTM + x * 16 + 4 * r + (lane >> 4);
+        const int n = n0 + wn * WTN + y * 16 + (lane & 15);
         if (m >= d.M || n >= d.N) continue;
-        const double val = acc[a][b][r];
+        const double val = acc[x][y][r];
         if (d.mode == kPartial) {
-          d.partial[((size_t)blockIdx.z * d.M + m) * d.N + n] = val;
+          d.partial[((size_t)split * d.M + m) * d.N + n] = val;
         } else {
           const int qm = m / d.C.mdiv, rm = m - qm * d.C.mdiv;
           const int qn = n / d.C.ndiv, rn = n - qn * d.C.ndiv;
@@ -264,8 +396,17 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(GemmDesc d) {
 
 hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s) {
   if (d.M <= 0 || d.N <= 0) return hipSuccess;
-  dim3 grid((d.M + GMT - 1) / GMT, (d.N + GNT - 1) / GNT, d.splits);
-  hipLaunchKernelGGL(k_gemm_f64, grid, dim3(256), 0, s, d);
+  // the larger operand panel is the one to read once: iterate the other index fastest
+  const int n_fast = d.M >= d.N ? 1 : 0;
+  if (d.N <= 64) {
+    const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 63) / 64) * d.splits;
+    hipLaunchKernelGGL((k_gemm_f64<128, 64, 4, 2>), dim3((unsigned)tiles), dim3(512), 0, s, d,
+                       n_fast);
+  } else {
+    const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 127) / 128) * d.splits;
+    hipLaunchKernelGGL((k_gemm_f64<128, 128, 2, 4>), dim3((unsigned)tiles), dim3(512), 0, s, d,
+                       n_fast);
+  }
   return hipGetLastError();
 }
 

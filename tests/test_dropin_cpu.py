@@ -1,0 +1,15 @@
+"""Host-side behaviour of the drop-in call: no CPU fallback exists."""
+import pytest
+import torch
+
+from pyscfad_amd.cc import ccsd_t
+from pyscfad_amd.cc.eris import SimpleCCSD, eris_from_problem
+from pyscfad_amd.gen import make_problem
+
+
+@pytest.mark.skipif(torch.cuda.is_available(), reason='checks the no-GPU error path')
+def test_kernel_fails_loudly_without_gpu():
+    p = make_problem(2, 6, 24, seed=0)
+    mycc = SimpleCCSD(t1=p.t1, t2=p.t2, verbose=0)
+    with pytest.raises(RuntimeError, match='no GPU'):
+        ccsd_t.kernel(mycc, eris_from_problem(p))
