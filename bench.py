@@ -143,8 +143,12 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         so, sv = (int(x) for x in args.cpu_sample.split(','))
-        cpu = cpu_baseline(o, v, (so, sv), args.cpu_threads)
-        if cpu is not None and args.energy_only:
+        try:
+            cpu = cpu_baseline(o, v, (so, sv), args.cpu_threads)
+        except Exception as exc:  # the checker library may be absent on a host
+            cpu = {'value': None, 'unit': 's', 'cores': args.cpu_threads, 'kind': 'reference',
+                   'sample': f'not run: {type(exc).__name__}: {exc}'}
+        if cpu is not None and args.energy_only and cpu['value']:
             cpu['value'] /= 3.0
     line = {
         'metric': 'E(T)+VJP sec @o40/v400: FP64 MFMA vs split-BF16x3, |dE|<1e-7 Eh',

@@ -91,6 +91,7 @@ hipError_t launch_fill(double *p, size_t n, double val, hipStream_t s);
 hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, const double *ovoo,
                           int o, int v, double *Vp, double *T2m, double *OOt, hipStream_t s);
 hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s);
+int gemm_tiles(int M, int N);  // workgroup tiles of one split
 hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s);
 hipError_t launch_vr(const VRArgs &a, hipStream_t s);
 hipError_t launch_combine(const CombineArgs &a, hipStream_t s);

@@ -201,8 +201,9 @@ struct Layout {
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// split-K factor giving ~6 workgroups per CU (two resident per CU at a time)
 static int splits_for(int tiles, int chunks) {
-  int s = (512 + tiles - 1) / std::max(1, tiles);
+  int s = (1536 + tiles - 1) / std::max(1, tiles);
   s = std::max(1, std::min(s, 64));
   return std::min(s, std::max(1, chunks));
 }
@@ -224,8 +225,8 @@ static Layout make_layout(int o, int v, int vjp) {
   L.off_W = take(v3);
   L.off_tri = take((size_t)o * o * o);
   L.off_part_t = take(kGmax);
-  const int tiles_p = ((v + 63) / 64) * ((v + 63) / 64);
-  const int tiles_h = ((v + 63) / 64) * ((o + 63) / 64);
+  const int tiles_p = gemm_tiles(v, v);
+  const int tiles_h = gemm_tiles(v, o);
   const int chunks = (int)((v2 + KC - 1) / KC);
   L.S_p = splits_for(tiles_p, chunks);
   L.S_h = splits_for(tiles_h, chunks);

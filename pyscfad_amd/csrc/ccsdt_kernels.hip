@@ -18,6 +18,7 @@
 namespace pamd {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
 
 // ---------------------------------------------------------------------------
 // generic helpers
@@ -145,7 +146,7 @@ struct OpStage {
   static constexpr int RSTEP = NTHR / R;    // rfast: k step between a thread's elements
   static constexpr int KSTEP = NTHR / 16;   // kfast: row step between a thread's elements
   static constexpr int LR = R + 16;         // [k][r] row stride (doubles)
-  static constexpr int LK = GKC + 1;        // [r][k] row stride (doubles)
+  static constexpr int LK = GKC + 2;        // [r][k] row stride (doubles; 16-B aligned rows)
   static constexpr int LDS = (GKC * LR > R * LK) ? GKC * LR : R * LK;
   // every (row, k) of the R x 16 panel is staged exactly once by both mappings
   static_assert((R * GKC) % NTHR == 0 && NTHR % R == 0 && NTHR % 16 == 0, "staging map");
@@ -348,21 +349,52 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, int n_fas
     }
     const int ska = cla ? SA::LR : 1, sra = cla ? 1 : SA::LK;
     const int skb = clb ? SB::LR : 1, srb = clb ? 1 : SB::LK;
-    const double *Ab = As[buf];
-    const double *Bb = Bs[buf];
+    // Within a chunk the K index is relabelled so that lane group g = lane/16
+    // uses LDS k positions 4g .. 4g+3 for its four MFMA steps (A and B agree,
+    // so the contraction is unchanged): a K-contiguous ([r][k]) panel then
+    // gives a lane its four steps with two 16-byte reads.
+    const int g4 = (lane >> 4) * 4;
+    const double *Ab = As[buf] + (wm * WTM + (lane & 15)) * sra + g4 * ska;
+    const double *Bb = Bs[buf] + (wn * WTN + (lane & 15)) * srb + g4 * skb;
 #pragma unroll
-    for (int ks = 0; ks < GKC / 4; ++ks) {
-      const int kr = ks * 4 + (lane >> 4);
-      double a[TM], b[TN];
+    for (int h = 0; h < 2; ++h) {
+      double fa[2][TM], fb[2][TN];
+      if (cla) {
 #pragma unroll
-      for (int t = 0; t < TM; ++t) a[t] = Ab[kr * ska + (wm * WTM + t * 16 + (lane & 15)) * sra];
+        for (int t = 0; t < TM; ++t) {
+          fa[0][t] = Ab[(2 * h) * ska + t * 16];
+          fa[1][t] = Ab[(2 * h + 1) * ska + t * 16];
+        }
+      } else {
 #pragma unroll
-      for (int t = 0; t < TN; ++t) b[t] = Bb[kr * skb + (wn * WTN + t * 16 + (lane & 15)) * srb];
+        for (int t = 0; t < TM; ++t) {
+          const d2 x = *(const d2 *)(Ab + t * 16 * SA::LK + 2 * h);
+          fa[0][t] = x[0];
+          fa[1][t] = x[1];
+        }
+      }
+      if (clb) {
 #pragma unroll
-      for (int x = 0; x < TM; ++x)
+        for (int t = 0; t < TN; ++t) {
+          fb[0][t] = Bb[(2 * h) * skb + t * 16];
+          fb[1][t] = Bb[(2 * h + 1) * skb + t * 16];
+        }
+      } else {
 #pragma unroll
-        for (int y = 0; y < TN; ++y)
-          acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
+        for (int t = 0; t < TN; ++t) {
+          const d2 x = *(const d2 *)(Bb + t * 16 * SB::LK + 2 * h);
+          fb[0][t] = x[0];
+          fb[1][t] = x[1];
+        }
+      }
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+#pragma unroll
+          for (int y = 0; y < TN; ++y)
+            acc[x][y] =
+                __builtin_amdgcn_mfma_f64_16x16x4f64(fa[ks][x], fb[ks][y], acc[x][y], 0, 0, 0);
     }
     buf ^= 1;
   }
@@ -396,18 +428,20 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, int n_fas
 
 hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s) {
   if (d.M <= 0 || d.N <= 0) return hipSuccess;
-  // the larger operand panel is the one to read once: iterate the other index fastest
-  const int n_fast = d.M >= d.N ? 1 : 0;
-  if (d.N <= 64) {
+  // 8 waves, two workgroups per CU; the wide side of the tile goes on the
+  // larger output dimension, whose operand panel is then read once (n_fast).
+  if (d.M >= d.N) {
     const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 63) / 64) * d.splits;
-    hipLaunchKernelGGL((k_gemm_f64<128, 64, 4, 2>), dim3((unsigned)tiles), dim3(512), 0, s, d,
-                       n_fast);
+    hipLaunchKernelGGL((k_gemm_f64<128, 64, 4, 2>), dim3((unsigned)tiles), dim3(512), 0, s, d, 1);
   } else {
-    const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 127) / 128) * d.splits;
-    hipLaunchKernelGGL((k_gemm_f64<128, 128, 2, 4>), dim3((unsigned)tiles), dim3(512), 0, s, d,
-                       n_fast);
+    const long long tiles = (long long)((d.M + 63) / 64) * ((d.N + 127) / 128) * d.splits;
+    hipLaunchKernelGGL((k_gemm_f64<64, 128, 2, 4>), dim3((unsigned)tiles), dim3(512), 0, s, d, 0);
   }
   return hipGetLastError();
+}
+
+int gemm_tiles(int M, int N) {
+  return M >= N ? ((M + 127) / 128) * ((N + 63) / 64) : ((M + 63) / 64) * ((N + 127) / 128);
 }
 
 // ---------------------------------------------------------------------------
