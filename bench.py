@@ -64,6 +64,8 @@ def main():
     ap.add_argument('--naux', type=int, default=None)
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--profile-every', type=int, default=64)
+    ap.add_argument('--progress-every', type=int, default=512,
+                    help='stderr progress line every N occupied triples (0: off)')
     ap.add_argument('--cpu-sample', default='16,160', help='nocc,nvir of the CPU baseline sample')
     ap.add_argument('--cpu-threads', type=int, default=16)
     ap.add_argument('--no-cpu-baseline', action='store_true')
@@ -116,6 +118,8 @@ def main():
 
     _lib.profile_reset()
     _lib.profile_enable(args.profile_every)
+    # a progress line every ~15 s of work keeps long single-call steps visibly alive
+    _lib.profile_progress(args.progress_every)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -123,6 +127,7 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     _lib.profile_enable(0)
+    _lib.profile_progress(0)
     prof = _lib.profile_read()
     sec = (t1 - t0) / args.steps
     e_val = float(e.item())

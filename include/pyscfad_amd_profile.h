@@ -13,6 +13,9 @@ extern "C" {
 #endif
 
 int pamd_profile_enable(int every);   /* 0 disables */
+/* Progress line on stderr every `every` occupied triples (0 disables); each
+ * report first synchronises the stream, so it reflects finished GPU work. */
+int pamd_profile_progress(int every);
 int pamd_profile_reset(void);
 int pamd_profile_num_kinds(void);
 const char *pamd_profile_kind_name(int kind);

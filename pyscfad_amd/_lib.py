@@ -24,8 +24,10 @@ PRECISIONS = {'fp64': 0, 'bf16x3': 1, 'fp32x2': 2}
 # exported symbols of include/pyscfad_amd_ccsdt.h and include/pyscfad_amd_mlp.h
 EXPORTS = ('pamd_version', 'pamd_ccsdt_workspace_size', 'pamd_ccsdt_energy',
            'pamd_ccsdt_energy_vjp', 'pamd_ccsdt_pair_energy', 'pamd_profile_enable',
-           'pamd_profile_reset', 'pamd_profile_num_kinds', 'pamd_profile_kind_name',
-           'pamd_profile_read', 'pamd_debug_gemm_descriptors', 'pamd_debug_split_factors')
+           'pamd_profile_reset', 'pamd_profile_progress', 'pamd_profile_num_kinds', 'pamd_profile_kind_name',
+           'pamd_profile_read', 'pamd_debug_gemm_descriptors', 'pamd_debug_split_factors',
+           'pamd_bf16_gemm_tn', 'pamd_bf16_act_fwd', 'pamd_bf16_act_bwd', 'pamd_adam_step',
+           'pamd_mse', 'pamd_f32_to_bf16')
 
 
 class Config(ctypes.Structure):
@@ -60,6 +62,8 @@ def load():
     lib.pamd_ccsdt_energy_vjp.argtypes = [cfgp] + [vp] * 7 + [d, vp, sz] + [vp] * 8 + [vp]
     lib.pamd_profile_enable.argtypes = [i]
     lib.pamd_profile_enable.restype = i
+    lib.pamd_profile_progress.argtypes = [i]
+    lib.pamd_profile_progress.restype = i
     lib.pamd_profile_reset.argtypes = []
     lib.pamd_profile_reset.restype = i
     lib.pamd_profile_num_kinds.argtypes = []
@@ -99,6 +103,11 @@ def workspace_size(cfg, with_vjp):
 def profile_enable(every):
     """Time the launches of one occupied triple in ``every`` (0 disables)."""
     check(load().pamd_profile_enable(int(every)), 'pamd_profile_enable')
+
+
+def profile_progress(every):
+    """Print a progress line (stderr) every ``every`` occupied triples (0 disables)."""
+    check(load().pamd_profile_progress(int(every)), 'pamd_profile_progress')
 
 
 def profile_reset():

@@ -1,0 +1,32 @@
+// FP64 -> three bf16 pieces, packed in one 8-byte element:
+// bits 0-15 hi = bf16(x), 16-31 mid = bf16(x - hi), 32-47 lo = bf16(x - hi - mid).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pamd {
+
+// round-to-nearest-even FP32 -> bf16 bits
+__device__ __forceinline__ unsigned int bf16_bits(float f) {
+  unsigned int u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+__device__ __forceinline__ double bf16_value(unsigned int b) {
+  return (double)__uint_as_float(b << 16);
+}
+
+__device__ __forceinline__ unsigned long long split3(double x) {
+  const unsigned int h = bf16_bits((float)x);
+  const double r1 = x - bf16_value(h);
+  const unsigned int m = bf16_bits((float)r1);
+  const double r2 = r1 - bf16_value(m);
+  const unsigned int l = bf16_bits((float)r2);
+  return (unsigned long long)h | ((unsigned long long)m << 16) | ((unsigned long long)l << 32);
+}
+
+// store policy of the layout kernels: FP64 copy or packed split
+__device__ __forceinline__ void put(double *p, size_t t, double x) { p[t] = x; }
+__device__ __forceinline__ void put(unsigned long long *p, size_t t, double x) { p[t] = split3(x); }
+
+}  // namespace pamd

@@ -89,7 +89,10 @@ struct CombineArgs {
 // host-side launchers (ccsdt_kernels.hip)
 hipError_t launch_fill(double *p, size_t n, double val, hipStream_t s);
 hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, const double *ovoo,
-                          int o, int v, double *Vp, double *T2m, double *OOt, hipStream_t s);
+                          int o, int v, double *Vp, double *T2m, double *OOt, int split_bf16,
+                          hipStream_t s);
+hipError_t launch_split_bf16x3(const double *x, size_t n, unsigned long long *out, hipStream_t s);
+hipError_t launch_gemm_bf16x3(const GemmDesc &d, hipStream_t s);
 hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s);
 int gemm_tiles(int M, int N);  // workgroup tiles of one split
 hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s);

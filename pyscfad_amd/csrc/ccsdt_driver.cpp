@@ -22,6 +22,7 @@
 #include <cstring>
 #include <algorithm>
 #include <vector>
+#include <chrono>
 #include "ccsdt_common.h"
 #include "../../include/pyscfad_amd_ccsdt.h"
 
@@ -50,6 +51,7 @@ struct ProfStat {
   double ms, flops, bytes;
 };
 static int g_prof_every = 0;
+static int g_progress_every = 0;  // report (to stderr) every N triples, after a stream sync
 static bool g_prof_on = false;  // sampling the current triple
 static std::vector<ProfRec> g_prof_recs;
 static ProfStat g_prof_stat[kProfNum];
@@ -111,11 +113,15 @@ static hipError_t timed(hipStream_t s, int kind, double flops, double bytes, F &
   return prof_end(s, r);
 }
 
+static int g_prec = PAMD_PREC_FP64;  // precision of the GEMMs of the call in progress
+
 static hipError_t run_gemm(const GemmDesc &d, hipStream_t s, int kind) {
   if (g_record) {
     g_record->push_back(d);
     return hipSuccess;
   }
+  if (g_prec == PAMD_PREC_BF16X3)
+    return timed(s, kind, gemm_flops(d), 0.0, [&] { return launch_gemm_bf16x3(d, s); });
   return timed(s, kind, gemm_flops(d), 0.0, [&] { return launch_gemm_f64(d, s); });
 }
 
@@ -193,7 +199,7 @@ static GemmDesc gemm(int M, int N, int mode, std::initializer_list<SegDesc> segs
 }
 
 struct Layout {
-  size_t off_Vp, off_T2m, off_OOt, off_W, off_tri, off_part_t;
+  size_t off_Vp, off_T2m, off_OOt, off_W, off_tri, off_part_t, off_T2g, off_W8;
   size_t off_Vb, off_Vpb, off_T2mb, off_part_mo, off_R, off_part_p, off_part_h;
   size_t total;
   int S_p, S_h;
@@ -208,7 +214,9 @@ static int splits_for(int tiles, int chunks) {
   return std::min(s, std::max(1, chunks));
 }
 
-static Layout make_layout(int o, int v, int vjp) {
+// Vp, T2m and OOt hold FP64 copies, or packed bf16 triplets for split-BF16x3
+// (same 8-byte element); split-BF16x3 adds packed copies of t2 and of Wbar.
+static Layout make_layout(int o, int v, int vjp, int prec) {
   Layout L;
   memset(&L, 0, sizeof(L));
   const size_t d = sizeof(double);
@@ -225,6 +233,10 @@ static Layout make_layout(int o, int v, int vjp) {
   L.off_W = take(v3);
   L.off_tri = take((size_t)o * o * o);
   L.off_part_t = take(kGmax);
+  if (prec == PAMD_PREC_BF16X3) {
+    L.off_T2g = take((size_t)o * o * v2);
+    if (vjp) L.off_W8 = take(v3);
+  }
   const int tiles_p = gemm_tiles(v, v);
   const int tiles_h = gemm_tiles(v, o);
   const int chunks = (int)((v2 + KC - 1) / KC);
@@ -244,7 +256,9 @@ static Layout make_layout(int o, int v, int vjp) {
 }
 
 struct Ctx {
-  int o, v, packed, has_fvo, vjp;
+  int o, v, packed, has_fvo, vjp, prec;
+  const double *gT2;  // t2 as a GEMM operand (FP64, or packed bf16 triplets)
+  double *gW;         // Wbar as a GEMM operand (= W, or its packed copy)
   hipStream_t s;
   const double *mo, *t1, *t2, *ovvv, *ovoo, *ovov, *fvo;
   double *Vp, *T2m, *OOt, *W, *tri, *part_t;
@@ -271,7 +285,7 @@ static int build_w(const Ctx &c, int i, int j, int k) {
   const long long o = c.o, v = c.v;
   const long long v2 = v * v, v3 = v2 * v, vo = v * o;
   auto Vp = [&](int x) { return c.Vp + x * v3; };
-  auto T2 = [&](int x, int y) { return c.t2 + (x * o + y) * v2; };
+  auto T2 = [&](int x, int y) { return c.gT2 + (x * o + y) * v2; };
   auto T2m = [&](int x) { return c.T2m + x * v2 * o; };
   auto OOt = [&](int x, int y) { return c.OOt + (x * o + y) * vo; };
   const int V = c.v, O = c.o;
@@ -315,10 +329,10 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
   const long long o = c.o, v = c.v;
   const long long v2 = v * v, v3 = v2 * v, vo = v * o;
   const int V = c.v, O = c.o;
-  const double *Wb = c.W;
+  const double *Wb = c.gW;
   auto Vp = [&](int x) { return c.Vp + x * v3; };
   auto Vpb = [&](int x) { return c.Vpb + x * v3; };
-  auto T2 = [&](int x, int y) { return c.t2 + (x * o + y) * v2; };
+  auto T2 = [&](int x, int y) { return c.gT2 + (x * o + y) * v2; };
   auto T2m = [&](int x) { return c.T2m + x * v2 * o; };
   auto T2mb = [&](int x) { return c.T2mb + x * v2 * o; };
   auto OOt = [&](int x, int y) { return c.OOt + (x * o + y) * vo; };
@@ -464,7 +478,11 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
 
 static int run(Ctx &c) {
   const int o = c.o, v = c.v;
-  PAMD_CHECK(launch_prepare(c.ovvv, c.packed, c.t2, c.ovoo, o, v, c.Vp, c.T2m, c.OOt, c.s));
+  g_prec = c.prec;
+  const int bf = c.prec == PAMD_PREC_BF16X3;
+  PAMD_CHECK(launch_prepare(c.ovvv, c.packed, c.t2, c.ovoo, o, v, c.Vp, c.T2m, c.OOt, bf, c.s));
+  if (bf)
+    PAMD_CHECK(launch_split_bf16x3(c.t2, (size_t)o * o * v * v, (unsigned long long *)c.gT2, c.s));
   PAMD_CHECK(launch_fill(c.tri, (size_t)o * o * o, 0.0, c.s));
   const int nt = (v + 7) / 8;
   const long long norb = (long long)nt * (nt + 1) * (nt + 2) / 6;
@@ -481,6 +499,8 @@ static int run(Ctx &c) {
     PAMD_CHECK(launch_fill(c.fvo_bar, (size_t)v * o, 0.0, c.s));
   }
   long long tcount = 0;
+  const long long ntrip = (long long)o * (o + 1) * (o + 2) / 6 - o;
+  const auto t_start = std::chrono::steady_clock::now();
   for (int i = 0; i < o; ++i)
     for (int j = i; j < o; ++j)
       for (int k = j; k < o; ++k) {
@@ -517,8 +537,17 @@ static int run(Ctx &c) {
         }
         PAMD_CHECK(launch_triple_energy(c.part_t, c.G, c.tri + ((size_t)i * o + j) * o + k, c.s));
         if (c.vjp) {
+          if (bf) PAMD_CHECK(launch_split_bf16x3(c.W, (size_t)v * v * v, (unsigned long long *)c.gW, c.s));
           st = adjoint(c, i, j, k);
           if (st != PAMD_OK) return st;
+        }
+        if (g_progress_every > 0 && tcount % g_progress_every == 0) {
+          PAMD_CHECK(hipStreamSynchronize(c.s));
+          const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() -
+                                                           t_start).count();
+          fprintf(stderr, "pyscfad_amd: (T)%s triples %lld/%lld done, %.1f s\n",
+                  c.vjp ? "+VJP" : "", tcount, ntrip, el);
+          fflush(stderr);
         }
       }
   g_prof_on = false;
@@ -532,7 +561,8 @@ static int check_common(const pamd_ccsdt_config *cfg, const double *mo, const do
   if (!cfg || cfg->nocc < 1 || cfg->nvir < 1) return PAMD_ERR_ARG;
   if (!mo || !t1 || !t2 || !ovvv || !ovoo || !ovov) return PAMD_ERR_ARG;
   if (cfg->has_fvo && !fvo) return PAMD_ERR_ARG;
-  if (cfg->precision != PAMD_PREC_FP64) return PAMD_ERR_UNSUPPORTED;
+  if (cfg->precision != PAMD_PREC_FP64 && cfg->precision != PAMD_PREC_BF16X3)
+    return PAMD_ERR_UNSUPPORTED;
   if (cfg->nvir > 4096 || cfg->nocc > 1024) return PAMD_ERR_ARG;
   return PAMD_OK;
 }
@@ -545,6 +575,8 @@ static void bind_workspace(Ctx &c, const Layout &L, void *ws) {
   c.W = (double *)(b + L.off_W);
   c.tri = (double *)(b + L.off_tri);
   c.part_t = (double *)(b + L.off_part_t);
+  c.gT2 = c.prec == PAMD_PREC_BF16X3 ? (const double *)(b + L.off_T2g) : c.t2;
+  c.gW = (c.prec == PAMD_PREC_BF16X3 && c.vjp) ? (double *)(b + L.off_W8) : c.W;
   if (c.vjp) {
     c.Vb = (double *)(b + L.off_Vb);
     c.Vpb = (double *)(b + L.off_Vpb);
@@ -570,7 +602,7 @@ const char *pamd_version(void) { return "pyscfad_amd 0.1.0 (gfx950)"; }
 
 int pamd_ccsdt_workspace_size(const pamd_ccsdt_config *cfg, int with_vjp, size_t *bytes) {
   if (!cfg || !bytes || cfg->nocc < 1 || cfg->nvir < 1) return PAMD_ERR_ARG;
-  *bytes = make_layout(cfg->nocc, cfg->nvir, with_vjp ? 1 : 0).total;
+  *bytes = make_layout(cfg->nocc, cfg->nvir, with_vjp ? 1 : 0, cfg->precision).total;
   return PAMD_OK;
 }
 
@@ -580,7 +612,7 @@ static int energy_impl(const pamd_ccsdt_config *cfg, const double *mo, const dou
                        hipStream_t s, Ctx &c) {
   int st = check_common(cfg, mo, t1, t2, ovvv, ovoo, ovov, fvo);
   if (st != PAMD_OK) return st;
-  Layout L = make_layout(cfg->nocc, cfg->nvir, 0);
+  Layout L = make_layout(cfg->nocc, cfg->nvir, 0, cfg->precision);
   if (!ws || ws_bytes < L.total) return PAMD_ERR_WORKSPACE;
   memset(&c, 0, sizeof(c));
   c.o = cfg->nocc;
@@ -588,6 +620,7 @@ static int energy_impl(const pamd_ccsdt_config *cfg, const double *mo, const dou
   c.packed = cfg->ovvv_packed ? 1 : 0;
   c.has_fvo = cfg->has_fvo ? 1 : 0;
   c.vjp = 0;
+  c.prec = cfg->precision;
   c.s = s;
   c.mo = mo;
   c.t1 = t1;
@@ -640,7 +673,7 @@ int pamd_ccsdt_energy_vjp(const pamd_ccsdt_config *cfg, const double *mo_energy,
       !fvo_bar)
     return PAMD_ERR_ARG;
   if (cfg->nvir > 512) return PAMD_ERR_UNSUPPORTED;
-  Layout L = make_layout(cfg->nocc, cfg->nvir, 1);
+  Layout L = make_layout(cfg->nocc, cfg->nvir, 1, cfg->precision);
   if (!workspace || workspace_bytes < L.total) return PAMD_ERR_WORKSPACE;
   Ctx c;
   memset(&c, 0, sizeof(c));
@@ -649,6 +682,7 @@ int pamd_ccsdt_energy_vjp(const pamd_ccsdt_config *cfg, const double *mo_energy,
   c.packed = cfg->ovvv_packed ? 1 : 0;
   c.has_fvo = cfg->has_fvo ? 1 : 0;
   c.vjp = 1;
+  c.prec = cfg->precision;
   c.s = (hipStream_t)stream;
   c.mo = mo_energy;
   c.t1 = t1;
@@ -679,7 +713,7 @@ int pamd_debug_gemm_descriptors(int nocc, int nvir, int i, int j, int k, int wit
                                 int *n_desc, int *desc_bytes) {
   if (nocc < 1 || nvir < 1 || !bases || !n_desc || !desc_bytes) return PAMD_ERR_ARG;
   if (i < 0 || j < i || k < j || k >= nocc) return PAMD_ERR_ARG;
-  Layout L = make_layout(nocc, nvir, with_vjp ? 1 : 0);
+  Layout L = make_layout(nocc, nvir, with_vjp ? 1 : 0, PAMD_PREC_FP64);
   Ctx c;
   memset(&c, 0, sizeof(c));
   c.o = nocc;
@@ -689,7 +723,9 @@ int pamd_debug_gemm_descriptors(int nocc, int nvir, int i, int j, int k, int wit
   c.T2m = (double *)bases[1];
   c.OOt = (double *)bases[2];
   c.W = (double *)bases[3];
+  c.gW = c.W;
   c.t2 = (const double *)bases[4];
+  c.gT2 = c.t2;
   c.Vb = (double *)bases[5];
   c.Vpb = (double *)bases[6];
   c.T2mb = (double *)bases[7];
@@ -718,6 +754,12 @@ int pamd_profile_enable(int every) {
   return PAMD_OK;
 }
 
+int pamd_profile_progress(int every) {
+  if (every < 0) return PAMD_ERR_ARG;
+  g_progress_every = every;
+  return PAMD_OK;
+}
+
 int pamd_profile_reset(void) {
   memset(g_prof_stat, 0, sizeof(g_prof_stat));
   return PAMD_OK;
@@ -740,7 +782,7 @@ int pamd_profile_read(int kind, long long *launches, double *ms, double *flops, 
 
 int pamd_debug_split_factors(int nocc, int nvir, int *S_p, int *S_h) {
   if (nocc < 1 || nvir < 1 || !S_p || !S_h) return PAMD_ERR_ARG;
-  Layout L = make_layout(nocc, nvir, 1);
+  Layout L = make_layout(nocc, nvir, 1, PAMD_PREC_FP64);
   *S_p = L.S_p;
   *S_h = L.S_h;
   return PAMD_OK;

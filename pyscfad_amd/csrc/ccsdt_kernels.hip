@@ -14,6 +14,8 @@
 // * k_combine    -- per-triple gather of the adjoint pieces into the cotangents.
 // * prepare/fold -- layout changes at entry/exit.
 #include "ccsdt_common.h"
+#include "engine_staging.h"
+#include "bf16_split.h"
 
 namespace pamd {
 
@@ -61,8 +63,9 @@ __device__ double block_sum256(double x, double *scratch) {
 //   T2m[x][p][q][m] = t2[x,m,p,q]
 //   OOt[x][y][p][m] = ovoo[x,p,m,y]
 // ---------------------------------------------------------------------------
+template <class T>
 __global__ void k_prep_vp(const double *__restrict__ ovvv, int packed, int o, int v,
-                          double *__restrict__ Vp) {
+                          T *__restrict__ Vp) {
   const size_t total = (size_t)o * v * v * v;
   const size_t vv = (size_t)v * v;
   const size_t npair = (size_t)v * (v + 1) / 2;
@@ -78,11 +81,12 @@ __global__ void k_prep_vp(const double *__restrict__ ovvv, int packed, int o, in
     } else {
       val = ovvv[xp * vv + f * v + q];
     }
-    Vp[t] = val;
+    put(Vp, t, val);
   }
 }
 
-__global__ void k_prep_t2m(const double *__restrict__ t2, int o, int v, double *__restrict__ T2m) {
+template <class T>
+__global__ void k_prep_t2m(const double *__restrict__ t2, int o, int v, T *__restrict__ T2m) {
   const size_t total = (size_t)o * o * v * v;
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (size_t)gridDim.x * blockDim.x) {
@@ -90,11 +94,12 @@ __global__ void k_prep_t2m(const double *__restrict__ t2, int o, int v, double *
     size_t q = (t / o) % v;
     size_t p = (t / ((size_t)o * v)) % v;
     size_t x = t / ((size_t)o * v * v);
-    T2m[t] = t2[((x * o + m) * v + p) * v + q];
+    put(T2m, t, t2[((x * o + m) * v + p) * v + q]);
   }
 }
 
-__global__ void k_prep_oot(const double *__restrict__ ovoo, int o, int v, double *__restrict__ OOt) {
+template <class T>
+__global__ void k_prep_oot(const double *__restrict__ ovoo, int o, int v, T *__restrict__ OOt) {
   const size_t total = (size_t)o * o * v * o;
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (size_t)gridDim.x * blockDim.x) {
@@ -102,7 +107,7 @@ __global__ void k_prep_oot(const double *__restrict__ ovoo, int o, int v, double
     size_t p = (t / o) % v;
     size_t y = (t / ((size_t)o * v)) % o;
     size_t x = t / ((size_t)o * v * o);
-    OOt[t] = ovoo[((x * v + p) * o + m) * o + y];
+    put(OOt, t, ovoo[((x * v + p) * o + m) * o + y]);
   }
 }
 
@@ -113,15 +118,27 @@ static unsigned grid_for(size_t n) {
   return (unsigned)b;
 }
 
-hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, const double *ovoo,
-                          int o, int v, double *Vp, double *T2m, double *OOt, hipStream_t s) {
-  hipLaunchKernelGGL(k_prep_vp, dim3(grid_for((size_t)o * v * v * v)), dim3(256), 0, s, ovvv,
+template <class T>
+static hipError_t prepare_as(const double *ovvv, int packed, const double *t2, const double *ovoo,
+                             int o, int v, T *Vp, T *T2m, T *OOt, hipStream_t s) {
+  hipLaunchKernelGGL(k_prep_vp<T>, dim3(grid_for((size_t)o * v * v * v)), dim3(256), 0, s, ovvv,
                      packed, o, v, Vp);
-  hipLaunchKernelGGL(k_prep_t2m, dim3(grid_for((size_t)o * o * v * v)), dim3(256), 0, s, t2, o, v,
-                     T2m);
-  hipLaunchKernelGGL(k_prep_oot, dim3(grid_for((size_t)o * o * v * o)), dim3(256), 0, s, ovoo, o,
+  hipLaunchKernelGGL(k_prep_t2m<T>, dim3(grid_for((size_t)o * o * v * v)), dim3(256), 0, s, t2, o,
+                     v, T2m);
+  hipLaunchKernelGGL(k_prep_oot<T>, dim3(grid_for((size_t)o * o * v * o)), dim3(256), 0, s, ovoo, o,
                      v, OOt);
   return hipGetLastError();
+}
+
+// split_bf16: write packed (hi, mid, lo) bf16 triplets instead of FP64 copies
+hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, const double *ovoo,
+                          int o, int v, double *Vp, double *T2m, double *OOt, int split_bf16,
+                          hipStream_t s) {
+  if (split_bf16) {
+    typedef unsigned long long u64;
+    return prepare_as<u64>(ovvv, packed, t2, ovoo, o, v, (u64 *)Vp, (u64 *)T2m, (u64 *)OOt, s);
+  }
+  return prepare_as<double>(ovvv, packed, t2, ovoo, o, v, Vp, T2m, OOt, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -138,123 +155,6 @@ hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, cons
 // the occupied-index slab into the pointer and checks the range); row offsets
 // are computed once per segment and K offsets advance without divisions.
 // ---------------------------------------------------------------------------
-constexpr int GKC = 16;
-
-template <int R, int NTHR>
-struct OpStage {
-  static constexpr int E = R * GKC / NTHR;  // staged elements per thread
-  static constexpr int RSTEP = NTHR / R;    // rfast: k step between a thread's elements
-  static constexpr int KSTEP = NTHR / 16;   // kfast: row step between a thread's elements
-  static constexpr int LR = R + 16;         // [k][r] row stride (doubles)
-  static constexpr int LK = GKC + 2;        // [r][k] row stride (doubles; 16-B aligned rows)
-  static constexpr int LDS = (GKC * LR > R * LK) ? GKC * LR : R * LK;
-  // every (row, k) of the R x 16 panel is staged exactly once by both mappings
-  static_assert((R * GKC) % NTHR == 0 && NTHR % R == 0 && NTHR % 16 == 0, "staging map");
-  static_assert(KSTEP * E == R && RSTEP * E == GKC, "staging map covers the panel");
-};
-
-template <int R, int NTHR>
-__device__ __forceinline__ void op_rows(const OpDesc &d, int R_total, int r0, int tid,
-                                        int (&roff)[OpStage<R, NTHR>::E],
-                                        bool (&rok)[OpStage<R, NTHR>::E]) {
-  using S = OpStage<R, NTHR>;
-  if (d.rfast) {
-    const int r = r0 + tid % R;
-    rok[0] = r < R_total;
-    const int q = r / d.rdiv;
-    roff[0] = rok[0] ? (int)(q * d.rx + (r - q * d.rdiv) * d.ry) : 0;
-#pragma unroll
-    for (int e = 1; e < S::E; ++e) {
-      rok[e] = false;
-      roff[e] = 0;
-    }
-  } else {
-#pragma unroll
-    for (int e = 0; e < S::E; ++e) {
-      const int r = r0 + (tid >> 4) + S::KSTEP * e;
-      rok[e] = r < R_total;
-      const int q = r / d.rdiv;
-      roff[e] = rok[e] ? (int)(q * d.rx + (r - q * d.rdiv) * d.ry) : 0;
-    }
-  }
-}
-
-// K position of a chunk as (q, r) = divmod(kappa0, kdiv), advanced without division
-struct KPos {
-  int q, r;
-};
-
-__device__ __forceinline__ int k_offset(const OpDesc &d, KPos kp, int add) {
-  int r = kp.r + add, q = kp.q;
-  if (d.kdiv >= GKC) {
-    if (r >= d.kdiv) {
-      r -= d.kdiv;
-      ++q;
-    }
-  } else {
-    while (r >= d.kdiv) {
-      r -= d.kdiv;
-      ++q;
-    }
-  }
-  return (int)(q * d.kx + r * d.ky);
-}
-
-// Loads are issued unconditionally from a clamped (in-range) address; the raw
-// values stay in registers and the mask and alpha are applied when they are
-// written to LDS, so nothing waits on the loads before the MFMA phase.
-template <int R, int NTHR>
-__device__ __forceinline__ void op_fetch(const OpDesc &d, int L, int kappa0, KPos kp, int tid,
-                                         const int (&roff)[OpStage<R, NTHR>::E],
-                                         const bool (&rok)[OpStage<R, NTHR>::E],
-                                         double (&v)[OpStage<R, NTHR>::E],
-                                         bool (&ok)[OpStage<R, NTHR>::E]) {
-  using S = OpStage<R, NTHR>;
-  if (d.rfast) {
-#pragma unroll
-    for (int e = 0; e < S::E; ++e) {
-      const int kl = tid / R + S::RSTEP * e;
-      ok[e] = rok[0] && (kappa0 + kl) < L;
-      const int off = ok[e] ? roff[0] + k_offset(d, kp, kl) : 0;
-      v[e] = d.p[off];
-    }
-  } else {
-    const int kl = tid & 15;
-    const bool kok = (kappa0 + kl) < L;
-    const int ko = kok ? k_offset(d, kp, kl) : 0;
-#pragma unroll
-    for (int e = 0; e < S::E; ++e) {
-      ok[e] = kok && rok[e];
-      v[e] = d.p[ok[e] ? roff[e] + ko : 0];
-    }
-  }
-}
-
-template <int R, int NTHR>
-__device__ __forceinline__ void op_stash(int rfast, double alpha, int tid,
-                                         const double (&v)[OpStage<R, NTHR>::E],
-                                         const bool (&ok)[OpStage<R, NTHR>::E], double *Sm) {
-  using S = OpStage<R, NTHR>;
-  if (rfast) {
-    const int r = tid % R;
-#pragma unroll
-    for (int e = 0; e < S::E; ++e)
-      Sm[(tid / R + S::RSTEP * e) * S::LR + r] = ok[e] ? v[e] * alpha : 0.0;
-  } else {
-    const int k = tid & 15;
-#pragma unroll
-    for (int e = 0; e < S::E; ++e)
-      Sm[((tid >> 4) + S::KSTEP * e) * S::LK + k] = ok[e] ? v[e] * alpha : 0.0;
-  }
-}
-
-// Workgroup w runs on XCD w % 8; give each XCD a contiguous range of logical
-// tiles so that tiles sharing an operand panel share that XCD's L2.
-__device__ __forceinline__ int xcd_remap(int w, int n) {
-  const int q = n >> 3, r = n & 7, x = w & 7, k = w >> 3;
-  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-}
-
 // Tile MT x NT with WM x WN waves; each wave (MT/WM) x (NT/WN).  The grid is
 // 1-D over (split, tile) with the N tiles innermost when n_fast (the A panel
 // is then read once and reused from L2), else the M tiles innermost.
@@ -265,8 +165,8 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, int n_fas
   using SB = OpStage<NT, NTHR>;
   constexpr int WTM = MT / WM, WTN = NT / WN;
   constexpr int TM = WTM / 16, TN = WTN / 16;
-  __shared__ double As[2][SA::LDS];
-  __shared__ double Bs[2][SB::LDS];
+  __shared__ __align__(16) double As[2][SA::LDS];
+  __shared__ __align__(16) double Bs[2][SB::LDS];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;

@@ -117,12 +117,24 @@ def make_problem(nocc, nvir, naux, seed=0, fvo_scale=0.0, device=None,
     import torch
     dev = torch.device(device)
     tri_t = torch.from_numpy(tri).to(dev)
-    iu_t = torch.from_numpy(iu).to(dev)
-    ju_t = torch.from_numpy(ju).to(dev)
-    L = torch.zeros((naux, n, n), dtype=torch.float64, device=dev)
+    L = unpack_L_torch(torch.from_numpy(iu).to(dev), torch.from_numpy(ju).to(dev), tri_t, naux, n)
+    del tri_t
+    return assemble_torch(L, mo_energy, fock, t1, nocc, nvir)
+
+
+def unpack_L_torch(iu_t, ju_t, tri_t, naux, n):
+    import torch
+    L = torch.zeros((naux, n, n), dtype=torch.float64, device=tri_t.device)
     L[:, iu_t, ju_t] = tri_t
     L[:, ju_t, iu_t] = tri_t
-    del tri_t
+    return L
+
+
+def assemble_torch(L, mo_energy, fock, t1, nocc, nvir):
+    """ERI blocks and MP2 t2 on L's device from a symmetric L (naux, n, n)."""
+    import torch
+    dev = L.device
+    naux = L.shape[0]
     Lov = L[:, :nocc, nocc:].reshape(naux, nocc * nvir)
     Loo = L[:, :nocc, :nocc].reshape(naux, nocc * nocc)
     # ovvv in slabs of the occupied index to bound the temporary
@@ -134,14 +146,12 @@ def make_problem(nocc, nvir, naux, seed=0, fvo_scale=0.0, device=None,
     del Lvv
     ovoo = (Lov.T @ Loo).reshape(nocc, nvir, nocc, nocc).contiguous()
     ovov = (Lov.T @ Lov).reshape(nocc, nvir, nocc, nvir).contiguous()
-    del L, Lov, Loo
-    e_occ_t = torch.from_numpy(e_occ).to(dev)
-    e_vir_t = torch.from_numpy(e_vir).to(dev)
-    eia = e_occ_t[:, None] - e_vir_t[None, :]
+    del Lov, Loo
+    me = torch.as_tensor(mo_energy, dtype=torch.float64, device=dev)
+    eia = me[:nocc, None] - me[None, nocc:]
     t2 = (ovov.permute(0, 2, 1, 3) /
           (eia[:, None, :, None] + eia[None, :, None, :])).contiguous()
-    return Problem(nocc, nvir, naux,
-                   torch.from_numpy(mo_energy).to(dev),
-                   torch.from_numpy(fock).to(dev),
-                   torch.from_numpy(t1).to(dev),
+    return Problem(nocc, nvir, naux, me,
+                   torch.as_tensor(fock, dtype=torch.float64, device=dev),
+                   torch.as_tensor(t1, dtype=torch.float64, device=dev),
                    t2, ovvv, ovoo, ovov)

@@ -1,0 +1,53 @@
+"""MP2-level pair features and pair-resolved (T) labels.
+
+Features (128 for nocc = 16): the 120 unordered occupied-pair MP2 energies
+eps_ij = e_ij + e_ji (i < j), with e_ij = sum_ab t2[i,j,a,b] (2 (ia|jb) - (ib|ja)),
+the pair resolution of the MP2 energy of pyscfad/cc/ccsd.py:380-381, followed by
+8 orbital statistics: HOMO-LUMO gap, mean/std/min of the occupied energies,
+mean/std/max of the virtual energies, and the total MP2 energy.
+
+Labels (136 for nocc = 16): the pair-resolved (T) energy of the exact kernel
+(``pamd_ccsdt_pair_energy``), folded over unordered pairs i <= j:
+tau_ij = e(T)_ij + e(T)_ji (i < j), tau_ii = e(T)_ii; their sum is E(T).
+"""
+from __future__ import annotations
+
+import torch
+
+N_GAP = 8
+
+
+def n_features(nocc):
+    return nocc * (nocc - 1) // 2 + N_GAP
+
+
+def n_labels(nocc):
+    return nocc * (nocc + 1) // 2
+
+
+def mp2_pair_energies(t2, ovov):
+    """(o, o) MP2 pair energies e_ij (torch, float64)."""
+    direct = torch.einsum('ijab,iajb->ij', t2, ovov)
+    exch = torch.einsum('ijab,ibja->ij', t2, ovov)
+    return 2.0 * direct - exch
+
+
+def features(mo_energy, t2, ovov, nocc):
+    e = mp2_pair_energies(t2, ovov)
+    iu, ju = torch.triu_indices(nocc, nocc, offset=1, device=e.device)
+    pair = (e + e.T)[iu, ju]
+    eo, ev = mo_energy[:nocc], mo_energy[nocc:]
+    stats = torch.stack([ev.min() - eo.max(), eo.mean(), eo.std(), eo.min(),
+                         ev.mean(), ev.std(), ev.max(), e.sum()])
+    return torch.cat([pair, stats])
+
+
+def fold_pairs(e_pair):
+    """(o, o) pair-resolved energies -> i <= j vector (sum preserved)."""
+    o = e_pair.shape[0]
+    iu, ju = torch.triu_indices(o, o, offset=0, device=e_pair.device)
+    sym = e_pair + e_pair.T
+    out = sym[iu, ju]
+    diag = iu == ju
+    out[diag] = e_pair.diagonal()
+    return out

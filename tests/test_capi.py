@@ -58,3 +58,23 @@ def test_argument_validation_without_gpu():
 
 def test_config_struct_layout():
     assert ctypes.sizeof(_lib.Config) == 24
+
+
+def test_mlp_argument_validation_without_gpu():
+    from pyscfad_amd.screen import mlp
+    null = ctypes.c_void_p(0)
+    fake = ctypes.c_void_p(4096)   # never dereferenced: every call below is rejected first
+    gemm = mlp._fn('pamd_bf16_gemm_tn')
+    # K not a multiple of 8, lda < K, misaligned operand, missing operand
+    assert gemm(4, 4, 12, fake, 16, fake, 16, null, fake, 4, 0, null) == _lib.PAMD_ERR_ARG
+    assert gemm(4, 4, 16, fake, 8, fake, 16, null, fake, 4, 0, null) == _lib.PAMD_ERR_ARG
+    assert gemm(4, 4, 16, ctypes.c_void_p(4098), 16, fake, 16, null, fake, 4, 0,
+                null) == _lib.PAMD_ERR_ARG
+    assert gemm(4, 4, 16, null, 16, fake, 16, null, fake, 4, 0, null) == _lib.PAMD_ERR_ARG
+    act = mlp._fn('pamd_bf16_act_fwd')
+    assert act(4, 4, fake, 2, fake, null, null) == _lib.PAMD_ERR_ARG
+    adam = mlp._fn('pamd_adam_step')
+    assert adam(2, 2, fake, fake, fake, fake, null, null, 1e-3, 0.9, 0.999, 1e-8, 0.0, 0,
+                null) == _lib.PAMD_ERR_ARG
+    conv = mlp._fn('pamd_f32_to_bf16')
+    assert conv(15, fake, fake, 4, 4, null, null) == _lib.PAMD_ERR_ARG
