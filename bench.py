@@ -1,16 +1,20 @@
 #!/usr/bin/env python
 """Benchmark of the closed-shell (T) energy + VJP on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|bf16x3]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision bf16x3|fp64]
+                    [--compare fp64|none]
 
 One step = one E(T)+VJP pass (E and every cotangent block, et_bar = 1) over the
 seeded synthetic problem nocc=40, nvir=400, naux=1600 (packed ovvv, the DF layout),
-with all inputs already resident in HBM when the timer starts.  Prints one JSON
-line: seconds per E(T)+VJP, the roofline of the dominant kernel (the FP64 MFMA
-contraction engine; its launches are timed with hipEvents on their own stream
-for one occupied triple in --profile-every), and the reference's own C
-(oracle/_ref, built from the reference sources) timed on this host's cores on a
-bounded sample and scaled by algorithmic flops.
+with all inputs already resident in HBM when the timer starts.  The timed form
+is the split-BF16x3 one by default; afterwards one plain FP64 step runs on the
+same inputs and the line reports its time, the speed-up, |dE(T)| and the
+relative error of every gradient block against it.  Prints one JSON line:
+seconds per E(T)+VJP, the roofline of the dominant kernel (the contraction
+engine; its launches are timed with hipEvents on their own stream for one
+occupied triple in --profile-every), and the reference's own C (oracle/_ref,
+built from the reference sources) timed on this host's cores on a bounded
+sample and scaled by algorithmic flops.  Progress lines go to stderr.
 """
 from __future__ import annotations
 
@@ -53,12 +57,39 @@ def cpu_baseline(o_full, v_full, sample, threads):
             'gflops': alg_flops(so, sv) / dt / 1e9}
 
 
+def engine_roofline(prof, precision):
+    """Roofline of the contraction engine from the sampled hipEvent timings."""
+    gemm = [prof[k] for k in ('gemm_wbuild', 'gemm_adjoint') if prof[k]['launches']]
+    g_ms = sum(x['ms'] for x in gemm)
+    g_fl = sum(x['flops'] for x in gemm)   # FP64-equivalent: 2 M N K per launch
+    g_n = sum(x['launches'] for x in gemm)
+    fp64_equiv = g_fl / (g_ms * 1e-3) / 1e12 if g_ms > 0 else None
+    if precision == 'bf16x3':
+        # six bf16 32x32x16 MFMA products per FP64 multiply-add
+        kernel, mfma_factor, peak = 'k_gemm_bf16x3', 6.0, PEAKS['bf16']
+    else:
+        kernel, mfma_factor, peak = 'k_gemm_f64', 1.0, PEAKS['fp64']
+    achieved = fp64_equiv * mfma_factor if fp64_equiv else None
+    roofline = {'bound': 'mfma', 'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s',
+                'frac': (achieved / peak) if achieved else None, 'traffic': None,
+                'kernel': kernel, 'mfma_flops_per_alg_flop': mfma_factor,
+                'fp64_equiv_tflops': fp64_equiv,
+                'frac_fp64_peak': (fp64_equiv / PEAKS['fp64']) if fp64_equiv else None,
+                'launches_timed': g_n,
+                'avg_launch_ms': g_ms / g_n if g_n else None,
+                'flops_per_launch': g_fl * mfma_factor / g_n if g_n else None}
+    return roofline
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=1)
     ap.add_argument('--warmup', type=int, default=0)
-    ap.add_argument('--precision', default='fp64', choices=['fp64', 'bf16x3', 'fp32x2'])
+    ap.add_argument('--precision', default='bf16x3', choices=['fp64', 'bf16x3'])
+    ap.add_argument('--compare', default='fp64', choices=['fp64', 'none'],
+                    help='after the timed steps, run one plain FP64 step on the same inputs '
+                         'and report its time and the |dE| / gradient errors against it')
     ap.add_argument('--nocc', type=int, default=40)
     ap.add_argument('--nvir', type=int, default=400)
     ap.add_argument('--naux', type=int, default=None)
@@ -132,19 +163,36 @@ def main():
     sec = (t1 - t0) / args.steps
     e_val = float(e.item())
 
-    gemm = [prof[k] for k in ('gemm_f64_wbuild', 'gemm_f64_adjoint') if prof[k]['launches']]
-    g_ms = sum(x['ms'] for x in gemm)
-    g_fl = sum(x['flops'] for x in gemm)
-    g_n = sum(x['launches'] for x in gemm)
-    achieved = g_fl / (g_ms * 1e-3) / 1e12 if g_ms > 0 else None
-    peak = PEAKS['fp64']
-    roofline = {'bound': 'mfma', 'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s',
-                'frac': (achieved / peak) if achieved else None, 'traffic': None,
-                'kernel': 'k_gemm_f64', 'launches_timed': g_n,
-                'avg_launch_ms': g_ms / g_n if g_n else None,
-                'flops_per_launch': g_fl / g_n if g_n else None}
+    roofline = engine_roofline(prof, args.precision)
 
     F = alg_flops(o, v) if not args.energy_only else alg_flops(o, v) / 3.0
+    compare = None
+    if args.compare == 'fp64' and args.precision != 'fp64':
+        print('bench: FP64 comparison step on the same inputs', file=sys.stderr, flush=True)
+        ref_inp = _ops.Inputs(inp.mo_energy, inp.t1, inp.t2, inp.ovvv, inp.ovoo, inp.ovov, None,
+                              precision='fp64', device=dev)
+        _lib.profile_reset()
+        _lib.profile_enable(args.profile_every)
+        _lib.profile_progress(args.progress_every)
+        torch.cuda.synchronize()
+        c0 = time.perf_counter()
+        if args.energy_only:
+            e64, g64 = _ops.energy(ref_inp), None
+        else:
+            e64, g64 = _ops.energy_vjp(ref_inp, et_bar=1.0)
+        torch.cuda.synchronize()
+        sec64 = time.perf_counter() - c0
+        _lib.profile_enable(0)
+        _lib.profile_progress(0)
+        compare = {'precision': 'fp64', 'sec': sec64, 'e_t': float(e64.item()),
+                   'abs_dE': abs(float(e64.item()) - e_val), 'speedup': sec64 / sec,
+                   'tflops_alg': F / sec64 / 1e12,
+                   'roofline': engine_roofline(_lib.profile_read(), 'fp64')}
+        if g64 is not None:
+            compare['grad_rel_err'] = {
+                k: float(torch.linalg.norm(grads[k] - g64[k]) / torch.linalg.norm(g64[k]))
+                for k in g64}
+        del g64
     cpu = None
     if not args.no_cpu_baseline:
         so, sv = (int(x) for x in args.cpu_sample.split(','))
@@ -176,6 +224,7 @@ def main():
         'frac_fp64_peak_alg': F / sec / 1e12 / PEAKS['fp64'],
         'input_gen_s': t_gen,
         'roofline': roofline,
+        'compare_fp64': compare,
         'cpu_baseline': cpu,
         'kernels': {k: dict(x, avg_ms=(x['ms'] / x['launches'] if x['launches'] else None))
                     for k, x in prof.items()},

@@ -60,7 +60,8 @@ struct GemmDesc {
 struct OrbitArgs {
   double *W;            // v^3; W in, Wbar out (vjp)
   double *Vb;           // v^3; Vbar out (vjp)
-  const double *t1, *t2, *ovov, *fvo, *mo;
+  // OVS[x][y][p][q] = ovov[x,p,y,q] + ovov[y,q,x,p], T2S[x][y][p][q] = t2[x,y,p,q] + t2[y,x,q,p]
+  const double *t1, *ovs, *t2s, *fvo, *mo;
   int o, v, i, j, k, has_fvo, vjp, nt;
   long long norb;
   double omega;         // 2 s_ijk: 2, 1 (two indices equal)
@@ -71,7 +72,7 @@ struct OrbitArgs {
 
 // V-bar reductions for one triple.
 struct VRArgs {
-  const double *Vb, *t1, *fvo, *ovov, *t2;
+  const double *Vb, *t1, *fvo, *ovs, *t2s;  // ovs, t2s: pair-symmetrised blocks (OrbitArgs)
   int o, v, i, j, k, has_fvo;
   double *R;            // scratch: 10 v^2 + 2 v doubles (layout in kernels)
 };
@@ -88,6 +89,7 @@ struct CombineArgs {
 
 // host-side launchers (ccsdt_kernels.hip)
 hipError_t launch_fill(double *p, size_t n, double val, hipStream_t s);
+hipError_t launch_prep_sym(const double *src, int o, int v, int kind, double *dst, hipStream_t s);
 hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, const double *ovoo,
                           int o, int v, double *Vp, double *T2m, double *OOt, int split_bf16,
                           hipStream_t s);

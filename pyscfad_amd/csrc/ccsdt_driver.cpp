@@ -39,7 +39,7 @@ static std::vector<GemmDesc> *g_record = nullptr;
 // launch is bracketed by hipEvents on its own stream; the events are read after
 // the call has been queued completely.
 enum ProfKind { kProfWBuild = 0, kProfAdjGemm, kProfOrbit, kProfVR, kProfCombine, kProfNum };
-static const char *kProfNames[kProfNum] = {"gemm_f64_wbuild", "gemm_f64_adjoint", "orbit_epilogue",
+static const char *kProfNames[kProfNum] = {"gemm_wbuild", "gemm_adjoint", "orbit_epilogue",
                                            "vbar_reduce", "combine"};
 struct ProfRec {
   int kind;
@@ -199,7 +199,7 @@ static GemmDesc gemm(int M, int N, int mode, std::initializer_list<SegDesc> segs
 }
 
 struct Layout {
-  size_t off_Vp, off_T2m, off_OOt, off_W, off_tri, off_part_t, off_T2g, off_W8;
+  size_t off_Vp, off_T2m, off_OOt, off_OVS, off_T2S, off_W, off_tri, off_part_t, off_T2g, off_W8;
   size_t off_Vb, off_Vpb, off_T2mb, off_part_mo, off_R, off_part_p, off_part_h;
   size_t total;
   int S_p, S_h;
@@ -230,6 +230,8 @@ static Layout make_layout(int o, int v, int vjp, int prec) {
   L.off_Vp = take((size_t)o * v3);
   L.off_T2m = take((size_t)o * o * v2);
   L.off_OOt = take((size_t)o * o * o * v);
+  L.off_OVS = take((size_t)o * o * v2);
+  L.off_T2S = take((size_t)o * o * v2);
   L.off_W = take(v3);
   L.off_tri = take((size_t)o * o * o);
   L.off_part_t = take(kGmax);
@@ -261,7 +263,7 @@ struct Ctx {
   double *gW;         // Wbar as a GEMM operand (= W, or its packed copy)
   hipStream_t s;
   const double *mo, *t1, *t2, *ovvv, *ovoo, *ovov, *fvo;
-  double *Vp, *T2m, *OOt, *W, *tri, *part_t;
+  double *Vp, *T2m, *OOt, *OVS, *T2S, *W, *tri, *part_t;
   double *Vb, *Vpb, *T2mb, *part_mo, *R, *part_p, *part_h;
   int S_p, S_h, G;
   double ebar;
@@ -342,8 +344,8 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
   vr.Vb = c.Vb;
   vr.t1 = c.t1;
   vr.fvo = c.fvo;
-  vr.ovov = c.ovov;
-  vr.t2 = c.t2;
+  vr.ovs = c.OVS;
+  vr.t2s = c.T2S;
   vr.o = c.o;
   vr.v = c.v;
   vr.i = i;
@@ -481,6 +483,8 @@ static int run(Ctx &c) {
   g_prec = c.prec;
   const int bf = c.prec == PAMD_PREC_BF16X3;
   PAMD_CHECK(launch_prepare(c.ovvv, c.packed, c.t2, c.ovoo, o, v, c.Vp, c.T2m, c.OOt, bf, c.s));
+  PAMD_CHECK(launch_prep_sym(c.ovov, o, v, 0, c.OVS, c.s));
+  PAMD_CHECK(launch_prep_sym(c.t2, o, v, 1, c.T2S, c.s));
   if (bf)
     PAMD_CHECK(launch_split_bf16x3(c.t2, (size_t)o * o * v * v, (unsigned long long *)c.gT2, c.s));
   PAMD_CHECK(launch_fill(c.tri, (size_t)o * o * o, 0.0, c.s));
@@ -513,8 +517,8 @@ static int run(Ctx &c) {
         oa.W = c.W;
         oa.Vb = c.Vb;
         oa.t1 = c.t1;
-        oa.t2 = c.t2;
-        oa.ovov = c.ovov;
+        oa.ovs = c.OVS;
+        oa.t2s = c.T2S;
         oa.fvo = c.fvo;
         oa.mo = c.mo;
         oa.o = o;
@@ -572,6 +576,8 @@ static void bind_workspace(Ctx &c, const Layout &L, void *ws) {
   c.Vp = (double *)(b + L.off_Vp);
   c.T2m = (double *)(b + L.off_T2m);
   c.OOt = (double *)(b + L.off_OOt);
+  c.OVS = (double *)(b + L.off_OVS);
+  c.T2S = (double *)(b + L.off_T2S);
   c.W = (double *)(b + L.off_W);
   c.tri = (double *)(b + L.off_tri);
   c.part_t = (double *)(b + L.off_part_t);

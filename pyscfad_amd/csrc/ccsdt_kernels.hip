@@ -141,6 +141,42 @@ hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, cons
   return prepare_as<double>(ovvv, packed, t2, ovoo, o, v, Vp, T2m, OOt, s);
 }
 
+// Pair-symmetrised blocks, dst[x][y][p][q] = src(x,p,y,q) + src(y,q,x,p), for
+//   ovov (kind 0): src(x,p,y,q) = ovov[x,p,y,q]  ->  OVS
+//   t2   (kind 1): src(x,p,y,q) = t2[x,y,p,q]    ->  T2S
+// so that the V terms of the epilogue and of its adjoint read them contiguous
+// in the fastest virtual index.  One thread per output element; the second
+// read is strided (a one-time pass).
+__global__ void k_prep_sym(const double *__restrict__ src, int o, int v, int kind,
+                           double *__restrict__ dst) {
+  const size_t n = (size_t)o * o * v * v;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int q = (int)(t % v);
+    const size_t r = t / v;
+    const int p = (int)(r % v);
+    const size_t xy = r / v;
+    const int y = (int)(xy % o), x = (int)(xy / o);
+    double a, b;
+    if (kind == 0) {
+      a = src[(((size_t)x * v + p) * o + y) * v + q];
+      b = src[(((size_t)y * v + q) * o + x) * v + p];
+    } else {
+      a = src[(((size_t)x * o + y) * v + p) * v + q];
+      b = src[(((size_t)y * o + x) * v + q) * v + p];
+    }
+    dst[t] = a + b;
+  }
+}
+
+hipError_t launch_prep_sym(const double *src, int o, int v, int kind, double *dst, hipStream_t s) {
+  const size_t n = (size_t)o * o * v * v;
+  size_t b = (n + 255) / 256;
+  if (b > 262144) b = 262144;
+  hipLaunchKernelGGL(k_prep_sym, dim3((unsigned)b), dim3(256), 0, s, src, o, v, kind, dst);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // FP64 MFMA contraction engine
 //
@@ -395,9 +431,8 @@ __global__ __launch_bounds__(256) void k_orbit(OrbitArgs A) {
   double eacc = 0.0, occ = 0.0;
   const size_t vv = (size_t)v * v;
   // slab bases
-  const double *t1 = A.t1, *ovov = A.ovov, *t2 = A.t2, *fvo = A.fvo;
-#define OVOV(x, p, y, q) ovov[(((size_t)(x) * v + (p)) * o + (y)) * v + (q)]
-#define T2(x, y, p, q) t2[(((size_t)(x) * o + (y)) * v + (p)) * v + (q)]
+  const double *t1 = A.t1, *ovs = A.ovs, *t2s = A.t2s, *fvo = A.fvo;
+#define SYM(arr, x, y, p, q) arr[(((size_t)(x) * o + (y)) * v + (p)) * v + (q)]
 
   for (long long q = blockIdx.x; q < A.norb; q += gridDim.x) {
     if (tid == 0) {
@@ -437,12 +472,10 @@ __global__ __launch_bounds__(256) void k_orbit(OrbitArgs A) {
       if (a < v && b < v && c < v) {
         wv = A.W[(size_t)a * vv + (size_t)b * v + c];
         const double t1a = t1[i * v + a], t1b = t1[j * v + b], t1c = t1[k * v + c];
-        vv_ = (OVOV(i, a, j, b) + OVOV(j, b, i, a)) * t1c + (OVOV(i, a, k, c) + OVOV(k, c, i, a)) * t1b +
-              (OVOV(j, b, k, c) + OVOV(k, c, j, b)) * t1a;
+        vv_ = SYM(ovs, i, j, a, b) * t1c + SYM(ovs, i, k, a, c) * t1b + SYM(ovs, j, k, b, c) * t1a;
         if (A.has_fvo) {
           const double fck = fvo[c * o + k], fbj = fvo[b * o + j], fai = fvo[a * o + i];
-          vv_ += (T2(j, i, b, a) + T2(i, j, a, b)) * fck + (T2(k, i, c, a) + T2(i, k, a, c)) * fbj +
-                 (T2(k, j, c, b) + T2(j, k, b, c)) * fai;
+          vv_ += SYM(t2s, i, j, a, b) * fck + SYM(t2s, i, k, a, c) * fbj + SYM(t2s, j, k, b, c) * fai;
         }
         vv_ *= 0.5;
       }
@@ -515,8 +548,7 @@ __global__ __launch_bounds__(256) void k_orbit(OrbitArgs A) {
       pm[k] += otot;
     }
   }
-#undef OVOV
-#undef T2
+#undef SYM
 }
 
 hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s) {
@@ -559,15 +591,14 @@ __global__ __launch_bounds__(256) void k_vr_a(VRArgs A) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int o = A.o, v = A.v, i = A.i, j = A.j, k = A.k;
   const size_t vv = (size_t)v * v;
-  const double *ovov = A.ovov, *t2 = A.t2, *t1 = A.t1, *fvo = A.fvo;
-#define OVOV(x, p, y, q) ovov[(((size_t)(x) * v + (p)) * o + (y)) * v + (q)]
-#define T2(x, y, p, q) t2[(((size_t)(x) * o + (y)) * v + (p)) * v + (q)]
+  const double *ovs = A.ovs, *t2s = A.t2s, *t1 = A.t1, *fvo = A.fvo;
+#define SYM(arr, x, y, p, q) arr[(((size_t)(x) * o + (y)) * v + (p)) * v + (q)]
   const int nq = (v + 63) / 64;
   for (int c = tid; c < v; c += 256) {
     cst[0][c] = t1[k * v + c];
-    cst[2][c] = OVOV(i, a, k, c) + OVOV(k, c, i, a);
+    cst[2][c] = SYM(ovs, i, k, a, c);
     cst[1][c] = A.has_fvo ? fvo[c * o + k] : 0.0;
-    cst[3][c] = T2(k, i, c, a) + T2(i, k, a, c);
+    cst[3][c] = SYM(t2s, i, k, a, c);
   }
   __syncthreads();
   double r3[kMaxVQ], r4[kMaxVQ], pk[kMaxVQ], fk[kMaxVQ];
@@ -577,9 +608,11 @@ __global__ __launch_bounds__(256) void k_vr_a(VRArgs A) {
   double *R = A.R;
   for (int b = wave; b < v; b += 4) {
     const double t1j = t1[j * v + b];
-    const double S1 = OVOV(i, a, j, b) + OVOV(j, b, i, a);
+    const double S1 = SYM(ovs, i, j, a, b);
     const double fvj = A.has_fvo ? fvo[b * o + j] : 0.0;
-    const double T1 = T2(j, i, b, a) + T2(i, j, a, b);
+    const double T1 = SYM(t2s, i, j, a, b);
+    const double *sjk = ovs + (((size_t)j * o + k) * v + b) * v;
+    const double *tjk = t2s + (((size_t)j * o + k) * v + b) * v;
     const double *row = A.Vb + (size_t)a * vv + (size_t)b * v;
     double sr1 = 0.0, sr2 = 0.0, spj = 0.0, sfj = 0.0;
 #pragma unroll
@@ -591,11 +624,11 @@ __global__ __launch_bounds__(256) void k_vr_a(VRArgs A) {
         spj += x * cst[2][c];
         r3[qq] += x * t1j;
         pk[qq] += x * S1;
-        t1i += x * (OVOV(j, b, k, c) + OVOV(k, c, j, b));
+        t1i += x * sjk[c];
         // fvo_bar pieces depend on t2 only; the fvo-valued pieces only matter when fvo != 0
         sfj += x * cst[3][c];
         fk[qq] += x * T1;
-        fvoi += x * (T2(k, j, c, b) + T2(j, k, b, c));
+        fvoi += x * tjk[c];
         if (A.has_fvo) {
           sr2 += x * cst[1][c];
           r4[qq] += x * fvj;
@@ -624,8 +657,7 @@ __global__ __launch_bounds__(256) void k_vr_a(VRArgs A) {
     R[10 * vv + a] = t1i_s;
     R[10 * vv + v + a] = fvoi_s;
   }
-#undef OVOV
-#undef T2
+#undef SYM
 }
 
 __global__ __launch_bounds__(256) void k_vr_b(VRArgs A) {
@@ -723,23 +755,42 @@ __global__ void k_combine(CombineArgs A) {
       A.ovoo_bar[(((size_t)xs[pr] * v + p) * o + m) * o + ys[pr]] += s;
     }
   }
-  if (t < (size_t)v) {
-    const int c = (int)t;
-    double sk = 0.0, sj = 0.0, fk = 0.0, fj = 0.0;
-    for (int a = 0; a < v; ++a) {
-      sk += R[6 * vv + (size_t)a * v + c];
-      sj += R[8 * vv + (size_t)a * v + c];
-      fk += R[7 * vv + (size_t)a * v + c];
-      fj += R[9 * vv + (size_t)a * v + c];
+}
+
+// t1_bar / fvo_bar pieces of one triple: column sums over a of four v x v
+// slices of R, one block per 64 columns (the 4 waves split the rows).
+__global__ __launch_bounds__(256) void k_vr_colsums(CombineArgs A) {
+  __shared__ double red[4][4][64];
+  const int o = A.o, v = A.v, i = A.i, j = A.j, k = A.k;
+  const size_t vv = (size_t)v * v;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + lane;
+  const double *R = A.R;
+  double sk = 0.0, sj = 0.0, fk = 0.0, fj = 0.0;
+  if (c < v) {
+    for (int a = wave; a < v; a += 4) {
+      const size_t ac = (size_t)a * v + c;
+      sk += R[6 * vv + ac];
+      sj += R[8 * vv + ac];
+      fk += R[7 * vv + ac];
+      fj += R[9 * vv + ac];
     }
-    A.t1_bar[k * v + c] += 0.5 * sk;
-    A.t1_bar[j * v + c] += 0.5 * sj;
+  }
+  red[0][wave][lane] = sk;
+  red[1][wave][lane] = sj;
+  red[2][wave][lane] = fk;
+  red[3][wave][lane] = fj;
+  __syncthreads();
+  if (wave == 0 && c < v) {
+    double s[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s[q] = red[q][0][lane] + red[q][1][lane] + red[q][2][lane] + red[q][3][lane];
+    A.t1_bar[k * v + c] += 0.5 * s[0];
+    A.t1_bar[j * v + c] += 0.5 * s[1];
     A.t1_bar[i * v + c] += 0.5 * R[10 * vv + c];
-    {
-      A.fvo_bar[(size_t)c * o + k] += 0.5 * fk;
-      A.fvo_bar[(size_t)c * o + j] += 0.5 * fj;
-      A.fvo_bar[(size_t)c * o + i] += 0.5 * R[10 * vv + v + c];
-    }
+    A.fvo_bar[(size_t)c * o + k] += 0.5 * s[2];
+    A.fvo_bar[(size_t)c * o + j] += 0.5 * s[3];
+    A.fvo_bar[(size_t)c * o + i] += 0.5 * R[10 * vv + v + c];
   }
 }
 
@@ -747,6 +798,7 @@ hipError_t launch_combine(const CombineArgs &a, hipStream_t s) {
   size_t n = (size_t)a.v * a.v;
   if ((size_t)a.v * a.o > n) n = (size_t)a.v * a.o;
   hipLaunchKernelGGL(k_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_vr_colsums, dim3((unsigned)((a.v + 63) / 64)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
