@@ -59,14 +59,14 @@ def cpu_baseline(o_full, v_full, sample, threads):
 
 def engine_roofline(prof, precision):
     """Roofline of the contraction engine from the sampled hipEvent timings."""
-    gemm = [prof[k] for k in ('gemm_wbuild', 'gemm_adjoint') if prof[k]['launches']]
+    gemm = [x for k, x in prof.items() if k.startswith('gemm_') and x['launches']]
     g_ms = sum(x['ms'] for x in gemm)
     g_fl = sum(x['flops'] for x in gemm)   # FP64-equivalent: 2 M N K per launch
     g_n = sum(x['launches'] for x in gemm)
     fp64_equiv = g_fl / (g_ms * 1e-3) / 1e12 if g_ms > 0 else None
     if precision == 'bf16x3':
         # six bf16 32x32x16 MFMA products per FP64 multiply-add
-        kernel, mfma_factor, peak = 'k_gemm_bf16x3', 6.0, PEAKS['bf16']
+        kernel, mfma_factor, peak = 'k_gemm_b3', 6.0, PEAKS['bf16']
     else:
         kernel, mfma_factor, peak = 'k_gemm_f64', 1.0, PEAKS['fp64']
     achieved = fp64_equiv * mfma_factor if fp64_equiv else None
@@ -97,11 +97,14 @@ def main():
     ap.add_argument('--profile-every', type=int, default=64)
     ap.add_argument('--progress-every', type=int, default=512,
                     help='stderr progress line every N occupied triples (0: off)')
-    ap.add_argument('--cpu-sample', default='16,160', help='nocc,nvir of the CPU baseline sample')
+    ap.add_argument('--cpu-sample', default='12,120', help='nocc,nvir of the CPU baseline sample')
     ap.add_argument('--cpu-threads', type=int, default=16)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--energy-only', action='store_true', help='time E(T) without the VJP')
     ap.add_argument('--out', default=None, help='also write the JSON line to this file')
+    ap.add_argument('--calib-copy-mib', type=int, default=0,
+                    help='run one plain device copy of this many MiB before the timed region '
+                         '(reference dispatch for HBM byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU only; the (T) path would shard over '
@@ -142,6 +145,15 @@ def main():
         if args.energy_only:
             return _ops.energy(inp), None
         return _ops.energy_vjp(inp, et_bar=1.0)
+
+    if args.calib_copy_mib:
+        src = torch.empty(args.calib_copy_mib * (1 << 20) // 8, dtype=torch.float64, device=dev)
+        src.fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     for _ in range(args.warmup):
         step()

@@ -35,53 +35,278 @@ hipError_t launch_split_bf16x3(const double *x, size_t n, unsigned long long *ou
   return hipGetLastError();
 }
 
-// Gather the three bf16 planes of 8 packed elements (one per k) into MFMA
-// operand vectors.
-__device__ __forceinline__ void unpack8(const unsigned long long (&u)[8], bf16x8 &h, bf16x8 &m,
-                                        bf16x8 &l) {
-  u32x4 vh, vm, vl;
-#pragma unroll
-  for (int d = 0; d < 4; ++d) {
-    const unsigned int L0 = (unsigned int)u[2 * d], L1 = (unsigned int)u[2 * d + 1];
-    const unsigned int H0 = (unsigned int)(u[2 * d] >> 32), H1 = (unsigned int)(u[2 * d + 1] >> 32);
-    vh[d] = (L0 & 0xFFFFu) | (L1 << 16);
-    vm[d] = (L0 >> 16) | (L1 & 0xFFFF0000u);
-    vl[d] = (H0 & 0xFFFFu) | (H1 << 16);
-  }
-  h = __builtin_bit_cast(bf16x8, vh);
-  m = __builtin_bit_cast(bf16x8, vm);
-  l = __builtin_bit_cast(bf16x8, vl);
+// ---------------------------------------------------------------------------
+// Split-BF16x3 engine, K stages of 32 with planar LDS.
+//
+// Each stage stages R rows x 32 k of an operand and writes them to LDS split
+// into three planes (hi, mid, lo) of bf16, 64 bytes per row and plane, so that
+// an MFMA operand (8 consecutive k of one row) is one 16-byte LDS read per
+// plane with no unpacking.  The four 16-byte chunks of row r are stored
+// XOR-swizzled by ((r / 4) ^ (r / 16)) % 4, which keeps the 16-lane groups of
+// those reads (16 consecutive rows, one chunk) conflict-free.
+// A thread stages "quads" (4 consecutive k of one row, four 8-byte packed
+// elements), R / 64 of them per stage, and writes each as one 8-byte LDS store
+// per plane:
+//   K-contiguous operand: row = tid / 8 + 64 j, quad = tid % 8 (8 lanes cover
+//     a row's 32 k; two 16-byte loads per quad);
+//   row-contiguous operand: row = tid % 64 + 64 j, quad = tid / 64 (64 lanes
+//     read 64 consecutive rows at one k per load).
+// Accumulation: FP32 MFMA tile sums over four stages (128 k), then added into
+// FP64 accumulators.  Two stages of global loads are in flight at a time.
+// ---------------------------------------------------------------------------
+constexpr int BKC = 32;
+
+template <int R>
+struct B3Stage {
+  static constexpr int NE = R / 16;  // elements per thread per stage
+  static constexpr int NQ = R / 64;  // quads per thread per stage
+  static constexpr int PLANE = R * 64;
+  static constexpr int BYTES = 3 * PLANE;
+  static_assert(R % 64 == 0 && R <= 512, "B3 staging map");
+};
+
+__device__ __forceinline__ int b3_lds_off(int r, int chunk) {
+  return r * 64 + 16 * (chunk ^ (((r >> 2) ^ (r >> 4)) & 3));
 }
 
-// 8 consecutive-k packed elements of one operand row from the staged panel
-template <int R, int NTHR>
-__device__ __forceinline__ void read8(const unsigned long long *S, int rfast, int r, int k0,
-                                      unsigned long long (&u)[8]) {
-  using ST = OpStage<R, NTHR>;
-  if (rfast) {
+// this thread's staging map for one operand of one segment
+struct B3Rows {
+  int rfast, row0, slot;  // rows row0 + 64 j, quad index slot
+};
+
+__device__ __forceinline__ B3Rows b3_map(int rfast, int tid) {
+  B3Rows m;
+  m.rfast = rfast;
+  m.row0 = rfast ? (tid & 63) : (tid >> 3);
+  m.slot = rfast ? (tid >> 6) : (tid & 7);
+  return m;
+}
+
+template <int R>
+__device__ __forceinline__ void b3_rows(const OpDesc &d, int R_total, int r0, const B3Rows &m,
+                                        int (&roff)[B3Stage<R>::NQ],
+                                        bool (&rok)[B3Stage<R>::NQ]) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) u[e] = S[(k0 + e) * ST::LR + r];
-  } else {
-    const u32x4 *p = (const u32x4 *)(S + r * ST::LK + k0);
+  for (int j = 0; j < B3Stage<R>::NQ; ++j) {
+    const int r = r0 + m.row0 + 64 * j;
+    rok[j] = r < R_total;
+    const int q = r / d.rdiv;
+    roff[j] = rok[j] ? (int)(q * d.rx + (r - q * d.rdiv) * d.ry) : 0;
+  }
+}
+
+// K position of this thread's quad in the current stage: the element offset
+// of its first k, the step between its k (ky) and which of its k lie inside
+// the segment (bit u).  With kdiv % 4 == 0 a quad never straddles a kdiv
+// boundary (kappa0 % 32 == 0); otherwise `wrap` is set and the offsets are
+// formed one by one.
+struct B3K {
+  int off, step;
+  unsigned kmask;
+  bool wrap;
+};
+
+__device__ __forceinline__ int b3_koff(const OpDesc &d, KPos kp, int add) {
+  int r = kp.r + add, q = kp.q;
+  while (r >= d.kdiv) {
+    r -= d.kdiv;
+    ++q;
+  }
+  return q * (int)d.kx + r * (int)d.ky;
+}
+
+__device__ __forceinline__ B3K b3_kpos(const OpDesc &d, int L, int kappa0, KPos kp,
+                                       const B3Rows &m) {
+  B3K k;
+  const int add = 4 * m.slot;
+  k.off = b3_koff(d, kp, add);
+  k.step = (int)d.ky;
+  const int rem = L - (kappa0 + add);
+  k.kmask = rem >= 4 ? 15u : (rem <= 0 ? 0u : (1u << rem) - 1u);
+  k.wrap = (d.kdiv & 3) != 0;
+  return k;
+}
+
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+template <int R>
+__device__ __forceinline__ void b3_fetch(const OpDesc &d, const B3Rows &m,
+                                         const int (&roff)[B3Stage<R>::NQ],
+                                         const bool (&rok)[B3Stage<R>::NQ], const B3K &k,
+                                         KPos kp, bool vec, unsigned long long (&v)[B3Stage<R>::NE],
+                                         unsigned &rowmask, unsigned &kmask) {
+  const unsigned long long *p = reinterpret_cast<const unsigned long long *>(d.p);
+  unsigned rm = 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const u32x4 w = p[q];
-      u[2 * q] = (unsigned long long)w[0] | ((unsigned long long)w[1] << 32);
-      u[2 * q + 1] = (unsigned long long)w[2] | ((unsigned long long)w[3] << 32);
+  for (int j = 0; j < B3Stage<R>::NQ; ++j) rm |= rok[j] ? (1u << j) : 0u;
+  rowmask = rm;
+  kmask = k.kmask;
+#pragma unroll
+  for (int j = 0; j < B3Stage<R>::NQ; ++j) {
+    const unsigned msk = rok[j] ? k.kmask : 0u;
+    const int base = roff[j] + k.off;
+    if (msk == 15u && vec) {
+      // four consecutive elements, 16-byte aligned (checked per segment)
+      const u64x2 *q2 = reinterpret_cast<const u64x2 *>(p + base);
+      const u64x2 x0 = q2[0], x1 = q2[1];
+      v[4 * j] = x0[0];
+      v[4 * j + 1] = x0[1];
+      v[4 * j + 2] = x1[0];
+      v[4 * j + 3] = x1[1];
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        int o = 0;
+        if ((msk >> u) & 1u) o = k.wrap ? roff[j] + b3_koff(d, kp, 4 * m.slot + u) : base + u * k.step;
+        v[4 * j + u] = p[o];
+      }
     }
   }
 }
 
-// Tile MT x NT with WM x WN waves of 32 x 32 (one 32x32x16 MFMA tile each).
+// a segment's operand qualifies for 16-byte loads of whole quads
+__device__ __forceinline__ bool b3_vec_ok(const OpDesc &d) {
+  return !d.rfast && d.ky == 1 && ((d.rx | d.ry | d.kx) & 1) == 0 && (d.kdiv & 3) == 0 &&
+         ((unsigned long long)d.p & 15ull) == 0;
+}
+
+// hi / mid / lo planes of 2 packed elements (one dword each)
+__device__ __forceinline__ void b3_planes2(unsigned long long w0, unsigned long long w1,
+                                           unsigned &hi, unsigned &mid, unsigned &lo) {
+  const unsigned a0 = (unsigned)w0, a1 = (unsigned)w1;
+  hi = __builtin_amdgcn_perm(a1, a0, 0x05040100u);
+  mid = __builtin_amdgcn_perm(a1, a0, 0x07060302u);
+  lo = __builtin_amdgcn_perm((unsigned)(w1 >> 32), (unsigned)(w0 >> 32), 0x05040100u);
+}
+
+template <int R>
+__device__ __forceinline__ void b3_stash(const B3Rows &m, bool neg,
+                                         const unsigned long long (&v)[B3Stage<R>::NE],
+                                         unsigned rowmask, unsigned kmask, char *S) {
+  typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+  const unsigned long long flip = neg ? 0x0000800080008000ull : 0ull;
+#pragma unroll
+  for (int j = 0; j < B3Stage<R>::NQ; ++j) {
+    unsigned long long w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[u] = v[4 * j + u] ^ flip;
+    const unsigned okj = ((rowmask >> j) & 1u) ? kmask : 0u;
+    if (okj != 15u) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) w[u] = ((okj >> u) & 1u) ? w[u] : 0ull;
+    }
+    unsigned h0, m0, l0, h1, m1, l1;
+    b3_planes2(w[0], w[1], h0, m0, l0);
+    b3_planes2(w[2], w[3], h1, m1, l1);
+    const u32x2 hi = {h0, h1}, mid = {m0, m1}, lo = {l0, l1};
+    const int r = m.row0 + 64 * j;
+    const int off = b3_lds_off(r, m.slot >> 1) + 8 * (m.slot & 1);
+    *(u32x2 *)(S + off) = hi;
+    *(u32x2 *)(S + B3Stage<R>::PLANE + off) = mid;
+    *(u32x2 *)(S + 2 * B3Stage<R>::PLANE + off) = lo;
+  }
+}
+
+__device__ __forceinline__ bf16x8 b3_read(const char *S, int plane_bytes, int plane, int r, int chunk) {
+  const u32x4 w = *(const u32x4 *)(S + plane * plane_bytes + b3_lds_off(r, chunk));
+  return __builtin_bit_cast(bf16x8, w);
+}
+
+// Operand data of one stage held in registers between its global loads and
+// its LDS stash, with the staging map and sign it was fetched under.
+template <int MT, int NT>
+struct B3Set {
+  unsigned long long va[B3Stage<MT>::NE], vb[B3Stage<NT>::NE];
+  unsigned rma, kma, rmb, kmb;  // valid rows (bit per group) and valid k of the group
+  B3Rows mA, mB;
+  bool neg;
+};
+
+// The stage producer: walks the K stages of this workgroup's split in order,
+// across segment boundaries, and fetches one stage per call.
+template <int MT, int NT>
+struct B3Producer {
+  const GemmDesc *d;
+  int tid, m0, n0, gp, g1;
+  int fs, FL, Fend, kappa;
+  OpDesc FA, FB;
+  bool Fneg, vA, vB;
+  KPos ka, kb;
+  B3Rows mA, mB;
+  int ra[B3Stage<MT>::NQ], rb[B3Stage<NT>::NQ];
+  bool raok[B3Stage<MT>::NQ], rbok[B3Stage<NT>::NQ];
+
+  __device__ __forceinline__ void segment(int f) {
+    fs = f;
+    FA = d->seg[fs].A;
+    FB = d->seg[fs].B;
+    FL = d->seg[fs].L;
+    Fend = d->chunk_start[fs + 1];
+    Fneg = d->seg[fs].alpha < 0.0;
+    mA = b3_map(FA.rfast, tid);
+    mB = b3_map(FB.rfast, tid);
+    b3_rows<MT>(FA, d->M, m0, mA, ra, raok);
+    b3_rows<NT>(FB, d->N, n0, mB, rb, rbok);
+    vA = b3_vec_ok(FA);
+    vB = b3_vec_ok(FB);
+  }
+
+  __device__ __forceinline__ void start(const GemmDesc &dd, int t, int mm0, int nn0, int g0, int gg1) {
+    d = &dd;
+    tid = t;
+    m0 = mm0;
+    n0 = nn0;
+    gp = g0;
+    g1 = gg1;
+    int f = 0;
+    while (f + 1 < dd.nseg && g0 >= dd.chunk_start[f + 1]) ++f;
+    segment(f);
+    kappa = (g0 - dd.chunk_start[f]) * BKC;
+    ka = KPos{kappa / FA.kdiv, kappa - (kappa / FA.kdiv) * FA.kdiv};
+    kb = KPos{kappa / FB.kdiv, kappa - (kappa / FB.kdiv) * FB.kdiv};
+  }
+
+  __device__ __forceinline__ void produce(B3Set<MT, NT> &S) {
+    if (gp >= g1) return;
+    const B3K kA = b3_kpos(FA, FL, kappa, ka, mA), kB = b3_kpos(FB, FL, kappa, kb, mB);
+    b3_fetch<MT>(FA, mA, ra, raok, kA, ka, vA, S.va, S.rma, S.kma);
+    b3_fetch<NT>(FB, mB, rb, rbok, kB, kb, vB, S.vb, S.rmb, S.kmb);
+    S.mA = mA;
+    S.mB = mB;
+    S.neg = Fneg;
+    ++gp;
+    if (gp >= g1) return;
+    if (gp >= Fend) {
+      segment(fs + 1);
+      kappa = 0;
+      ka = KPos{0, 0};
+      kb = KPos{0, 0};
+    } else {
+      kappa += BKC;
+      ka.r += BKC;
+      while (ka.r >= FA.kdiv) {
+        ka.r -= FA.kdiv;
+        ++ka.q;
+      }
+      kb.r += BKC;
+      while (kb.r >= FB.kdiv) {
+        kb.r -= FB.kdiv;
+        ++kb.q;
+      }
+    }
+  }
+};
+
 template <int MT, int NT, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void k_gemm_bf16x3(GemmDesc d, int n_fast) {
-  constexpr int NTHR = 64 * WM * WN;
-  static_assert(MT == 32 * WM && NT == 32 * WN, "one 32x32 MFMA tile per wave");
-  using SA = OpStage<MT, NTHR>;
-  using SB = OpStage<NT, NTHR>;
-  typedef unsigned long long u64;
-  __shared__ __align__(16) u64 As[2][SA::LDS];
-  __shared__ __align__(16) u64 Bs[2][SB::LDS];
+__global__ __launch_bounds__(512) void k_gemm_b3(GemmDesc d, int n_fast) {
+  static_assert(WM * WN == 8, "8 waves");
+  constexpr int WTM = MT / WM, WTN = NT / WN;
+  constexpr int TM = WTM / 32, TN = WTN / 32;
+  static_assert(TM * 32 == WTM && TN * 32 == WTN, "32x32 MFMA tiles");
+  using SA = B3Stage<MT>;
+  using SB = B3Stage<NT>;
+  __shared__ __align__(16) char As[2][SA::BYTES];
+  __shared__ __align__(16) char Bs[2][SB::BYTES];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;
@@ -97,125 +322,141 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_bf16x3(GemmDesc d, int n_
   const int g0 = split * cps;
   const int g1 = min(total, g0 + cps);
 
-  double acc[16];
+  double acc[TM][TN][16];
+  f32x16 fa[TM][TN];  // FP32 tile sums (all six products, smallest first)
 #pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.0;
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        acc[a][b][r] = 0.0;
+        fa[a][b][r] = 0.f;
+      }
 
-  int fs = 0;
-  while (fs + 1 < d.nseg && g0 >= d.chunk_start[fs + 1]) ++fs;
-  OpDesc FA = d.seg[fs].A, FB = d.seg[fs].B;
-  int FL = d.seg[fs].L, Fend = d.chunk_start[fs + 1];
-  double Falpha = d.seg[fs].alpha;
-  int kappa = (g0 - d.chunk_start[fs]) * GKC;
-  KPos ka{kappa / FA.kdiv, kappa - (kappa / FA.kdiv) * FA.kdiv};
-  KPos kb{kappa / FB.kdiv, kappa - (kappa / FB.kdiv) * FB.kdiv};
-  int ra_off[SA::E], rb_off[SB::E];
-  bool ra_ok[SA::E], rb_ok[SB::E];
-  u64 va[SA::E], vb[SB::E];
-  bool oka[SA::E], okb[SB::E];
-  op_rows<MT, NTHR>(FA, d.M, m0, tid, ra_off, ra_ok);
-  op_rows<NT, NTHR>(FB, d.N, n0, tid, rb_off, rb_ok);
-  if (g0 < g1) {
-    op_fetch<MT, NTHR>(FA, FL, kappa, ka, tid, ra_off, ra_ok, va, oka);
-    op_fetch<NT, NTHR>(FB, FL, kappa, kb, tid, rb_off, rb_ok, vb, okb);
-  }
-  int la = FA.rfast, lb = FB.rfast;
-  double alpha = Falpha;
-  int buf = 0;
-  const int ar = wm * 32 + (lane & 31), br = wn * 32 + (lane & 31), k0 = 8 * (lane >> 5);
-  for (int g = g0; g < g1; ++g) {
-    op_stash<MT, NTHR>(la, alpha, tid, va, oka, As[buf]);
-    op_stash<NT, NTHR>(lb, 1.0, tid, vb, okb, Bs[buf]);
+  // two stages of loads in flight: stage g is stashed while g + 1 is landing
+  // and g + 2 is issued
+  B3Producer<MT, NT> P;
+  P.start(d, tid, m0, n0, g0, g1);
+  B3Set<MT, NT> S0, S1;
+  P.produce(S0);
+  P.produce(S1);
+  int buf = 0, nfl = 0;
+  constexpr int kFlush = 4;
+
+  auto consume = [&](B3Set<MT, NT> &S, int g) {
+    b3_stash<MT>(S.mA, S.neg, S.va, S.rma, S.kma, As[buf]);
+    b3_stash<NT>(S.mB, false, S.vb, S.rmb, S.kmb, Bs[buf]);
     __syncthreads();
-    const int cla = la, clb = lb;
-    if (g + 1 < g1) {
-      if (g + 1 >= Fend) {
-        ++fs;
-        FA = d.seg[fs].A;
-        FB = d.seg[fs].B;
-        FL = d.seg[fs].L;
-        Fend = d.chunk_start[fs + 1];
-        Falpha = d.seg[fs].alpha;
-        kappa = 0;
-        ka = KPos{0, 0};
-        kb = KPos{0, 0};
-        op_rows<MT, NTHR>(FA, d.M, m0, tid, ra_off, ra_ok);
-        op_rows<NT, NTHR>(FB, d.N, n0, tid, rb_off, rb_ok);
-      } else {
-        kappa += GKC;
-        ka.r += GKC;
-        while (ka.r >= FA.kdiv) {
-          ka.r -= FA.kdiv;
-          ++ka.q;
-        }
-        kb.r += GKC;
-        while (kb.r >= FB.kdiv) {
-          kb.r -= FB.kdiv;
-          ++kb.q;
+    P.produce(S);
+    const char *SAb = As[buf];
+    const char *SBb = Bs[buf];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int chunk = 2 * s + (lane >> 5);
+      bf16x8 bh[TN], bm[TN], bl[TN];
+#pragma unroll
+      for (int b = 0; b < TN; ++b) {
+        const int r = wn * WTN + 32 * b + (lane & 31);
+        bh[b] = b3_read(SBb, SB::PLANE, 0, r, chunk);
+        bm[b] = b3_read(SBb, SB::PLANE, 1, r, chunk);
+        bl[b] = b3_read(SBb, SB::PLANE, 2, r, chunk);
+      }
+#pragma unroll
+      for (int a = 0; a < TM; ++a) {
+        const int r = wm * WTM + 32 * a + (lane & 31);
+        const bf16x8 ah = b3_read(SAb, SA::PLANE, 0, r, chunk);
+        const bf16x8 am = b3_read(SAb, SA::PLANE, 1, r, chunk);
+        const bf16x8 al = b3_read(SAb, SA::PLANE, 2, r, chunk);
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+          fa[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bm[b], fa[a][b], 0, 0, 0);
+          fa[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[b], fa[a][b], 0, 0, 0);
+          fa[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[b], fa[a][b], 0, 0, 0);
+          fa[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bm[b], fa[a][b], 0, 0, 0);
+          fa[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bh[b], fa[a][b], 0, 0, 0);
+          fa[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[b], fa[a][b], 0, 0, 0);
         }
       }
-      op_fetch<MT, NTHR>(FA, FL, kappa, ka, tid, ra_off, ra_ok, va, oka);
-      op_fetch<NT, NTHR>(FB, FL, kappa, kb, tid, rb_off, rb_ok, vb, okb);
-      la = FA.rfast;
-      lb = FB.rfast;
-      alpha = Falpha;
     }
-    u64 ua[8], ub[8];
-    read8<MT, NTHR>(As[buf], cla, ar, k0, ua);
-    read8<NT, NTHR>(Bs[buf], clb, br, k0, ub);
-    bf16x8 ah, am, al, bh, bm, bl;
-    unpack8(ua, ah, am, al);
-    unpack8(ub, bh, bm, bl);
-    f32x16 big, small;
+    // FP32 tile sums go into the FP64 accumulators every kFlush stages (128 k)
+    if (++nfl == kFlush || g + 1 == g1) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) big[r] = small[r] = 0.f;
-    // two independent chains, smallest terms first within each
-    small = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bm, small, 0, 0, 0);
-    big = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bm, big, 0, 0, 0);
-    small = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, small, 0, 0, 0);
-    big = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bh, big, 0, 0, 0);
-    small = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, small, 0, 0, 0);
-    big = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, big, 0, 0, 0);
+      for (int a = 0; a < TM; ++a)
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] += (double)big[r] + (double)small[r];
+        for (int b = 0; b < TN; ++b)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            acc[a][b][r] += (double)fa[a][b][r];
+            fa[a][b][r] = 0.f;
+          }
+      nfl = 0;
+    }
     buf ^= 1;
+  };
+  for (int g = g0; g < g1; g += 2) {
+    consume(S0, g);
+    if (g + 1 < g1) consume(S1, g + 1);
   }
 
-  // D layout of v_mfma_f32_32x32x16_bf16 on gfx950: lane l, element r holds
-  // row 8*(r/4) + 4*(l/32) + r%4, column l%32.
+  // D layout of v_mfma_f32_32x32x16_bf16: lane l, element r -> row 8 (r/4) + 4 (l/32) + r%4,
+  // column l%32
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int m = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-    const int n = n0 + wn * 32 + (lane & 31);
-    if (m >= d.M || n >= d.N) continue;
-    const double val = acc[r];
-    if (d.mode == kPartial) {
-      d.partial[((size_t)split * d.M + m) * d.N + n] = val;
-    } else {
-      const int qm = m / d.C.mdiv, rm = m - qm * d.C.mdiv;
-      const int qn = n / d.C.ndiv, rn = n - qn * d.C.ndiv;
-      double *c = d.C.p + (long long)qm * d.C.mx + (long long)rm * d.C.my +
-                  (long long)qn * d.C.nx + (long long)rn * d.C.ny;
-      if (d.mode == kAccumulate)
-        *c += val;
-      else
-        *c = val;
-    }
-  }
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int ml = wm * WTM + 32 * a + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int nl = wn * WTN + 32 * b + (lane & 31);
+        const bool part = d.mode == kPartial;
+        const OutPos pm = out_pos<MT>(tm, ml, part ? (1 << 30) : d.C.mdiv);
+        const OutPos pn = out_pos<NT>(tn, nl, part ? (1 << 30) : d.C.ndiv);
+        const int m = pm.idx, n = pn.idx;
+        if (m >= d.M || n >= d.N) continue;
+        const double val = acc[a][b][r];
+        if (part) {
+          d.partial[((size_t)split * d.M + m) * d.N + n] = val;
+        } else {
+          double *c = d.C.p + (long long)pm.q * d.C.mx + (long long)pm.r * d.C.my +
+                      (long long)pn.q * d.C.nx + (long long)pn.r * d.C.ny;
+          if (d.mode == kAccumulate)
+            *c += val;
+          else
+            *c = val;
+        }
+      }
 }
 
-hipError_t launch_gemm_bf16x3(const GemmDesc &d, hipStream_t s) {
-  if (d.M <= 0 || d.N <= 0) return hipSuccess;
-  if (d.M >= d.N) {
-    const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 63) / 64) * d.splits;
-    hipLaunchKernelGGL((k_gemm_bf16x3<128, 64, 4, 2>), dim3((unsigned)tiles), dim3(512), 0, s, d,
-                       1);
+// fraction of a padded MT x NT tiling that is real output
+static double tile_eff(int M, int N, int MT, int NT) {
+  const double pm = (double)((M + MT - 1) / MT) * MT, pn = (double)((N + NT - 1) / NT) * NT;
+  return (double)M * N / (pm * pn);
+}
+
+int gemm_tiles_b3(int M, int N) {
+  if (tile_eff(M, N, 256, 64) >= tile_eff(M, N, 64, 256)) return ((M + 255) / 256) * ((N + 63) / 64);
+  return ((M + 63) / 64) * ((N + 255) / 256);
+}
+
+hipError_t launch_gemm_b3(const GemmDesc &d0, hipStream_t s) {
+  if (d0.M <= 0 || d0.N <= 0) return hipSuccess;
+  GemmDesc d = d0;  // chunks of BKC
+  d.chunk_start[0] = 0;
+  for (int q = 0; q < d.nseg; ++q) d.chunk_start[q + 1] = d.chunk_start[q] + (d.seg[q].L + BKC - 1) / BKC;
+  for (int q = d.nseg + 1; q <= kMaxSeg; ++q) d.chunk_start[q] = d.chunk_start[d.nseg];
+  const double e_tall = tile_eff(d.M, d.N, 256, 64), e_wide = tile_eff(d.M, d.N, 64, 256);
+  const int n_fast = d.M >= d.N ? 1 : 0;
+  if (e_tall >= e_wide) {
+    const long long tiles = (long long)((d.M + 255) / 256) * ((d.N + 63) / 64) * d.splits;
+    hipLaunchKernelGGL((k_gemm_b3<256, 64, 4, 2>), dim3((unsigned)tiles), dim3(512), 0, s, d, n_fast);
   } else {
-    const long long tiles = (long long)((d.M + 63) / 64) * ((d.N + 127) / 128) * d.splits;
-    hipLaunchKernelGGL((k_gemm_bf16x3<64, 128, 2, 4>), dim3((unsigned)tiles), dim3(512), 0, s, d,
-                       0);
+    const long long tiles = (long long)((d.M + 63) / 64) * ((d.N + 255) / 256) * d.splits;
+    hipLaunchKernelGGL((k_gemm_b3<64, 256, 2, 4>), dim3((unsigned)tiles), dim3(512), 0, s, d, n_fast);
   }
   return hipGetLastError();
 }
+
+hipError_t launch_gemm_bf16x3(const GemmDesc &d, hipStream_t s) { return launch_gemm_b3(d, s); }
 
 }  // namespace pamd

@@ -58,11 +58,13 @@ struct GemmDesc {
 
 // Epilogue over orbits of virtual tiles for one occupied triple.
 struct OrbitArgs {
-  double *W;            // v^3; W in, Wbar out (vjp)
+  double *W;            // v^3; W in (and Wbar out when Wb8 is null, vjp)
+  unsigned long long *Wb8;  // v^3 packed bf16 triplets of Wbar (split-BF16x3 vjp) or null
   double *Vb;           // v^3; Vbar out (vjp)
   // OVS[x][y][p][q] = ovov[x,p,y,q] + ovov[y,q,x,p], T2S[x][y][p][q] = t2[x,y,p,q] + t2[y,x,q,p]
   const double *t1, *ovs, *t2s, *fvo, *mo;
-  int o, v, i, j, k, has_fvo, vjp, nt;
+  const unsigned short *tab;  // [norb][4] orbit table (launch_orbit_table)
+  int o, v, i, j, k, has_fvo, vjp;
   long long norb;
   double omega;         // 2 s_ijk: 2, 1 (two indices equal)
   double ebar;          // cotangent of E
@@ -80,8 +82,8 @@ struct VRArgs {
 // Per-triple gather of the adjoint pieces into the output cotangents.
 struct CombineArgs {
   const double *R;                // VR scratch
-  const double *part_p;           // [6][S][v][v] particle t2_bar partials
-  const double *part_h;           // [6][S][v][o] hole ovoo_bar partials
+  double *part_p;                 // [6][S][v][v] particle t2_bar partials (reduced in place)
+  double *part_h;                 // [6][S][v][o] hole ovoo_bar partials (reduced in place)
   int S_p, S_h;
   int o, v, i, j, k, has_fvo;
   double *t2_bar, *ovov_bar, *ovoo_bar, *t1_bar, *fvo_bar;
@@ -89,6 +91,7 @@ struct CombineArgs {
 
 // host-side launchers (ccsdt_kernels.hip)
 hipError_t launch_fill(double *p, size_t n, double val, hipStream_t s);
+hipError_t launch_orbit_table(long long norb, unsigned short *tab, hipStream_t s);
 hipError_t launch_prep_sym(const double *src, int o, int v, int kind, double *dst, hipStream_t s);
 hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, const double *ovoo,
                           int o, int v, double *Vp, double *T2m, double *OOt, int split_bf16,
@@ -96,7 +99,8 @@ hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, cons
 hipError_t launch_split_bf16x3(const double *x, size_t n, unsigned long long *out, hipStream_t s);
 hipError_t launch_gemm_bf16x3(const GemmDesc &d, hipStream_t s);
 hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s);
-int gemm_tiles(int M, int N);  // workgroup tiles of one split
+int gemm_tiles(int M, int N);     // FP64 engine: workgroup tiles of one split
+int gemm_tiles_b3(int M, int N);  // split-BF16x3 engine: workgroup tiles of one split
 hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s);
 hipError_t launch_vr(const VRArgs &a, hipStream_t s);
 hipError_t launch_combine(const CombineArgs &a, hipStream_t s);

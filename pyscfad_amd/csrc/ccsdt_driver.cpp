@@ -38,8 +38,21 @@ static std::vector<GemmDesc> *g_record = nullptr;
 // Sampled launch timing (pamd_profile_*): for one triple in g_prof_every, every
 // launch is bracketed by hipEvents on its own stream; the events are read after
 // the call has been queued completely.
-enum ProfKind { kProfWBuild = 0, kProfAdjGemm, kProfOrbit, kProfVR, kProfCombine, kProfNum };
-static const char *kProfNames[kProfNum] = {"gemm_wbuild", "gemm_adjoint", "orbit_epilogue",
+// GEMM kinds: W build; adjoint to ovvv (Vpb), to t2 via split-K partials
+// (particle), to ovoo (hole split-K partials), to t2 through T2mb (hole)
+enum ProfKind {
+  kProfWBuild = 0,
+  kProfAdjOvvv,
+  kProfAdjT2P,
+  kProfAdjOvoo,
+  kProfAdjT2H,
+  kProfOrbit,
+  kProfVR,
+  kProfCombine,
+  kProfNum
+};
+static const char *kProfNames[kProfNum] = {"gemm_wbuild", "gemm_adj_ovvv", "gemm_adj_t2p",
+                                           "gemm_adj_ovoo", "gemm_adj_t2h", "orbit_epilogue",
                                            "vbar_reduce", "combine"};
 struct ProfRec {
   int kind;
@@ -199,7 +212,7 @@ static GemmDesc gemm(int M, int N, int mode, std::initializer_list<SegDesc> segs
 }
 
 struct Layout {
-  size_t off_Vp, off_T2m, off_OOt, off_OVS, off_T2S, off_W, off_tri, off_part_t, off_T2g, off_W8;
+  size_t off_Vp, off_T2m, off_OOt, off_OVS, off_T2S, off_orb, off_W, off_tri, off_part_t, off_T2g, off_W8;
   size_t off_Vb, off_Vpb, off_T2mb, off_part_mo, off_R, off_part_p, off_part_h;
   size_t total;
   int S_p, S_h;
@@ -207,11 +220,27 @@ struct Layout {
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// split-K factor giving ~6 workgroups per CU (two resident per CU at a time)
-static int splits_for(int tiles, int chunks) {
-  int s = (1536 + tiles - 1) / std::max(1, tiles);
-  s = std::max(1, std::min(s, 64));
-  return std::min(s, std::max(1, chunks));
+static int device_cus() {
+  static int cus = [] {
+    int dev = 0, n = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      n = prop.multiProcessorCount;
+    return n > 0 ? n : 256;
+  }();
+  return cus;
+}
+
+// Split-K factor for an M x N output with `chunks` K chunks of 16: three full
+// rounds of resident workgroups (two per CU for the FP64 engine, one for the
+// split-BF16x3 engine), rounded down so that the last round is not a tail.
+static int splits_for(int M, int N, int chunks, int prec) {
+  const bool b3 = prec == PAMD_PREC_BF16X3;
+  const int tiles = std::max(1, b3 ? gemm_tiles_b3(M, N) : gemm_tiles(M, N));
+  const int slots = (b3 ? 1 : 2) * device_cus();
+  int s = 3 * slots / tiles;
+  s = std::max(1, std::min(s, 512));
+  return std::min(s, std::max(1, b3 ? chunks / 2 : chunks));
 }
 
 // Vp, T2m and OOt hold FP64 copies, or packed bf16 triplets for split-BF16x3
@@ -232,6 +261,10 @@ static Layout make_layout(int o, int v, int vjp, int prec) {
   L.off_OOt = take((size_t)o * o * o * v);
   L.off_OVS = take((size_t)o * o * v2);
   L.off_T2S = take((size_t)o * o * v2);
+  {
+    const size_t nt = (v + 7) / 8;
+    L.off_orb = take(nt * (nt + 1) * (nt + 2) / 6);  // 4 x uint16 per orbit = one double
+  }
   L.off_W = take(v3);
   L.off_tri = take((size_t)o * o * o);
   L.off_part_t = take(kGmax);
@@ -239,11 +272,9 @@ static Layout make_layout(int o, int v, int vjp, int prec) {
     L.off_T2g = take((size_t)o * o * v2);
     if (vjp) L.off_W8 = take(v3);
   }
-  const int tiles_p = gemm_tiles(v, v);
-  const int tiles_h = gemm_tiles(v, o);
   const int chunks = (int)((v2 + KC - 1) / KC);
-  L.S_p = splits_for(tiles_p, chunks);
-  L.S_h = splits_for(tiles_h, chunks);
+  L.S_p = splits_for(v, v, chunks, prec);
+  L.S_h = splits_for(v, o, chunks, prec);
   if (vjp) {
     L.off_Vb = take(v3);
     L.off_Vpb = take((size_t)o * v3);
@@ -260,6 +291,7 @@ static Layout make_layout(int o, int v, int vjp, int prec) {
 struct Ctx {
   int o, v, packed, has_fvo, vjp, prec;
   const double *gT2;  // t2 as a GEMM operand (FP64, or packed bf16 triplets)
+  unsigned short *orb;  // orbit table
   double *gW;         // Wbar as a GEMM operand (= W, or its packed copy)
   hipStream_t s;
   const double *mo, *t1, *t2, *ovvv, *ovoo, *ovov, *fvo;
@@ -364,21 +396,21 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
                       {seg(opR(Wb, v2, v, V, 1, 0), op1(T2(k, j), 1, v, 1), V, 1.0),
                        seg(opR(Wb, v2, 1, V, v, 1), op1(T2(j, k), 1, v, 1), V, 1.0)},
                       out(Vpb(i), v2, v, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, kProfAdjGemm));
+    PAMD_CHECK(run_gemm(d, c.s, kProfAdjOvvv));
   }
   {  // Vpb[j][b][a|c][f]: P3 (K=c), P4 (K=a)
     GemmDesc d = gemm(V * V, V, kAccumulate,
                       {seg(opR(Wb, v, v2, V, 1, 0), op1(T2(k, i), 1, v, 1), V, 1.0),
                        seg(opR(Wb, v, 1, V, v2, 1), op1(T2(i, k), 1, v, 1), V, 1.0)},
                       out(Vpb(j), v2, v, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, kProfAdjGemm));
+    PAMD_CHECK(run_gemm(d, c.s, kProfAdjOvvv));
   }
   {  // Vpb[k][c][a|b][f]: rows m = y*v + x with x = c: P5 (K=b), P6 (K=a)
     GemmDesc d = gemm(V * V, V, kAccumulate,
                       {seg(opR(Wb, v2, 1, V, v, 1), op1(T2(j, i), 1, v, 1), V, 1.0),
                        seg(opR(Wb, v, 1, V, v2, 1), op1(T2(i, j), 1, v, 1), V, 1.0)},
                       out(Vpb(k), v, v2, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, kProfAdjGemm));
+    PAMD_CHECK(run_gemm(d, c.s, kProfAdjOvvv));
   }
 
   // particle t2_bar pieces: split-K over the virtual pair, partials per problem
@@ -404,7 +436,7 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
         gemm(V, V, kPartial, {seg(opK(Wb, v2, v, 1, V, 0), opK(Vp(k), 1, v, v2, V, 1), V * V, 1.0)},
              out(nullptr, 0, 0, 1, 0, 0, 1), c.S_p, c.part_p + 5 * pp),
     };
-    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(ds[q], c.s, kProfAdjGemm));
+    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(ds[q], c.s, kProfAdjT2P));
   }
   // hole ovoo_bar pieces: split-K over the virtual pair
   const size_t ph = (size_t)c.S_h * v * o;
@@ -429,7 +461,7 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
         gemm(V, O, kPartial, {seg(opK(Wb, 1, v2, v, V, 1), opK(T2m(i), 1, vo, o, V, 1), V * V, -1.0)},
              out(nullptr, 0, 0, 1, 0, 0, 1), c.S_h, c.part_h + 5 * ph),
     };
-    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(ds[q], c.s, kProfAdjGemm));
+    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(ds[q], c.s, kProfAdjOvoo));
   }
   // hole-term t2 reads: T2mb[x][p][q][m] (= dE/d t2[x,m,p,q])
   {  // T2mb[i]: H4 (p=a,q=c,K=b), H6 (p=a,q=b,K=c)
@@ -437,21 +469,21 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
                       {seg(opR(Wb, v2, 1, V, v, 1), op1(OOt(j, k), 1, o, 1), V, -1.0),
                        seg(opR(Wb, v2, v, V, 1, 0), op1(OOt(k, j), 1, o, 1), V, -1.0)},
                       out(T2mb(i), vo, o, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, kProfAdjGemm));
+    PAMD_CHECK(run_gemm(d, c.s, kProfAdjT2H));
   }
   {  // T2mb[j]: H2 (p=b,q=c,K=a), H5 (p=b,q=a,K=c)
     GemmDesc d = gemm(V * V, O, kAccumulate,
                       {seg(opR(Wb, v, 1, V, v2, 1), op1(OOt(i, k), 1, o, 1), V, -1.0),
                        seg(opR(Wb, v, v2, V, 1, 0), op1(OOt(k, i), 1, o, 1), V, -1.0)},
                       out(T2mb(j), vo, o, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, kProfAdjGemm));
+    PAMD_CHECK(run_gemm(d, c.s, kProfAdjT2H));
   }
   {  // T2mb[k]: rows m = q*v + p with p = c: H1 (q=b,K=a), H3 (q=a,K=b)
     GemmDesc d = gemm(V * V, O, kAccumulate,
                       {seg(opR(Wb, v, 1, V, v2, 1), op1(OOt(i, j), 1, o, 1), V, -1.0),
                        seg(opR(Wb, v2, 1, V, v, 1), op1(OOt(j, i), 1, o, 1), V, -1.0)},
                       out(T2mb(k), o, vo, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, kProfAdjGemm));
+    PAMD_CHECK(run_gemm(d, c.s, kProfAdjT2H));
   }
   // gather into the cotangents
   CombineArgs ca;
@@ -490,6 +522,7 @@ static int run(Ctx &c) {
   PAMD_CHECK(launch_fill(c.tri, (size_t)o * o * o, 0.0, c.s));
   const int nt = (v + 7) / 8;
   const long long norb = (long long)nt * (nt + 1) * (nt + 2) / 6;
+  PAMD_CHECK(launch_orbit_table(norb, c.orb, c.s));
   c.G = std::min(orbit_grid(v), kGmax);
   if (c.vjp) {
     const size_t v2 = (size_t)v * v;
@@ -515,6 +548,8 @@ static int run(Ctx &c) {
         if (st != PAMD_OK) return st;
         OrbitArgs oa;
         oa.W = c.W;
+        oa.Wb8 = (bf && c.vjp) ? (unsigned long long *)c.gW : nullptr;
+        oa.tab = c.orb;
         oa.Vb = c.Vb;
         oa.t1 = c.t1;
         oa.ovs = c.OVS;
@@ -528,7 +563,6 @@ static int run(Ctx &c) {
         oa.k = k;
         oa.has_fvo = c.has_fvo;
         oa.vjp = c.vjp;
-        oa.nt = nt;
         oa.norb = norb;
         oa.omega = (i == j || j == k) ? 1.0 : 2.0;
         oa.ebar = c.ebar;
@@ -541,7 +575,6 @@ static int run(Ctx &c) {
         }
         PAMD_CHECK(launch_triple_energy(c.part_t, c.G, c.tri + ((size_t)i * o + j) * o + k, c.s));
         if (c.vjp) {
-          if (bf) PAMD_CHECK(launch_split_bf16x3(c.W, (size_t)v * v * v, (unsigned long long *)c.gW, c.s));
           st = adjoint(c, i, j, k);
           if (st != PAMD_OK) return st;
         }
@@ -578,6 +611,7 @@ static void bind_workspace(Ctx &c, const Layout &L, void *ws) {
   c.OOt = (double *)(b + L.off_OOt);
   c.OVS = (double *)(b + L.off_OVS);
   c.T2S = (double *)(b + L.off_T2S);
+  c.orb = (unsigned short *)(b + L.off_orb);
   c.W = (double *)(b + L.off_W);
   c.tri = (double *)(b + L.off_tri);
   c.part_t = (double *)(b + L.off_part_t);

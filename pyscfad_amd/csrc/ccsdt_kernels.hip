@@ -343,17 +343,19 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, int n_fas
     for (int y = 0; y < TN; ++y)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm * WTM + x * 16 + 4 * r + (lane >> 4);
-        const int n = n0 + wn * WTN + y * 16 + (lane & 15);
+        const int ml = wm * WTM + x * 16 + 4 * r + (lane >> 4);
+        const int nl = wn * WTN + y * 16 + (lane & 15);
+        const bool part = d.mode == kPartial;
+        const OutPos pm = out_pos<MT>(tm, ml, part ? (1 << 30) : d.C.mdiv);
+        const OutPos pn = out_pos<NT>(tn, nl, part ? (1 << 30) : d.C.ndiv);
+        const int m = pm.idx, n = pn.idx;
         if (m >= d.M || n >= d.N) continue;
         const double val = acc[x][y][r];
-        if (d.mode == kPartial) {
+        if (part) {
           d.partial[((size_t)split * d.M + m) * d.N + n] = val;
         } else {
-          const int qm = m / d.C.mdiv, rm = m - qm * d.C.mdiv;
-          const int qn = n / d.C.ndiv, rn = n - qn * d.C.ndiv;
-          double *c = d.C.p + (long long)qm * d.C.mx + (long long)rm * d.C.my +
-                      (long long)qn * d.C.nx + (long long)rn * d.C.ny;
+          double *c = d.C.p + (long long)pm.q * d.C.mx + (long long)pm.r * d.C.my +
+                      (long long)pn.q * d.C.nx + (long long)pn.r * d.C.ny;
           if (d.mode == kAccumulate)
             *c += val;
           else
@@ -404,105 +406,129 @@ int orbit_grid(int v) {
   }
   long long nt = (v + OT - 1) / OT;
   long long norb = nt * (nt + 1) * (nt + 2) / 6;
-  long long g = 2LL * cus;
+  long long g = 3LL * cus;  // three 512-thread workgroups per CU fit the LDS
   if (g > norb) g = norb;
   if (g < 1) g = 1;
   return (int)g;
 }
 
-__global__ __launch_bounds__(256) void k_orbit(OrbitArgs A) {
+// Distinct tiles of an orbit and where the five non-identity permutations of a
+// position land, for the four equality patterns of A >= B >= C: 0 all distinct,
+// 1 A == B > C, 2 A > B == C, 3 A == B == C.  sel[p] picks the tile indices of
+// the p-th distinct tile from (A, B, C) (candidates in the order abc, acb, bac,
+// bca, cab, cba, first occurrence kept); pmap[s][p] is the tile that holds the
+// bca, cab, cba, acb, bac permutations (s = 0..4) of a position in tile p.
+struct OrbitPattern {
+  int np;
+  unsigned char sel[6][3];
+  unsigned char pmap[5][6];
+};
+__constant__ OrbitPattern kOrbitPat[4] = {
+    {6, {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}},
+     {{3, 5, 1, 4, 0, 2}, {4, 2, 5, 0, 3, 1}, {5, 3, 4, 1, 2, 0}, {1, 0, 3, 2, 5, 4}, {2, 4, 0, 5, 1, 3}}},
+    {3, {{0, 0, 2}, {0, 2, 0}, {2, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
+     {{1, 2, 0, 0, 0, 0}, {2, 0, 1, 0, 0, 0}, {2, 1, 0, 0, 0, 0}, {1, 0, 2, 0, 0, 0}, {0, 2, 1, 0, 0, 0}}},
+    {3, {{0, 1, 1}, {1, 0, 1}, {1, 1, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
+     {{2, 0, 1, 0, 0, 0}, {1, 2, 0, 0, 0, 0}, {2, 1, 0, 0, 0, 0}, {0, 2, 1, 0, 0, 0}, {1, 0, 2, 0, 0, 0}}},
+    {1, {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
+     {{0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}}},
+};
+
+// Orbit table: entry q = (A, B, C, pattern) of the q-th tile triple A >= B >= C;
+// pattern = (A == B) + 2 (B == C) selects kOrbitPat.
+__global__ void k_orbit_table(long long norb, unsigned short *tab) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= norb) return;
+  int a, b, c;
+  decode_orbit(q, a, b, c);
+  unsigned short *e = tab + 4 * q;
+  e[0] = (unsigned short)a;
+  e[1] = (unsigned short)b;
+  e[2] = (unsigned short)c;
+  e[3] = (unsigned short)((a == b ? 1 : 0) + (b == c ? 2 : 0));
+}
+
+hipError_t launch_orbit_table(long long norb, unsigned short *tab, hipStream_t s) {
+  hipLaunchKernelGGL(k_orbit_table, dim3((unsigned)((norb + 255) / 256)), dim3(256), 0, s, norb, tab);
+  return hipGetLastError();
+}
+
+// The orbit epilogue.  A workgroup takes one orbit of 8x8x8 virtual tiles at a
+// time: the distinct tiles (P1, P2, P3) among the permutations of (A, B, C).
+// Phase 1 loads W of those tiles into LDS and forms X = W + V there; phase 2
+// evaluates, for every element of every tile, r3 over the six permuted
+// positions (read from LDS), the energy term, Wbar, Vbar and the energy
+// denominators' adjoint.  Thread t handles position t of every tile, so wave w
+// holds the slab a = 8 P1 + w: the dE/d(e_virtual) sums over two of the three
+// tile axes reduce within the wave, with a few LDS atomics per wave.
+__global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
   __shared__ double Ws[6][OT3];
-  __shared__ double Vs[6][OT3];
-  __shared__ double Ds[6][OT3];
-  __shared__ int pl[6][3];
-  __shared__ int pmap[5][6];
-  __shared__ int npl_s;
-  __shared__ double red[4];
+  __shared__ double Xs[6][OT3];
+  __shared__ double red[8];
   extern __shared__ double vir[];  // v doubles: dE/de_virtual accumulated by this workgroup
 
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int o = A.o, v = A.v, n = o + v;
   const int i = A.i, j = A.j, k = A.k;
   const double eo = A.mo[i] + A.mo[j] + A.mo[k];
   const double *ev = A.mo + o;
   const double g = A.ebar * A.omega;
-  for (int x = tid; x < v; x += 256) vir[x] = 0.0;
+  for (int x = tid; x < v; x += 512) vir[x] = 0.0;
+  __syncthreads();
 
   double eacc = 0.0, occ = 0.0;
   const size_t vv = (size_t)v * v;
-  // slab bases
   const double *t1 = A.t1, *ovs = A.ovs, *t2s = A.t2s, *fvo = A.fvo;
 #define SYM(arr, x, y, p, q) arr[(((size_t)(x) * o + (y)) * v + (p)) * v + (q)]
+  // this thread's position inside a tile
+  const int al = tid >> 6, be = (tid >> 3) & 7, ga = tid & 7;
+  const int l_bca = (be << 6) | (ga << 3) | al;
+  const int l_cab = (ga << 6) | (al << 3) | be;
+  const int l_cba = (ga << 6) | (be << 3) | al;
+  const int l_acb = (al << 6) | (ga << 3) | be;
+  const int l_bac = (be << 6) | (al << 3) | ga;
 
   for (long long q = blockIdx.x; q < A.norb; q += gridDim.x) {
-    if (tid == 0) {
-      int a, b, c;
-      decode_orbit(q, a, b, c);
-      const int cand[6][3] = {{a, b, c}, {a, c, b}, {b, a, c}, {b, c, a}, {c, a, b}, {c, b, a}};
-      int np = 0;
-      for (int s = 0; s < 6; ++s) {
-        bool dup = false;
-        for (int t = 0; t < np; ++t)
-          if (pl[t][0] == cand[s][0] && pl[t][1] == cand[s][1] && pl[t][2] == cand[s][2]) dup = true;
-        if (!dup) {
-          pl[np][0] = cand[s][0];
-          pl[np][1] = cand[s][1];
-          pl[np][2] = cand[s][2];
-          ++np;
-        }
-      }
-      npl_s = np;
-      for (int p = 0; p < np; ++p) {
-        const int P1 = pl[p][0], P2 = pl[p][1], P3 = pl[p][2];
-        const int tgt[5][3] = {{P2, P3, P1}, {P3, P1, P2}, {P3, P2, P1}, {P1, P3, P2}, {P2, P1, P3}};
-        for (int s = 0; s < 5; ++s)
-          for (int t = 0; t < np; ++t)
-            if (pl[t][0] == tgt[s][0] && pl[t][1] == tgt[s][1] && pl[t][2] == tgt[s][2])
-              pmap[s][p] = t;
-      }
-    }
-    __syncthreads();
-    const int np = npl_s;
-    for (int e = tid; e < np * OT3; e += 256) {
-      const int p = e / OT3, loc = e - p * OT3;
-      const int a = pl[p][0] * OT + (loc >> 6);
-      const int b = pl[p][1] * OT + ((loc >> 3) & 7);
-      const int c = pl[p][2] * OT + (loc & 7);
-      double wv = 0.0, vv_ = 0.0;
+    const unsigned short *ent = A.tab + 4 * q;
+    const int val[3] = {ent[0], ent[1], ent[2]};
+    const OrbitPattern &pt = kOrbitPat[ent[3]];
+    const int np = pt.np;
+    // phase 1: W and X = W + V of the orbit's tiles
+    for (int p = 0; p < np; ++p) {
+      const int a = val[pt.sel[p][0]] * OT + al;
+      const int b = val[pt.sel[p][1]] * OT + be;
+      const int c = val[pt.sel[p][2]] * OT + ga;
+      double wv = 0.0, xv = 0.0;
       if (a < v && b < v && c < v) {
         wv = A.W[(size_t)a * vv + (size_t)b * v + c];
         const double t1a = t1[i * v + a], t1b = t1[j * v + b], t1c = t1[k * v + c];
-        vv_ = SYM(ovs, i, j, a, b) * t1c + SYM(ovs, i, k, a, c) * t1b + SYM(ovs, j, k, b, c) * t1a;
+        double vv_ = SYM(ovs, i, j, a, b) * t1c + SYM(ovs, i, k, a, c) * t1b +
+                     SYM(ovs, j, k, b, c) * t1a;
         if (A.has_fvo) {
           const double fck = fvo[c * o + k], fbj = fvo[b * o + j], fai = fvo[a * o + i];
-          vv_ += SYM(t2s, i, j, a, b) * fck + SYM(t2s, i, k, a, c) * fbj + SYM(t2s, j, k, b, c) * fai;
+          vv_ += SYM(t2s, i, j, a, b) * fck + SYM(t2s, i, k, a, c) * fbj +
+                 SYM(t2s, j, k, b, c) * fai;
         }
-        vv_ *= 0.5;
+        xv = wv + 0.5 * vv_;
       }
-      Ws[p][loc] = wv;
-      Vs[p][loc] = vv_;
+      Ws[p][tid] = wv;
+      Xs[p][tid] = xv;
     }
     __syncthreads();
-    for (int e = tid; e < np * OT3; e += 256) {
-      const int p = e / OT3, loc = e - p * OT3;
-      const int al = loc >> 6, be = (loc >> 3) & 7, ga = loc & 7;
-      const int a = pl[p][0] * OT + al, b = pl[p][1] * OT + be, c = pl[p][2] * OT + ga;
+    // phase 2
+    for (int p = 0; p < np; ++p) {
+      const int Pa = val[pt.sel[p][0]], Pb = val[pt.sel[p][1]], Pc = val[pt.sel[p][2]];
+      const int a = Pa * OT + al, b = Pb * OT + be, c = Pc * OT + ga;
       double dbar = 0.0;
       if (a < v && b < v && c < v) {
-        // permuted element positions: bca, cab, cba, acb, bac
-        const int l_bca = (be << 6) | (ga << 3) | al;
-        const int l_cab = (ga << 6) | (al << 3) | be;
-        const int l_cba = (ga << 6) | (be << 3) | al;
-        const int l_acb = (al << 6) | (ga << 3) | be;
-        const int l_bac = (be << 6) | (al << 3) | ga;
-        const int p0 = pmap[0][p], p1 = pmap[1][p], p2 = pmap[2][p], p3 = pmap[3][p],
-                  p4 = pmap[4][p];
-        const double w0 = Ws[p][loc];
-        const double x0 = w0 + Vs[p][loc];
+        const int p0 = pt.pmap[0][p], p1 = pt.pmap[1][p], p2 = pt.pmap[2][p], p3 = pt.pmap[3][p],
+                  p4 = pt.pmap[4][p];
+        const double w0 = Ws[p][tid];
         const double w1 = Ws[p0][l_bca], w2 = Ws[p1][l_cab], w3 = Ws[p2][l_cba],
                      w4 = Ws[p3][l_acb], w5 = Ws[p4][l_bac];
-        const double x1 = w1 + Vs[p0][l_bca], x2 = w2 + Vs[p1][l_cab], x3 = w3 + Vs[p2][l_cba],
-                     x4 = w4 + Vs[p3][l_acb], x5 = w5 + Vs[p4][l_bac];
+        const double x0 = Xs[p][tid];
+        const double x1 = Xs[p0][l_bca], x2 = Xs[p1][l_cab], x3 = Xs[p2][l_cba],
+                     x4 = Xs[p3][l_acb], x5 = Xs[p4][l_bac];
         const double Z = 4.0 * x0 + x1 + x2 - 2.0 * (x3 + x4 + x5);
         const double D = eo - ev[a] - ev[b] - ev[c];
         const double rD = 1.0 / D;
@@ -511,37 +537,53 @@ __global__ __launch_bounds__(256) void k_orbit(OrbitArgs A) {
           const double r3W = 4.0 * w0 + w1 + w2 - 2.0 * (w3 + w4 + w5);
           const size_t idx = (size_t)a * vv + (size_t)b * v + c;
           const double vbar = g * r3W * rD;
-          A.W[idx] = g * Z * rD + vbar;
+          const double wbar = g * Z * rD + vbar;
+          if (A.Wb8)
+            A.Wb8[idx] = split3(wbar);
+          else
+            A.W[idx] = wbar;
           A.Vb[idx] = vbar;
           dbar = -g * w0 * Z * rD * rD;
           occ += dbar;
         }
       }
-      Ds[p][loc] = dbar;
+      if (A.vjp) {
+        // axis c: sum over be (lane bits 3-5); axis b: sum over ga (bits 0-2); axis a: all
+        double sc = dbar;
+        sc += __shfl_xor(sc, 8, 64);
+        sc += __shfl_xor(sc, 16, 64);
+        sc += __shfl_xor(sc, 32, 64);
+        double sb = dbar;
+        sb += __shfl_xor(sb, 1, 64);
+        sb += __shfl_xor(sb, 2, 64);
+        sb += __shfl_xor(sb, 4, 64);
+        double sa = sb;
+        sa += __shfl_xor(sa, 8, 64);
+        sa += __shfl_xor(sa, 16, 64);
+        sa += __shfl_xor(sa, 32, 64);
+        if (lane < 8 && Pc * OT + lane < v) atomicAdd(&vir[Pc * OT + lane], -sc);
+        if ((lane & 7) == 0 && b < v) atomicAdd(&vir[b], -sb);
+        if (lane == 0 && a < v) atomicAdd(&vir[a], -sa);
+      }
     }
     __syncthreads();
-    if (A.vjp) {
-      for (int t = tid; t < np * 24; t += 256) {
-        const int p = t / 24, s = (t >> 3) % 3, x = t & 7;
-        double sum = 0.0;
-        for (int y = 0; y < 8; ++y)
-          for (int z = 0; z < 8; ++z) {
-            const int loc = s == 0 ? (x << 6) | (y << 3) | z
-                                   : (s == 1 ? (y << 6) | (x << 3) | z : (y << 6) | (z << 3) | x);
-            sum += Ds[p][loc];
-          }
-        const int vi = pl[p][s] * OT + x;
-        if (vi < v) atomicAdd(&vir[vi], -sum);
-      }
-      __syncthreads();
-    }
   }
-  const double etot = block_sum256(eacc, red);
-  const double otot = block_sum256(occ, red);
+  // block sums (512 threads, deterministic order)
+  eacc = wave_sum(eacc);
+  occ = wave_sum(occ);
+  if (lane == 0) red[wave] = eacc;
+  __syncthreads();
+  double etot = 0.0;
+  for (int w = 0; w < 8; ++w) etot += red[w];
+  __syncthreads();
+  if (lane == 0) red[wave] = occ;
+  __syncthreads();
+  double otot = 0.0;
+  for (int w = 0; w < 8; ++w) otot += red[w];
   if (tid == 0) A.part_t[blockIdx.x] = etot;
   if (A.vjp) {
     double *pm = A.part_mo + (size_t)blockIdx.x * n;
-    for (int x = tid; x < v; x += 256) pm[o + x] += vir[x];
+    for (int x = tid; x < v; x += 512) pm[o + x] += vir[x];
     if (tid == 0) {
       pm[i] += otot;
       pm[j] += otot;
@@ -552,7 +594,7 @@ __global__ __launch_bounds__(256) void k_orbit(OrbitArgs A) {
 }
 
 hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_orbit, dim3(grid), dim3(256), sizeof(double) * a.v, s, a);
+  hipLaunchKernelGGL(k_orbit, dim3(grid), dim3(512), sizeof(double) * a.v, s, a);
   return hipGetLastError();
 }
 
@@ -571,6 +613,9 @@ hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s) {
 constexpr int kMaxVQ = 8;  // v <= 512 for the adjoint
 
 // write the 4-wave sum of a per-lane array (element c = lane + 64 qq) to dst[c]
+constexpr int kVRW = 8;  // waves per V-bar reduction workgroup
+
+// sum over the kVRW waves of per-lane arrays indexed by c = lane + 64 qq (fixed order)
 #define PAMD_WAVE_COMBINE(arr, dst)                                          \
   do {                                                                       \
     _Pragma("unroll") for (int qq = 0; qq < kMaxVQ; ++qq) {                  \
@@ -578,15 +623,31 @@ constexpr int kMaxVQ = 8;  // v <= 512 for the adjoint
       if (qq < nq && c_ < v) red[wave * 512 + c_] = (arr)[qq];               \
     }                                                                        \
     __syncthreads();                                                         \
-    for (int c_ = tid; c_ < v; c_ += 256)                                    \
-      (dst)[c_] = red[c_] + red[512 + c_] + red[1024 + c_] + red[1536 + c_]; \
+    for (int c_ = tid; c_ < v; c_ += 64 * kVRW) {                            \
+      double s_ = 0.0;                                                       \
+      for (int w_ = 0; w_ < kVRW; ++w_) s_ += red[w_ * 512 + c_];            \
+      (dst)[c_] = s_;                                                        \
+    }                                                                        \
     __syncthreads();                                                         \
   } while (0)
 
-__global__ __launch_bounds__(256) void k_vr_a(VRArgs A) {
-  __shared__ double red[4 * 512];
+// deterministic block sum over kVRW waves; scratch >= kVRW doubles
+__device__ double block_sum_vr(double x, double *scratch) {
+  x = wave_sum(x);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) scratch[w] = x;
+  __syncthreads();
+  double r = 0.0;
+  for (int q = 0; q < kVRW; ++q) r += scratch[q];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(64 * kVRW) void k_vr_a(VRArgs A) {
+  __shared__ double red[kVRW * 512];
   __shared__ double cst[4][512];  // per-c constants: t1[k,c], fvo[c,k], S2[a,c], T2[a,c]
-  __shared__ double sred[4];
+  __shared__ double sred[kVRW];
   const int a = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int o = A.o, v = A.v, i = A.i, j = A.j, k = A.k;
@@ -594,7 +655,7 @@ __global__ __launch_bounds__(256) void k_vr_a(VRArgs A) {
   const double *ovs = A.ovs, *t2s = A.t2s, *t1 = A.t1, *fvo = A.fvo;
 #define SYM(arr, x, y, p, q) arr[(((size_t)(x) * o + (y)) * v + (p)) * v + (q)]
   const int nq = (v + 63) / 64;
-  for (int c = tid; c < v; c += 256) {
+  for (int c = tid; c < v; c += 64 * kVRW) {
     cst[0][c] = t1[k * v + c];
     cst[2][c] = SYM(ovs, i, k, a, c);
     cst[1][c] = A.has_fvo ? fvo[c * o + k] : 0.0;
@@ -606,7 +667,7 @@ __global__ __launch_bounds__(256) void k_vr_a(VRArgs A) {
   for (int qq = 0; qq < kMaxVQ; ++qq) r3[qq] = r4[qq] = pk[qq] = fk[qq] = 0.0;
   double t1i = 0.0, fvoi = 0.0;
   double *R = A.R;
-  for (int b = wave; b < v; b += 4) {
+  for (int b = wave; b < v; b += kVRW) {
     const double t1j = t1[j * v + b];
     const double S1 = SYM(ovs, i, j, a, b);
     const double fvj = A.has_fvo ? fvo[b * o + j] : 0.0;
@@ -651,8 +712,8 @@ __global__ __launch_bounds__(256) void k_vr_a(VRArgs A) {
   PAMD_WAVE_COMBINE(r4, R + 3 * vv + (size_t)a * v);
   PAMD_WAVE_COMBINE(pk, R + 6 * vv + (size_t)a * v);
   PAMD_WAVE_COMBINE(fk, R + 7 * vv + (size_t)a * v);
-  const double t1i_s = block_sum256(t1i, sred);
-  const double fvoi_s = block_sum256(fvoi, sred);
+  const double t1i_s = block_sum_vr(t1i, sred);
+  const double fvoi_s = block_sum_vr(fvoi, sred);
   if (tid == 0) {
     R[10 * vv + a] = t1i_s;
     R[10 * vv + v + a] = fvoi_s;
@@ -660,8 +721,8 @@ __global__ __launch_bounds__(256) void k_vr_a(VRArgs A) {
 #undef SYM
 }
 
-__global__ __launch_bounds__(256) void k_vr_b(VRArgs A) {
-  __shared__ double red[4 * 512];
+__global__ __launch_bounds__(64 * kVRW) void k_vr_b(VRArgs A) {
+  __shared__ double red[kVRW * 512];
   const int b = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int o = A.o, v = A.v, i = A.i;
@@ -670,7 +731,7 @@ __global__ __launch_bounds__(256) void k_vr_b(VRArgs A) {
   double r5[kMaxVQ], r6[kMaxVQ];
 #pragma unroll
   for (int qq = 0; qq < kMaxVQ; ++qq) r5[qq] = r6[qq] = 0.0;
-  for (int a = wave; a < v; a += 4) {
+  for (int a = wave; a < v; a += kVRW) {
     const double t1a = A.t1[i * v + a];
     const double fa = A.has_fvo ? A.fvo[a * o + i] : 0.0;
     const double *row = A.Vb + (size_t)a * vv + (size_t)b * v;
@@ -691,8 +752,8 @@ __global__ __launch_bounds__(256) void k_vr_b(VRArgs A) {
 
 hipError_t launch_vr(const VRArgs &a, hipStream_t s) {
   if (a.v > 64 * kMaxVQ) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_vr_a, dim3(a.v), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(k_vr_b, dim3(a.v), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_vr_a, dim3(a.v), dim3(64 * kVRW), 0, s, a);
+  hipLaunchKernelGGL(k_vr_b, dim3(a.v), dim3(64 * kVRW), 0, s, a);
   return hipGetLastError();
 }
 
@@ -726,9 +787,7 @@ __global__ void k_combine(CombineArgs A) {
     const int xs[6] = {k, j, k, i, j, i};
     const int ys[6] = {j, k, i, k, i, j};
     for (int pr = 0; pr < 6; ++pr) {
-      double s = 0.0;
-      const double *pp = A.part_p + (size_t)pr * A.S_p * vv + pq;
-      for (int sp = 0; sp < A.S_p; ++sp) s += pp[(size_t)sp * vv];
+      double s = A.part_p[(size_t)pr * A.S_p * vv + pq];  // split-K sum (k_splitk_reduce)
       if (A.has_fvo) {
         switch (pr) {
           case 0: s += 0.5 * R[5 * vv + qp]; break;  // t2_bar[k,j,c,b] += R6[b,c]/2
@@ -749,9 +808,7 @@ __global__ void k_combine(CombineArgs A) {
     const int ys[6] = {j, k, i, k, i, j};
     const size_t vo = (size_t)v * o;
     for (int pr = 0; pr < 6; ++pr) {
-      double s = 0.0;
-      const double *pp = A.part_h + (size_t)pr * A.S_h * vo + t;
-      for (int sp = 0; sp < A.S_h; ++sp) s += pp[(size_t)sp * vo];
+      const double s = A.part_h[(size_t)pr * A.S_h * vo + t];  // split-K sum (k_splitk_reduce)
       A.ovoo_bar[(((size_t)xs[pr] * v + p) * o + m) * o + ys[pr]] += s;
     }
   }
@@ -794,7 +851,32 @@ __global__ __launch_bounds__(256) void k_vr_colsums(CombineArgs A) {
   }
 }
 
+// Split-K partial sums of six problems, in place: part[pr][0][t] = sum_s
+// part[pr][s][t], s = 0..S-1, in a fixed order (four interleaved partial sums).
+__global__ void k_splitk_reduce(double *part, int S, size_t T) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  double *p = part + (size_t)blockIdx.y * S * T + t;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int s = 0;
+  for (; s + 4 <= S; s += 4) {
+    a0 += p[(size_t)s * T];
+    a1 += p[(size_t)(s + 1) * T];
+    a2 += p[(size_t)(s + 2) * T];
+    a3 += p[(size_t)(s + 3) * T];
+  }
+  for (; s < S; ++s) a0 += p[(size_t)s * T];
+  p[0] = (a0 + a1) + (a2 + a3);
+}
+
 hipError_t launch_combine(const CombineArgs &a, hipStream_t s) {
+  {
+    const size_t Tp = (size_t)a.v * a.v, Th = (size_t)a.v * a.o;
+    hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)((Tp + 255) / 256), 6), dim3(256), 0, s,
+                       const_cast<double *>(a.part_p), a.S_p, Tp);
+    hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)((Th + 255) / 256), 6), dim3(256), 0, s,
+                       const_cast<double *>(a.part_h), a.S_h, Th);
+  }
   size_t n = (size_t)a.v * a.v;
   if ((size_t)a.v * a.o > n) n = (size_t)a.v * a.o;
   hipLaunchKernelGGL(k_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);

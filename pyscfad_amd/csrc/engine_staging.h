@@ -27,6 +27,27 @@ struct OpStage {
   static_assert(KSTEP * E == R && RSTEP * E == GKC, "staging map covers the panel");
 };
 
+// Output coordinate of position w of tile t along one dimension: the index,
+// and its split (q, r) = divmod(index, div) for the output descriptor (no
+// division when div is the "no split" value 2^30).
+struct OutPos {
+  int idx, q, r;
+};
+
+template <int R>
+__device__ __forceinline__ OutPos out_pos(int t, int w, int div) {
+  OutPos p;
+  p.idx = t * R + w;
+  if (div >= (1 << 30)) {
+    p.q = 0;
+    p.r = p.idx;
+  } else {
+    p.q = p.idx / div;
+    p.r = p.idx - p.q * div;
+  }
+  return p;
+}
+
 template <int R, int NTHR>
 __device__ __forceinline__ void op_rows(const OpDesc &d, int R_total, int r0, int tid,
                                         int (&roff)[OpStage<R, NTHR>::E],

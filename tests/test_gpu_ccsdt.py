@@ -60,7 +60,9 @@ def test_golden_energy_vjp(name, packed, cuda):
                 G_RTOL * float(z[ref_key + '_norm'])
 
 
-@pytest.mark.parametrize('o,v,seed,fvo', [(3, 9, 11, 0.0), (5, 17, 5, 2e-3), (7, 24, 8, 0.0)])
+# v = 48: several full workgroup tiles along the virtual-pair dimension
+@pytest.mark.parametrize('o,v,seed,fvo', [(3, 9, 11, 0.0), (5, 17, 5, 2e-3), (7, 24, 8, 0.0),
+                                          (4, 48, 2, 1e-3)])
 def test_oracle_parity(o, v, seed, fvo, cuda):
     p = make_problem(o, v, 4 * v, seed=seed, fvo_scale=fvo)
     print(f'case o{o}v{v} seed {seed}', flush=True)
