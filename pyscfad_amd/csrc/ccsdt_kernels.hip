@@ -477,11 +477,18 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
   __syncthreads();
 
   double eacc = 0.0, occ = 0.0;
-  const size_t vv = (size_t)v * v;
-  const double *t1 = A.t1, *ovs = A.ovs, *t2s = A.t2s, *fvo = A.fvo;
-#define SYM(arr, x, y, p, q) arr[(((size_t)(x) * o + (y)) * v + (p)) * v + (q)]
-  // this thread's position inside a tile
+  // Element (a, b, c) = 8 (Pa, Pb, Pc) + (al, be, ga): every address splits into a
+  // part that is uniform per tile and a part that is fixed per thread.
   const int al = tid >> 6, be = (tid >> 3) & 7, ga = tid & 7;
+  const long long thr_w = ((long long)al * v + be) * v + ga;
+  const int t_ab = al * v + be, t_ac = al * v + ga, t_bc = be * v + ga;
+  const size_t vv = (size_t)v * v;
+  const double *ovs_ij = A.ovs + ((size_t)i * o + j) * vv, *ovs_ik = A.ovs + ((size_t)i * o + k) * vv,
+               *ovs_jk = A.ovs + ((size_t)j * o + k) * vv;
+  const double *t2s_ij = A.t2s + ((size_t)i * o + j) * vv, *t2s_ik = A.t2s + ((size_t)i * o + k) * vv,
+               *t2s_jk = A.t2s + ((size_t)j * o + k) * vv;
+  const double *t1i = A.t1 + (size_t)i * v, *t1j = A.t1 + (size_t)j * v, *t1k = A.t1 + (size_t)k * v;
+  const double *fvo = A.fvo;
   const int l_bca = (be << 6) | (ga << 3) | al;
   const int l_cab = (ga << 6) | (al << 3) | be;
   const int l_cba = (ga << 6) | (be << 3) | al;
@@ -495,20 +502,16 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
     const int np = pt.np;
     // phase 1: W and X = W + V of the orbit's tiles
     for (int p = 0; p < np; ++p) {
-      const int a = val[pt.sel[p][0]] * OT + al;
-      const int b = val[pt.sel[p][1]] * OT + be;
-      const int c = val[pt.sel[p][2]] * OT + ga;
+      const int A0 = val[pt.sel[p][0]] * OT, B0 = val[pt.sel[p][1]] * OT, C0 = val[pt.sel[p][2]] * OT;
+      const int a = A0 + al, b = B0 + be, c = C0 + ga;
       double wv = 0.0, xv = 0.0;
       if (a < v && b < v && c < v) {
-        wv = A.W[(size_t)a * vv + (size_t)b * v + c];
-        const double t1a = t1[i * v + a], t1b = t1[j * v + b], t1c = t1[k * v + c];
-        double vv_ = SYM(ovs, i, j, a, b) * t1c + SYM(ovs, i, k, a, c) * t1b +
-                     SYM(ovs, j, k, b, c) * t1a;
-        if (A.has_fvo) {
-          const double fck = fvo[c * o + k], fbj = fvo[b * o + j], fai = fvo[a * o + i];
-          vv_ += SYM(t2s, i, j, a, b) * fck + SYM(t2s, i, k, a, c) * fbj +
-                 SYM(t2s, j, k, b, c) * fai;
-        }
+        const long long tile_w = ((long long)A0 * v + B0) * v + C0;
+        wv = A.W[tile_w + thr_w];
+        const int ab = A0 * v + B0 + t_ab, ac = A0 * v + C0 + t_ac, bc = B0 * v + C0 + t_bc;
+        double vv_ = ovs_ij[ab] * t1k[c] + ovs_ik[ac] * t1j[b] + ovs_jk[bc] * t1i[a];
+        if (A.has_fvo)
+          vv_ += t2s_ij[ab] * fvo[c * o + k] + t2s_ik[ac] * fvo[b * o + j] + t2s_jk[bc] * fvo[a * o + i];
         xv = wv + 0.5 * vv_;
       }
       Ws[p][tid] = wv;
@@ -517,8 +520,8 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
     __syncthreads();
     // phase 2
     for (int p = 0; p < np; ++p) {
-      const int Pa = val[pt.sel[p][0]], Pb = val[pt.sel[p][1]], Pc = val[pt.sel[p][2]];
-      const int a = Pa * OT + al, b = Pb * OT + be, c = Pc * OT + ga;
+      const int A0 = val[pt.sel[p][0]] * OT, B0 = val[pt.sel[p][1]] * OT, C0 = val[pt.sel[p][2]] * OT;
+      const int a = A0 + al, b = B0 + be, c = C0 + ga;
       double dbar = 0.0;
       if (a < v && b < v && c < v) {
         const int p0 = pt.pmap[0][p], p1 = pt.pmap[1][p], p2 = pt.pmap[2][p], p3 = pt.pmap[3][p],
@@ -531,11 +534,15 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
                      x4 = Xs[p3][l_acb], x5 = Xs[p4][l_bac];
         const double Z = 4.0 * x0 + x1 + x2 - 2.0 * (x3 + x4 + x5);
         const double D = eo - ev[a] - ev[b] - ev[c];
-        const double rD = 1.0 / D;
+        // 1/D: hardware reciprocal estimate refined by two Newton-Raphson steps
+        // (D <= -(HOMO-LUMO gap) < 0 is never near 0 or a denormal)
+        double rD = __builtin_amdgcn_rcp(D);
+        rD = fma(rD, fma(-D, rD, 1.0), rD);
+        rD = fma(rD, fma(-D, rD, 1.0), rD);
         eacc += w0 * Z * rD;
         if (A.vjp) {
           const double r3W = 4.0 * w0 + w1 + w2 - 2.0 * (w3 + w4 + w5);
-          const size_t idx = (size_t)a * vv + (size_t)b * v + c;
+          const long long idx = ((long long)A0 * v + B0) * v + C0 + thr_w;
           const double vbar = g * r3W * rD;
           const double wbar = g * Z * rD + vbar;
           if (A.Wb8)
@@ -561,7 +568,7 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
         sa += __shfl_xor(sa, 8, 64);
         sa += __shfl_xor(sa, 16, 64);
         sa += __shfl_xor(sa, 32, 64);
-        if (lane < 8 && Pc * OT + lane < v) atomicAdd(&vir[Pc * OT + lane], -sc);
+        if (lane < 8 && C0 + lane < v) atomicAdd(&vir[C0 + lane], -sc);
         if ((lane & 7) == 0 && b < v) atomicAdd(&vir[b], -sb);
         if (lane == 0 && a < v) atomicAdd(&vir[a], -sa);
       }
@@ -590,7 +597,6 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
       pm[k] += otot;
     }
   }
-#undef SYM
 }
 
 hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s) {

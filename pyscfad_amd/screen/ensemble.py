@@ -43,12 +43,15 @@ class Screen:
         Xn = ((X - self.x_mean) / self.x_std).float()
         return to_bf16(_pad_cols(Xn, self.p_in).contiguous(), with_t=True)
 
-    def fit(self, X, Y, epochs=300, batch=128, lr=2e-3, wd=1e-5, seed=0, log=None):
+    def fit(self, X, Y, epochs=300, batch=128, lr=2e-3, wd=1e-5, seed=0, bootstrap=True,
+            log=None):
         """X [n, n_in], Y [n, n_out] float64 device tensors (features, pair labels).
 
-        Each member sees its own shuffles (seeded by ``seed`` and its index);
-        the learning rate follows a cosine schedule.  Returns the final
-        training loss of each member (standardised units).
+        Each member sees its own shuffles (seeded by ``seed`` and its index)
+        and, with ``bootstrap``, its own resample (with replacement) of the
+        training set, so that the members' disagreement reflects what the data
+        leave undetermined; the learning rate follows a cosine schedule.
+        Returns the final training loss of each member (standardised units).
         """
         self.x_mean, self.x_std = X.mean(0), X.std(0).clamp_min(1e-30)
         self.y_mean, self.y_std = Y.mean(0), Y.std(0).clamp_min(1e-30)
@@ -59,10 +62,12 @@ class Screen:
         hist = []
         for m, net in enumerate(self.members):
             g = torch.Generator(device='cpu').manual_seed(seed * 1000 + m)
+            rows = (torch.randint(0, n, (n,), generator=g) if bootstrap
+                    else torch.arange(n)).to(self.dev)
             loss = None
             for ep in range(epochs):
                 lr_e = lr * 0.5 * (1 + np.cos(np.pi * ep / epochs))
-                perm = torch.randperm(n, generator=g).to(self.dev)
+                perm = rows[torch.randperm(n, generator=g).to(self.dev)]
                 for s0 in range(0, n - batch + 1, batch):
                     idx = perm[s0:s0 + batch]
                     xb = x16_all[idx].contiguous()

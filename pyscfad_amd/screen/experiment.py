@@ -40,6 +40,7 @@ def main(argv=None):
     ap.add_argument('--epochs', type=int, default=300)
     ap.add_argument('--batch', type=int, default=128)
     ap.add_argument('--lr', type=float, default=2e-3)
+    ap.add_argument('--no-bootstrap', action='store_true')
     ap.add_argument('--out', default=None)
     args = ap.parse_args(argv)
     if not torch.cuda.is_available():
@@ -79,7 +80,8 @@ def main(argv=None):
     tr_d, te_d = torch.as_tensor(tr, device=dev), torch.as_tensor(te, device=dev)
     sc = Screen(X.shape[1], Y.shape[1], dev, members=4)
     t0 = time.perf_counter()
-    losses = sc.fit(X[tr_d], Y[tr_d], epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
+    losses = sc.fit(X[tr_d], Y[tr_d], epochs=args.epochs, batch=args.batch, lr=args.lr,
+                    bootstrap=not args.no_bootstrap, log=log)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
 
@@ -114,6 +116,9 @@ def main(argv=None):
         'accepted_mae': float(err[acc].mean()) if bool(acc.any()) else None,
         'rejected_max_err': float(err[~acc].max()) if bool((~acc).any()) else None,
         'spread_median': float(spread.median()),
+        'bootstrap': not args.no_bootstrap, 'epochs': args.epochs,
+        'err_over_spread_median': float((err / spread.clamp_min(1e-30)).median()),
+        'accepted_err_gt_tol': int(((err > SPREAD_TOL) & acc).sum()),
         'exact_ms_per_instance': float(t_exact.mean() * 1e3),
         'infer_ms_test_set': t_infer * 1e3,
         'exact_pass_s': exact_pass, 'screened_pass_s': screened_pass,
