@@ -637,6 +637,27 @@ constexpr int kVRW = 8;  // waves per V-bar reduction workgroup
     __syncthreads();                                                         \
   } while (0)
 
+// Sums of four per-lane values over the wave: on return lane 16 q holds the
+// total of value q (q = 0..3), in 7 cross-lane steps instead of 4 x 6.
+__device__ __forceinline__ double wave_sum4(double v0, double v1, double v2, double v3) {
+  const int lane = threadIdx.x & 63;
+  const bool hi32 = (lane & 32) != 0, hi16 = (lane & 16) != 0;
+  // lanes below 32 collect values 0 and 1, the others 2 and 3
+  double x = hi32 ? v2 : v0, y = hi32 ? v3 : v1;
+  const double sx = hi32 ? v0 : v2, sy = hi32 ? v1 : v3;
+  x += __shfl_xor(sx, 32, 64);
+  y += __shfl_xor(sy, 32, 64);
+  // then bit 4 of the lane picks one of the two
+  double z = hi16 ? y : x;
+  const double sz = hi16 ? x : y;
+  z += __shfl_xor(sz, 16, 64);
+  z += __shfl_xor(z, 8, 64);
+  z += __shfl_xor(z, 4, 64);
+  z += __shfl_xor(z, 2, 64);
+  z += __shfl_xor(z, 1, 64);
+  return z;
+}
+
 // deterministic block sum over kVRW waves; scratch >= kVRW doubles
 __device__ double block_sum_vr(double x, double *scratch) {
   x = wave_sum(x);
@@ -702,15 +723,11 @@ __global__ __launch_bounds__(64 * kVRW) void k_vr_a(VRArgs A) {
         }
       }
     }
-    sr1 = wave_sum(sr1);
-    spj = wave_sum(spj);
-    sr2 = wave_sum(sr2);
-    sfj = wave_sum(sfj);
-    if (lane == 0) {
-      R[0 * vv + (size_t)a * v + b] = sr1;
-      R[1 * vv + (size_t)a * v + b] = sr2;
-      R[8 * vv + (size_t)a * v + b] = spj;
-      R[9 * vv + (size_t)a * v + b] = sfj;
+    // R1, R2, S_pj, S_fj of row (a, b): lane 16 q holds sum q
+    const double s4 = wave_sum4(sr1, sr2, spj, sfj);
+    if ((lane & 15) == 0) {
+      const int q = lane >> 4;
+      R[(size_t)(q < 2 ? q : q + 6) * vv + (size_t)a * v + b] = s4;
     }
   }
   __syncthreads();

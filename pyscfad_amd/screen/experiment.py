@@ -37,7 +37,7 @@ def main(argv=None):
     ap.add_argument('--n-train', type=int, default=D.N_TRAIN)
     ap.add_argument('--data', default=None, help='load (X, Y, E, t_exact) from this .npz')
     ap.add_argument('--save-data', default=None)
-    ap.add_argument('--epochs', type=int, default=300)
+    ap.add_argument('--epochs', type=int, default=600)
     ap.add_argument('--batch', type=int, default=128)
     ap.add_argument('--lr', type=float, default=2e-3)
     ap.add_argument('--no-bootstrap', action='store_true')
