@@ -58,11 +58,12 @@ constexpr int BKC = 32;
 
 template <int R>
 struct B3Stage {
-  static constexpr int NE = R / 16;  // elements per thread per stage
-  static constexpr int NQ = R / 64;  // quads per thread per stage
+  static constexpr int NQ = (R + 63) / 64;  // quads per thread per stage
+  static constexpr int NE = 4 * NQ;         // elements per thread per stage
+  static constexpr bool FULL = R % 64 == 0; // otherwise rows row0 + 64 j >= R are skipped
   static constexpr int PLANE = R * 64;
   static constexpr int BYTES = 3 * PLANE;
-  static_assert(R % 64 == 0 && R <= 512, "B3 staging map");
+  static_assert(R % 16 == 0 && R <= 512, "B3 staging map");
 };
 
 __device__ __forceinline__ int b3_lds_off(int r, int chunk) {
@@ -89,7 +90,7 @@ __device__ __forceinline__ void b3_rows(const OpDesc &d, int R_total, int r0, co
 #pragma unroll
   for (int j = 0; j < B3Stage<R>::NQ; ++j) {
     const int r = r0 + m.row0 + 64 * j;
-    rok[j] = r < R_total;
+    rok[j] = r < R_total && (B3Stage<R>::FULL || m.row0 + 64 * j < R);
     const int q = r / d.rdiv;
     roff[j] = rok[j] ? (int)(q * d.rx + (r - q * d.rdiv) * d.ry) : 0;
   }
@@ -187,6 +188,7 @@ __device__ __forceinline__ void b3_stash(const B3Rows &m, bool neg,
   const unsigned long long flip = neg ? 0x0000800080008000ull : 0ull;
 #pragma unroll
   for (int j = 0; j < B3Stage<R>::NQ; ++j) {
+    if (!B3Stage<R>::FULL && m.row0 + 64 * j >= R) continue;  // beyond the panel
     unsigned long long w[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) w[u] = v[4 * j + u] ^ flip;
@@ -428,15 +430,185 @@ __global__ __launch_bounds__(512) void k_gemm_b3(GemmDesc d, int n_fast) {
       }
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// The same engine on v_mfma_f32_16x16x32_bf16 (16 x 16 output tiles, one MFMA
+// per product per stage), for tile widths that are multiples of 16 but not of
+// 32: 80 (5 x 80 = 400 = v at the benchmark size, so the A panel is read five
+// times instead of seven and nothing is padded) and 48.  One stage of loads in
+// flight (the wider wave tiles need the registers).
+// the six split products of one 16x16 tile, smallest first, into one FP32 sum
+__device__ __forceinline__ void b3n_products(bf16x8 ah, bf16x8 am, bf16x8 al, bf16x8 bh, bf16x8 bm,
+                                             bf16x8 bl, f32x4 &acc) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+}
+
+template <int MT, int NT, int WM, int WN>
+__global__ __launch_bounds__(512) void k_gemm_b3n(GemmDesc d, int n_fast) {
+  static_assert(WM * WN == 8, "8 waves");
+  constexpr int WTM = MT / WM, WTN = NT / WN;
+  constexpr int TM = WTM / 16, TN = WTN / 16;
+  static_assert(TM * 16 == WTM && TN * 16 == WTN, "16x16 MFMA tiles");
+  using SA = B3Stage<MT>;
+  using SB = B3Stage<NT>;
+  __shared__ __align__(16) char As[2][SA::BYTES];
+  __shared__ __align__(16) char Bs[2][SB::BYTES];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+  const int mt = (d.M + MT - 1) / MT, nt = (d.N + NT - 1) / NT;
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const int split = logical / (mt * nt);
+  const int tile = logical - split * (mt * nt);
+  const int tm = n_fast ? tile / nt : tile % mt;
+  const int tn = n_fast ? tile % nt : tile / mt;
+  const int m0 = tm * MT, n0 = tn * NT;
+  const int total = d.chunk_start[d.nseg];
+  const int cps = (total + d.splits - 1) / d.splits;
+  const int g0 = split * cps;
+  const int g1 = min(total, g0 + cps);
+
+  double acc[TM][TN][4];
+  f32x4 fa[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        acc[a][b][r] = 0.0;
+        fa[a][b][r] = 0.f;
+      }
+
+  B3Producer<MT, NT> P;
+  P.start(d, tid, m0, n0, g0, g1);
+  B3Set<MT, NT> S;
+  P.produce(S);
+  int buf = 0, nfl = 0;
+  constexpr int kFlush = 4;
+  const int chunk = lane >> 4;
+  for (int g = g0; g < g1; ++g) {
+    b3_stash<MT>(S.mA, S.neg, S.va, S.rma, S.kma, As[buf]);
+    b3_stash<NT>(S.mB, false, S.vb, S.rmb, S.kmb, Bs[buf]);
+    __syncthreads();
+    P.produce(S);
+    const char *SAb = As[buf];
+    const char *SBb = Bs[buf];
+    // keep the fragments of the narrower side in registers, stream the other
+    if constexpr (TM <= TN) {
+      bf16x8 ah[TM], am[TM], al[TM];
+#pragma unroll
+      for (int a = 0; a < TM; ++a) {
+        const int r = wm * WTM + 16 * a + (lane & 15);
+        ah[a] = b3_read(SAb, SA::PLANE, 0, r, chunk);
+        am[a] = b3_read(SAb, SA::PLANE, 1, r, chunk);
+        al[a] = b3_read(SAb, SA::PLANE, 2, r, chunk);
+      }
+#pragma unroll
+      for (int b = 0; b < TN; ++b) {
+        const int r = wn * WTN + 16 * b + (lane & 15);
+        const bf16x8 bh = b3_read(SBb, SB::PLANE, 0, r, chunk);
+        const bf16x8 bm = b3_read(SBb, SB::PLANE, 1, r, chunk);
+        const bf16x8 bl = b3_read(SBb, SB::PLANE, 2, r, chunk);
+#pragma unroll
+        for (int a = 0; a < TM; ++a) b3n_products(ah[a], am[a], al[a], bh, bm, bl, fa[a][b]);
+      }
+    } else {
+      bf16x8 bh[TN], bm[TN], bl[TN];
+#pragma unroll
+      for (int b = 0; b < TN; ++b) {
+        const int r = wn * WTN + 16 * b + (lane & 15);
+        bh[b] = b3_read(SBb, SB::PLANE, 0, r, chunk);
+        bm[b] = b3_read(SBb, SB::PLANE, 1, r, chunk);
+        bl[b] = b3_read(SBb, SB::PLANE, 2, r, chunk);
+      }
+#pragma unroll
+      for (int a = 0; a < TM; ++a) {
+        const int r = wm * WTM + 16 * a + (lane & 15);
+        const bf16x8 ah = b3_read(SAb, SA::PLANE, 0, r, chunk);
+        const bf16x8 am = b3_read(SAb, SA::PLANE, 1, r, chunk);
+        const bf16x8 al = b3_read(SAb, SA::PLANE, 2, r, chunk);
+#pragma unroll
+        for (int b = 0; b < TN; ++b) b3n_products(ah, am, al, bh[b], bm[b], bl[b], fa[a][b]);
+      }
+    }
+    if (++nfl == kFlush || g + 1 == g1) {
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            acc[a][b][r] += (double)fa[a][b][r];
+            fa[a][b][r] = 0.f;
+          }
+      nfl = 0;
+    }
+    buf ^= 1;
+  }
+
+  // D layout of v_mfma_f32_16x16x32_bf16: lane l, element r -> row 4 (l/16) + r, column l%16
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ml = wm * WTM + 16 * a + 4 * (lane >> 4) + r;
+        const int nl = wn * WTN + 16 * b + (lane & 15);
+        const bool part = d.mode == kPartial;
+        const OutPos pm = out_pos<MT>(tm, ml, part ? (1 << 30) : d.C.mdiv);
+        const OutPos pn = out_pos<NT>(tn, nl, part ? (1 << 30) : d.C.ndiv);
+        const int m = pm.idx, n = pn.idx;
+        if (m >= d.M || n >= d.N) continue;
+        const double val = acc[a][b][r];
+        if (part) {
+          d.partial[((size_t)split * d.M + m) * d.N + n] = val;
+        } else {
+          double *c = d.C.p + (long long)pm.q * d.C.mx + (long long)pm.r * d.C.my +
+                      (long long)pn.q * d.C.nx + (long long)pn.r * d.C.ny;
+          if (d.mode == kAccumulate)
+            *c += val;
+          else
+            *c = val;
+        }
+      }
+}
+
 // fraction of a padded MT x NT tiling that is real output
 static double tile_eff(int M, int N, int MT, int NT) {
   const double pm = (double)((M + MT - 1) / MT) * MT, pn = (double)((N + NT - 1) / NT) * NT;
   return (double)M * N / (pm * pn);
 }
 
+// Tile shapes of the split-BF16x3 engine.  The score is the padded tiling's
+// efficiency times the panel's flops per staged byte (MT NT / (MT + NT)).
+enum B3Cfg { kB3_256x64 = 0, kB3_64x256, kB3_256x80, kB3_80x256, kB3_256x48, kB3_48x256, kB3Num };
+static const int kB3Shape[kB3Num][2] = {{256, 64}, {64, 256}, {256, 80}, {80, 256}, {256, 48}, {48, 256}};
+
+static int b3_pick(int M, int N) {
+  int best = 0;
+  double bs = -1.0;
+  for (int c = 0; c < kB3Num; ++c) {
+    const int MT = kB3Shape[c][0], NT = kB3Shape[c][1];
+    const double sc = tile_eff(M, N, MT, NT) * (double)MT * NT / (MT + NT);
+    if (sc > bs + 1e-9) {
+      bs = sc;
+      best = c;
+    }
+  }
+  return best;
+}
+
 int gemm_tiles_b3(int M, int N) {
-  if (tile_eff(M, N, 256, 64) >= tile_eff(M, N, 64, 256)) return ((M + 255) / 256) * ((N + 63) / 64);
-  return ((M + 63) / 64) * ((N + 255) / 256);
+  const int c = b3_pick(M, N);
+  const int MT = kB3Shape[c][0], NT = kB3Shape[c][1];
+  return ((M + MT - 1) / MT) * ((N + NT - 1) / NT);
 }
 
 hipError_t launch_gemm_b3(const GemmDesc &d0, hipStream_t s) {
@@ -445,14 +617,17 @@ hipError_t launch_gemm_b3(const GemmDesc &d0, hipStream_t s) {
   d.chunk_start[0] = 0;
   for (int q = 0; q < d.nseg; ++q) d.chunk_start[q + 1] = d.chunk_start[q] + (d.seg[q].L + BKC - 1) / BKC;
   for (int q = d.nseg + 1; q <= kMaxSeg; ++q) d.chunk_start[q] = d.chunk_start[d.nseg];
-  const double e_tall = tile_eff(d.M, d.N, 256, 64), e_wide = tile_eff(d.M, d.N, 64, 256);
+  const int c = b3_pick(d.M, d.N);
+  const int MT = kB3Shape[c][0], NT = kB3Shape[c][1];
+  const unsigned grid = (unsigned)((long long)((d.M + MT - 1) / MT) * ((d.N + NT - 1) / NT) * d.splits);
   const int n_fast = d.M >= d.N ? 1 : 0;
-  if (e_tall >= e_wide) {
-    const long long tiles = (long long)((d.M + 255) / 256) * ((d.N + 63) / 64) * d.splits;
-    hipLaunchKernelGGL((k_gemm_b3<256, 64, 4, 2>), dim3((unsigned)tiles), dim3(512), 0, s, d, n_fast);
-  } else {
-    const long long tiles = (long long)((d.M + 63) / 64) * ((d.N + 255) / 256) * d.splits;
-    hipLaunchKernelGGL((k_gemm_b3<64, 256, 2, 4>), dim3((unsigned)tiles), dim3(512), 0, s, d, n_fast);
+  switch (c) {
+    case kB3_256x64: hipLaunchKernelGGL((k_gemm_b3<256, 64, 4, 2>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
+    case kB3_64x256: hipLaunchKernelGGL((k_gemm_b3<64, 256, 2, 4>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
+    case kB3_256x80: hipLaunchKernelGGL((k_gemm_b3n<256, 80, 8, 1>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
+    case kB3_80x256: hipLaunchKernelGGL((k_gemm_b3n<80, 256, 1, 8>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
+    case kB3_256x48: hipLaunchKernelGGL((k_gemm_b3n<256, 48, 8, 1>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
+    default: hipLaunchKernelGGL((k_gemm_b3n<48, 256, 1, 8>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
   }
   return hipGetLastError();
 }

@@ -44,9 +44,11 @@ def test_bf16x3_vs_oracle(o, v, fvo, packed, cuda):
         assert err <= 1e-5, k
 
 
-def test_bf16x3_vs_fp64_larger(cuda):
-    # v = 128: several full workgroup tiles in every contraction
-    p = make_problem(12, 128, 512, seed=3, device=cuda)
+# v = 128: several full 256x64 / 64x256 tiles; v = 80: the 16x16-MFMA tiles
+# (256x80, 80x256, 256x48) that the engine picks for v = 400
+@pytest.mark.parametrize('o,v', [(12, 128), (6, 80)])
+def test_bf16x3_vs_fp64_larger(o, v, cuda):
+    p = make_problem(o, v, 4 * v, seed=3, device=cuda)
     ins = {}
     for prec in ('fp64', 'bf16x3'):
         ins[prec] = _ops.Inputs(p.mo_energy, p.t1, p.t2, p.ovvv, p.ovoo, p.ovov, None,
@@ -54,7 +56,7 @@ def test_bf16x3_vs_fp64_larger(cuda):
     e64, g64 = _ops.energy_vjp(ins['fp64'])
     e16, g16 = _ops.energy_vjp(ins['bf16x3'])
     de = abs(float(e16.item()) - float(e64.item()))
-    print(f'o12v128 E64={float(e64.item()):.12e} |dE|={de:.3e}', flush=True)
+    print(f'o{o}v{v} E64={float(e64.item()):.12e} |dE|={de:.3e}', flush=True)
     assert de <= 1e-6 * abs(float(e64.item()))
     for k in BLOCKS:
         err = _rel(g16[k].cpu().numpy(), g64[k].cpu().numpy())
