@@ -1,0 +1,146 @@
+"""CPU oracle for the MP2 pair energies and the DF (T) integrals -- TEST INFRASTRUCTURE.
+
+This module is the checker, never the product: only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may import
+it.  The product path (``pyscfad_amd``) never calls it.
+
+* ``mp2_init_amps`` restates the t2/emp2 part of ``CCSD.init_amps``
+  (pyscfad/cc/ccsd.py:364-381): ``eia = e_occ[:, None] - e_vir[None, :]``,
+  ``t2 = ovov.transpose(0,2,1,3) / (eia[:,None,:,None] + eia[None,:,None,:])``,
+  ``emp2 = 2 einsum('ijab,iajb', t2, ovov) - einsum('jiab,iajb', t2, ovov)``,
+  here resolved over ordered pairs: ``e[i,j] = sum_ab (2 t2[i,j,a,b] -
+  t2[j,i,a,b]) ovov[i,a,j,b]``.
+* ``df_eris`` restates the (T) blocks of ``_make_df_eris_incore``
+  (pyscfad/cc/dfccsd.py:62-89): Loo, Lov, ``Lvv = pack_tril(Lpq[:, o:, o:])``,
+  ``ovoo = Lov^T Loo``, ``ovov = Lov^T Lov``, ``ovvv = Lov^T Lvv``.
+* ``df_eris_vjp`` is the adjoint of ``df_eris``: the adjoints of the three
+  np.dot products, and pack_tril's adjoint, which writes the packed cotangent
+  on the lower triangle and leaves zeros above it
+  (pyscfad/lib/_numpy_helper_opt.py:70-95, NPdpack_tril_vjp in
+  pyscfadlib/pyscfadlib/vjp/np_helper/pack_tril_vjp.c:62-70).
+
+Pinned by (tests/test_eris_oracle.py): pure-Python loops over the defining
+sums at tiny sizes (``mp2_pairs_loops``, ``df_eris_loops``); the closed form
+e[0,0] = sum_ab ovov[0,a,0,b]^2 / D at one occupied orbital; the identity
+sum(e) = emp2 with both einsums written as in ccsd.py:380-381; the adjoint
+identity <Lpq_bar, dL> = <ovvv_bar, d ovvv> + <ovoo_bar, d ovoo> + <ovov_bar, d ovov>
+for random directions (which fixes the adjoint given the forward, including
+pack_tril's lower-triangle convention); and torch autograd of the forward.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def pack_tril(m):
+    """(..., n, n) -> (..., n(n+1)/2): rows b >= c in row-major order."""
+    n = m.shape[-1]
+    b, c = np.tril_indices(n)
+    return m[..., b, c]
+
+
+def pack_tril_vjp(g, n):
+    """Adjoint of pack_tril: packed cotangent -> (..., n, n), zeros above the diagonal."""
+    out = np.zeros(g.shape[:-1] + (n, n))
+    b, c = np.tril_indices(n)
+    out[..., b, c] = g
+    return out
+
+
+def mp2_init_amps(mo_energy, ovov, nocc):
+    """(emp2, t2, e_pair) following pyscfad/cc/ccsd.py:369-381."""
+    mo_energy = np.asarray(mo_energy, dtype=np.float64)
+    eia = mo_energy[:nocc, None] - mo_energy[None, nocc:]
+    t2 = ovov.transpose(0, 2, 1, 3) / (eia[:, None, :, None] + eia[None, :, None, :])
+    direct = np.einsum('ijab,iajb->ij', t2, ovov)
+    exch = np.einsum('jiab,iajb->ij', t2, ovov)
+    e_pair = 2.0 * direct - exch
+    return e_pair.sum(), t2, e_pair
+
+
+def mp2_pairs_loops(mo_energy, ovov, nocc):
+    """Pure-Python loops over the defining sums (tiny sizes only)."""
+    o = nocc
+    v = ovov.shape[1]
+    e = mo_energy
+    t2 = np.zeros((o, o, v, v))
+    for i in range(o):
+        for j in range(o):
+            for a in range(v):
+                for b in range(v):
+                    t2[i, j, a, b] = ovov[i, a, j, b] / ((e[i] - e[o + a]) + (e[j] - e[o + b]))
+    pairs = np.zeros((o, o))
+    for i in range(o):
+        for j in range(o):
+            s = 0.0
+            for a in range(v):
+                for b in range(v):
+                    s += (2.0 * t2[i, j, a, b] - t2[j, i, a, b]) * ovov[i, a, j, b]
+            pairs[i, j] = s
+    return t2, pairs
+
+
+def df_eris(Lpq, nocc):
+    """(ovvv packed, ovoo, ovov) from Lpq (naux, nmo, nmo), dfccsd.py:74-88."""
+    naux, nmo, _ = Lpq.shape
+    o, v = nocc, nmo - nocc
+    Loo = Lpq[:, :o, :o].reshape(naux, -1)
+    Lov = Lpq[:, :o, o:].reshape(naux, -1)
+    Lvv = pack_tril(Lpq[:, o:, o:])
+    ovoo = (Lov.T @ Loo).reshape(o, v, o, o)
+    ovov = (Lov.T @ Lov).reshape(o, v, o, v)
+    ovvv = (Lov.T @ Lvv).reshape(o, v, -1)
+    return ovvv, ovoo, ovov
+
+
+def df_eris_vjp(Lpq, nocc, ovvv_bar, ovoo_bar, ovov_bar):
+    """Cotangent of Lpq for the three block cotangents (None = zero)."""
+    naux, nmo, _ = Lpq.shape
+    o, v = nocc, nmo - nocc
+    Loo = Lpq[:, :o, :o].reshape(naux, -1)
+    Lov = Lpq[:, :o, o:].reshape(naux, -1)
+    Lvv = pack_tril(Lpq[:, o:, o:])
+    gLoo = np.zeros_like(Loo)
+    gLov = np.zeros_like(Lov)
+    gLvv = np.zeros_like(Lvv)
+    if ovoo_bar is not None:
+        g = ovoo_bar.reshape(o * v, o * o)
+        gLov += Loo @ g.T
+        gLoo += Lov @ g
+    if ovov_bar is not None:
+        g = ovov_bar.reshape(o * v, o * v)
+        gLov += Lov @ (g + g.T)
+    if ovvv_bar is not None:
+        g = ovvv_bar.reshape(o * v, -1)
+        gLov += Lvv @ g.T
+        gLvv += Lov @ g
+    out = np.zeros_like(Lpq)
+    out[:, :o, :o] = gLoo.reshape(naux, o, o)
+    out[:, :o, o:] = gLov.reshape(naux, o, v)
+    out[:, o:, o:] = pack_tril_vjp(gLvv, v)
+    return out
+
+
+def df_eris_loops(Lpq, nocc):
+    """Pure-Python loops over the defining sums (tiny sizes only)."""
+    naux, nmo, _ = Lpq.shape
+    o, v = nocc, nmo - nocc
+    npair = v * (v + 1) // 2
+    ovvv = np.zeros((o, v, npair))
+    ovoo = np.zeros((o, v, o, o))
+    ovov = np.zeros((o, v, o, v))
+    for i in range(o):
+        for a in range(v):
+            for P in range(naux):
+                x = Lpq[P, i, o + a]
+                bc = 0
+                for b in range(v):
+                    for c in range(b + 1):
+                        ovvv[i, a, bc] += x * Lpq[P, o + b, o + c]
+                        bc += 1
+                for j in range(o):
+                    for k in range(o):
+                        ovoo[i, a, j, k] += x * Lpq[P, j, k]
+                    for b in range(v):
+                        ovov[i, a, j, b] += x * Lpq[P, j, o + b]
+    return ovvv, ovoo, ovov

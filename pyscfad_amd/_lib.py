@@ -21,13 +21,14 @@ STATUS_NAMES = {0: 'ok', 1: 'bad argument', 2: 'HIP runtime error', 3: 'workspac
 
 PRECISIONS = {'fp64': 0, 'bf16x3': 1, 'fp32x2': 2}
 
-# exported symbols of include/pyscfad_amd_ccsdt.h and include/pyscfad_amd_mlp.h
+# exported symbols of include/pyscfad_amd_ccsdt.h, _profile.h, _debug.h, _mlp.h and _eris.h
 EXPORTS = ('pamd_version', 'pamd_ccsdt_workspace_size', 'pamd_ccsdt_energy',
            'pamd_ccsdt_energy_vjp', 'pamd_ccsdt_pair_energy', 'pamd_profile_enable',
            'pamd_profile_reset', 'pamd_profile_progress', 'pamd_profile_num_kinds', 'pamd_profile_kind_name',
            'pamd_profile_read', 'pamd_debug_gemm_descriptors', 'pamd_debug_split_factors',
            'pamd_bf16_gemm_tn', 'pamd_bf16_act_fwd', 'pamd_bf16_act_bwd', 'pamd_adam_step',
-           'pamd_mse', 'pamd_f32_to_bf16')
+           'pamd_mse', 'pamd_f32_to_bf16', 'pamd_mp2_init_amps', 'pamd_df_eris_workspace_size',
+           'pamd_df_eris', 'pamd_df_eris_vjp', 'pamd_debug_df_descriptors')
 
 
 class Config(ctypes.Structure):
@@ -73,6 +74,17 @@ def load():
     lib.pamd_profile_read.argtypes = [i, ctypes.POINTER(ctypes.c_longlong)] + \
         [ctypes.POINTER(ctypes.c_double)] * 3
     lib.pamd_profile_read.restype = i
+    lib.pamd_mp2_init_amps.restype = i
+    lib.pamd_mp2_init_amps.argtypes = [i, i] + [vp] * 5
+    lib.pamd_df_eris_workspace_size.restype = i
+    lib.pamd_df_eris_workspace_size.argtypes = [i, i, i, i, ctypes.POINTER(sz)]
+    lib.pamd_df_eris.restype = i
+    lib.pamd_df_eris.argtypes = [i, i, i] + [vp] * 5 + [sz, vp]
+    lib.pamd_df_eris_vjp.restype = i
+    lib.pamd_df_eris_vjp.argtypes = [i, i, i] + [vp] * 6 + [sz, vp]
+    lib.pamd_debug_df_descriptors.restype = i
+    lib.pamd_debug_df_descriptors.argtypes = [i, i, i, i, i, vp, vp, i, ctypes.POINTER(i),
+                                              ctypes.POINTER(i)]
     _lib = lib
     return lib
 

@@ -24,11 +24,11 @@
 #include <vector>
 #include <chrono>
 #include "ccsdt_common.h"
+#include "gemm_desc.h"
 #include "../../include/pyscfad_amd_ccsdt.h"
 
 namespace pamd {
 
-static const int kBig = 1 << 30;
 static const int kGmax = 1024;
 
 // When set, GEMM launches are recorded instead of run and the other per-triple
@@ -136,79 +136,6 @@ static hipError_t run_gemm(const GemmDesc &d, hipStream_t s, int kind) {
   if (g_prec == PAMD_PREC_BF16X3)
     return timed(s, kind, gemm_flops(d), 0.0, [&] { return launch_gemm_bf16x3(d, s); });
   return timed(s, kind, gemm_flops(d), 0.0, [&] { return launch_gemm_f64(d, s); });
-}
-
-static OpDesc op(const double *p, long long rx, long long ry, int rdiv, long long kx, long long ky,
-                 int kdiv, int rfast) {
-  OpDesc d;
-  d.p = p;
-  d.rx = rx;
-  d.ry = ry;
-  d.rdiv = rdiv;
-  d.kx = kx;
-  d.ky = ky;
-  d.kdiv = kdiv;
-  d.rfast = rfast;
-  d.pad_ = 0;
-  return d;
-}
-// row index is a single index with stride ry; K single with stride ky
-static OpDesc op1(const double *p, long long ry, long long ky, int rfast) {
-  return op(p, 0, ry, kBig, 0, ky, kBig, rfast);
-}
-// row index is a pair (r / rdiv, r % rdiv); K single
-static OpDesc opR(const double *p, long long rx, long long ry, int rdiv, long long ky, int rfast) {
-  return op(p, rx, ry, rdiv, 0, ky, kBig, rfast);
-}
-// row index single; K is a pair (k / kdiv, k % kdiv)
-static OpDesc opK(const double *p, long long ry, long long kx, long long ky, int kdiv, int rfast) {
-  return op(p, 0, ry, kBig, kx, ky, kdiv, rfast);
-}
-
-static SegDesc seg(OpDesc A, OpDesc B, int L, double alpha) {
-  SegDesc s;
-  s.A = A;
-  s.B = B;
-  s.L = L;
-  s.pad_ = 0;
-  s.alpha = alpha;
-  return s;
-}
-
-static OutDesc out(double *p, long long mx, long long my, int mdiv, long long nx, long long ny,
-                   int ndiv) {
-  OutDesc c;
-  c.p = p;
-  c.mx = mx;
-  c.my = my;
-  c.mdiv = mdiv;
-  c.nx = nx;
-  c.ny = ny;
-  c.ndiv = ndiv;
-  return c;
-}
-
-static const int KC = 16;
-
-static GemmDesc gemm(int M, int N, int mode, std::initializer_list<SegDesc> segs, OutDesc C,
-                     int splits = 1, double *partial = nullptr) {
-  GemmDesc d;
-  memset(&d, 0, sizeof(d));
-  d.M = M;
-  d.N = N;
-  d.mode = mode;
-  d.nseg = 0;
-  d.chunk_start[0] = 0;
-  for (const SegDesc &s : segs) {
-    d.seg[d.nseg] = s;
-    d.chunk_start[d.nseg + 1] = d.chunk_start[d.nseg] + (s.L + KC - 1) / KC;
-    ++d.nseg;
-  }
-  for (int q = d.nseg + 1; q <= kMaxSeg; ++q) d.chunk_start[q] = d.chunk_start[d.nseg];
-  d.splits = std::max(1, std::min(splits, d.chunk_start[d.nseg]));
-  d.C = C;
-  d.partial = partial;
-  return d;
 }
 
 struct Layout {

@@ -74,7 +74,7 @@ def build(seeds, device, log=None, every=128):
     for c, s in enumerate(seeds):
         p = iset.instance(int(s))
         ep = exact_pairs(p)
-        X.append(features(p.mo_energy, p.t2, p.ovov, p.nocc))
+        X.append(features(p.mo_energy, p.ovov, p.nocc))
         Y.append(fold_pairs(ep))
         E.append(ep.sum())
         del p

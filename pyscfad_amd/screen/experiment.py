@@ -63,7 +63,7 @@ def main(argv=None):
             ep = D.exact_pairs(p)
             torch.cuda.synchronize()
             t_exact.append(time.perf_counter() - t0)
-            X.append(features(p.mo_energy, p.t2, p.ovov, p.nocc))
+            X.append(features(p.mo_energy, p.ovov, p.nocc))
             Y.append(fold_pairs(ep))
             E.append(ep.sum())
             del p

@@ -78,10 +78,37 @@ def _decomp(idx, div, sx, sy):
     return q * sx + (idx - q * div) * sy
 
 
+DF_NAMES = ('Lpq', 'Lvv', 'Lvv_bar', 'ovvv', 'ovoo', 'ovov', 'Lpq_bar')
+
+
+def df_sizes(o, nmo, naux):
+    v = nmo - o
+    npair = v * (v + 1) // 2
+    return dict(Lpq=naux * nmo * nmo, Lvv=naux * npair, Lvv_bar=naux * npair,
+                ovvv=o * v * npair, ovoo=o * v * o * o, ovov=(o * v) ** 2,
+                Lpq_bar=naux * nmo * nmo)
+
+
+def record_df(o, nmo, naux, with_vjp, block_i=0):
+    """Descriptors of pamd_df_eris (with_vjp = False) or pamd_df_eris_vjp."""
+    lib = _lib.load()
+    bases = (ctypes.c_ulonglong * 7)(*[(n + 1) << 40 for n in range(7)])
+    n = ctypes.c_int()
+    nb = ctypes.c_int()
+    buf = (GemmDesc * 64)()
+    st = lib.pamd_debug_df_descriptors(o, nmo, naux, int(with_vjp), block_i, bases, buf, 64,
+                                       ctypes.byref(n), ctypes.byref(nb))
+    assert st == 0, st
+    assert nb.value == ctypes.sizeof(GemmDesc)
+    return [buf[q] for q in range(n.value)]
+
+
 class Memory:
-    def __init__(self, o, v):
-        self.arrays = {name: np.zeros(sz) for name, sz in sizes(o, v).items()}
-        self.base = {name: (n + 1) << 40 for n, name in enumerate(NAMES)}
+    def __init__(self, o=None, v=None, names=NAMES, array_sizes=None):
+        if array_sizes is None:
+            array_sizes = sizes(o, v)
+        self.arrays = {name: np.zeros(array_sizes[name]) for name in names}
+        self.base = {name: (n + 1) << 40 for n, name in enumerate(names)}
 
     def locate(self, addr_bytes):
         """map byte addresses to (array name, element offsets); all must hit one array"""

@@ -154,7 +154,7 @@ def test_screen_gain_small_instances(cuda):
         ep = D.exact_pairs(p)
         torch.cuda.synchronize()
         t_exact += time.perf_counter() - t0
-        X.append(features(p.mo_energy, p.t2, p.ovov, o))
+        X.append(features(p.mo_energy, p.ovov, o))
         Y.append(fold_pairs(ep))
         E.append(ep.sum())
     X, Y, E = torch.stack(X), torch.stack(Y), torch.stack(E)

@@ -5,8 +5,7 @@ import torch
 
 from pyscfad_amd.gen import make_problem
 from pyscfad_amd.screen import data as D
-from pyscfad_amd.screen.features import (features, fold_pairs, mp2_pair_energies, n_features,
-                                         n_labels)
+from pyscfad_amd.screen.features import fold_pairs, n_features, n_labels
 
 
 def test_sizes_of_the_screen_workload():
@@ -14,25 +13,6 @@ def test_sizes_of_the_screen_workload():
     tr, te = D.split()
     assert len(tr) == 1800 and len(te) == 248
     assert len(set(tr) | set(te)) == 2048
-
-
-def test_mp2_pair_energies_match_the_mp2_energy():
-    p = make_problem(4, 12, 40, seed=2)
-    t2, ovov = torch.from_numpy(p.t2), torch.from_numpy(p.ovov)
-    e = mp2_pair_energies(t2, ovov).numpy()
-    # E(MP2) = sum_ijab t2 (2 (ia|jb) - (ib|ja)), written out with loops over pairs
-    ref = np.zeros((4, 4))
-    for i in range(4):
-        for j in range(4):
-            ref[i, j] = (p.t2[i, j] * (2 * p.ovov[i, :, j, :] - p.ovov[i, :, j, :].T)).sum()
-    assert np.allclose(e, ref, rtol=1e-13, atol=0)
-    f = features(torch.from_numpy(p.mo_energy), t2, ovov, 4).numpy()
-    assert f.shape == (n_features(4),)
-    assert np.isclose(f[-1], ref.sum(), rtol=1e-13)
-    iu, ju = np.triu_indices(4, 1)
-    assert np.allclose(f[:6], (ref + ref.T)[iu, ju])
-    eo, ev = p.mo_energy[:4], p.mo_energy[4:]
-    assert np.isclose(f[6], ev.min() - eo.max())
 
 
 def test_fold_pairs_preserves_the_sum():
