@@ -72,11 +72,19 @@ struct OrbitArgs {
   double *part_mo;      // [G][n] per-workgroup mo_energy_bar accumulators (vjp)
 };
 
+// V-bar reductions: the sums over one virtual index are split into kVRS parts
+// (more workgroups in flight); the scratch R holds every part.
+constexpr int kVRS = 4;
+inline size_t vr_scratch_doubles(int v) {
+  const size_t vv = (size_t)v * v;
+  return 10 * vv + 2 * (size_t)v + (size_t)(kVRS - 1) * (6 * vv + 2 * (size_t)v);
+}
+
 // V-bar reductions for one triple.
 struct VRArgs {
   const double *Vb, *t1, *fvo, *ovs, *t2s;  // ovs, t2s: pair-symmetrised blocks (OrbitArgs)
   int o, v, i, j, k, has_fvo;
-  double *R;            // scratch: 10 v^2 + 2 v doubles (layout in kernels)
+  double *R;            // scratch: vr_scratch_doubles(v) (layout in kernels)
 };
 
 // Per-triple gather of the adjoint pieces into the output cotangents.

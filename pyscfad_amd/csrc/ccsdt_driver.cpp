@@ -207,7 +207,7 @@ static Layout make_layout(int o, int v, int vjp, int prec) {
     L.off_Vpb = take((size_t)o * v3);
     L.off_T2mb = take((size_t)o * o * v2);
     L.off_part_mo = take((size_t)kGmax * n);
-    L.off_R = take(10 * v2 + 2 * (size_t)v);
+    L.off_R = take(vr_scratch_doubles(v));
     L.off_part_p = take((size_t)6 * L.S_p * v2);
     L.off_part_h = take((size_t)6 * L.S_h * v * o);
   }

@@ -489,11 +489,17 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
                *t2s_jk = A.t2s + ((size_t)j * o + k) * vv;
   const double *t1i = A.t1 + (size_t)i * v, *t1j = A.t1 + (size_t)j * v, *t1k = A.t1 + (size_t)k * v;
   const double *fvo = A.fvo;
-  const int l_bca = (be << 6) | (ga << 3) | al;
-  const int l_cab = (ga << 6) | (al << 3) | be;
-  const int l_cba = (ga << 6) | (be << 3) | al;
-  const int l_acb = (al << 6) | (ga << 3) | be;
-  const int l_bac = (be << 6) | (al << 3) | ga;
+  // LDS slot of tile position (a, b, c) = 64 a + 8 b + c, with the low five
+  // bits XOR-swizzled by a function of a: then every one of the six permuted
+  // read patterns below (a wave spans two of the three tile axes) spreads its
+  // 64 lanes evenly over the banks, as the identity pattern does.
+  auto slot = [](int pos) { const int a = pos >> 6; return pos ^ (a | ((a & 3) << 3)); };
+  const int l_id = slot(tid);
+  const int l_bca = slot((be << 6) | (ga << 3) | al);
+  const int l_cab = slot((ga << 6) | (al << 3) | be);
+  const int l_cba = slot((ga << 6) | (be << 3) | al);
+  const int l_acb = slot((al << 6) | (ga << 3) | be);
+  const int l_bac = slot((be << 6) | (al << 3) | ga);
 
   for (long long q = blockIdx.x; q < A.norb; q += gridDim.x) {
     const unsigned short *ent = A.tab + 4 * q;
@@ -514,8 +520,8 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
           vv_ += t2s_ij[ab] * fvo[c * o + k] + t2s_ik[ac] * fvo[b * o + j] + t2s_jk[bc] * fvo[a * o + i];
         xv = wv + 0.5 * vv_;
       }
-      Ws[p][tid] = wv;
-      Xs[p][tid] = xv;
+      Ws[p][l_id] = wv;
+      Xs[p][l_id] = xv;
     }
     __syncthreads();
     // phase 2
@@ -526,10 +532,10 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
       if (a < v && b < v && c < v) {
         const int p0 = pt.pmap[0][p], p1 = pt.pmap[1][p], p2 = pt.pmap[2][p], p3 = pt.pmap[3][p],
                   p4 = pt.pmap[4][p];
-        const double w0 = Ws[p][tid];
+        const double w0 = Ws[p][l_id];
         const double w1 = Ws[p0][l_bca], w2 = Ws[p1][l_cab], w3 = Ws[p2][l_cba],
                      w4 = Ws[p3][l_acb], w5 = Ws[p4][l_bac];
-        const double x0 = Xs[p][tid];
+        const double x0 = Xs[p][l_id];
         const double x1 = Xs[p0][l_bca], x2 = Xs[p1][l_cab], x3 = Xs[p2][l_cba],
                      x4 = Xs[p3][l_acb], x5 = Xs[p4][l_bac];
         const double Z = 4.0 * x0 + x1 + x2 - 2.0 * (x3 + x4 + x5);
@@ -615,6 +621,9 @@ hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s) {
 //   9 Fj[a,b] = sum_c Vb (t2[k,i,c,a] + t2[i,k,a,c])
 //  10*v^2 + a : t1i[a] = sum_bc Vb (ovov[j,b,k,c] + ovov[k,c,j,b])
 //  10*v^2 + v + a : fvoi[a] = sum_bc Vb (t2[k,j,c,b] + t2[j,k,b,c])
+// Blocks 2-7 and the two vectors are sums over b (k_vr_a) or a (k_vr_b); the
+// kVRS parts of those sums are stored separately (parts 1.. after the slots
+// above: vr_scratch_doubles) and added in part order by the consumers.
 // ---------------------------------------------------------------------------
 constexpr int kMaxVQ = 8;  // v <= 512 for the adjoint
 
@@ -671,11 +680,40 @@ __device__ double block_sum_vr(double x, double *scratch) {
   return r;
 }
 
+// Part s of kVRS of the b-sums (k_vr_a) and a-sums (k_vr_b): arrays that are
+// sums over the split index are written per part; part 0 uses the slots of R
+// listed above, parts 1.. an extension after them (vr_part, vr_part_vec).
+__device__ __forceinline__ double *vr_part(double *R, int v, int s, int q) {
+  const size_t vv = (size_t)v * v;
+  if (s == 0) return R + (size_t)q * vv;
+  return R + 10 * vv + 2 * (size_t)v + ((size_t)(s - 1) * 6 + (q - 2)) * vv;
+}
+__device__ __forceinline__ double *vr_part_vec(double *R, int v, int s, int which) {
+  const size_t vv = (size_t)v * v;
+  if (s == 0) return R + 10 * vv + (size_t)which * v;
+  return R + 10 * vv + 2 * (size_t)v + (size_t)(kVRS - 1) * 6 * vv + ((size_t)(s - 1) * 2 + which) * v;
+}
+
+// sums over the parts, in part order
+__device__ __forceinline__ double vr_sum(const double *R, int v, int q, size_t idx) {
+  double x = 0.0;
+#pragma unroll
+  for (int s = 0; s < kVRS; ++s) x += vr_part(const_cast<double *>(R), v, s, q)[idx];
+  return x;
+}
+__device__ __forceinline__ double vr_sum_vec(const double *R, int v, int which, int idx) {
+  double x = 0.0;
+#pragma unroll
+  for (int s = 0; s < kVRS; ++s) x += vr_part_vec(const_cast<double *>(R), v, s, which)[idx];
+  return x;
+}
+
+template <bool FVO>
 __global__ __launch_bounds__(64 * kVRW) void k_vr_a(VRArgs A) {
   __shared__ double red[kVRW * 512];
   __shared__ double cst[4][512];  // per-c constants: t1[k,c], fvo[c,k], S2[a,c], T2[a,c]
   __shared__ double sred[kVRW];
-  const int a = blockIdx.x;
+  const int a = blockIdx.x, part = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int o = A.o, v = A.v, i = A.i, j = A.j, k = A.k;
   const size_t vv = (size_t)v * v;
@@ -685,7 +723,7 @@ __global__ __launch_bounds__(64 * kVRW) void k_vr_a(VRArgs A) {
   for (int c = tid; c < v; c += 64 * kVRW) {
     cst[0][c] = t1[k * v + c];
     cst[2][c] = SYM(ovs, i, k, a, c);
-    cst[1][c] = A.has_fvo ? fvo[c * o + k] : 0.0;
+    cst[1][c] = FVO ? fvo[c * o + k] : 0.0;
     cst[3][c] = SYM(t2s, i, k, a, c);
   }
   __syncthreads();
@@ -694,10 +732,10 @@ __global__ __launch_bounds__(64 * kVRW) void k_vr_a(VRArgs A) {
   for (int qq = 0; qq < kMaxVQ; ++qq) r3[qq] = r4[qq] = pk[qq] = fk[qq] = 0.0;
   double t1i = 0.0, fvoi = 0.0;
   double *R = A.R;
-  for (int b = wave; b < v; b += kVRW) {
+  for (int b = wave + kVRW * part; b < v; b += kVRW * kVRS) {
     const double t1j = t1[j * v + b];
     const double S1 = SYM(ovs, i, j, a, b);
-    const double fvj = A.has_fvo ? fvo[b * o + j] : 0.0;
+    const double fvj = FVO ? fvo[b * o + j] : 0.0;
     const double T1 = SYM(t2s, i, j, a, b);
     const double *sjk = ovs + (((size_t)j * o + k) * v + b) * v;
     const double *tjk = t2s + (((size_t)j * o + k) * v + b) * v;
@@ -717,7 +755,7 @@ __global__ __launch_bounds__(64 * kVRW) void k_vr_a(VRArgs A) {
         sfj += x * cst[3][c];
         fk[qq] += x * T1;
         fvoi += x * tjk[c];
-        if (A.has_fvo) {
+        if (FVO) {
           sr2 += x * cst[1][c];
           r4[qq] += x * fvj;
         }
@@ -727,26 +765,27 @@ __global__ __launch_bounds__(64 * kVRW) void k_vr_a(VRArgs A) {
     const double s4 = wave_sum4(sr1, sr2, spj, sfj);
     if ((lane & 15) == 0) {
       const int q = lane >> 4;
-      R[(size_t)(q < 2 ? q : q + 6) * vv + (size_t)a * v + b] = s4;
+      if (FVO || q != 1) R[(size_t)(q < 2 ? q : q + 6) * vv + (size_t)a * v + b] = s4;
     }
   }
   __syncthreads();
-  PAMD_WAVE_COMBINE(r3, R + 2 * vv + (size_t)a * v);
-  PAMD_WAVE_COMBINE(r4, R + 3 * vv + (size_t)a * v);
-  PAMD_WAVE_COMBINE(pk, R + 6 * vv + (size_t)a * v);
-  PAMD_WAVE_COMBINE(fk, R + 7 * vv + (size_t)a * v);
+  PAMD_WAVE_COMBINE(r3, vr_part(R, v, part, 2) + (size_t)a * v);
+  if (FVO) PAMD_WAVE_COMBINE(r4, vr_part(R, v, part, 3) + (size_t)a * v);
+  PAMD_WAVE_COMBINE(pk, vr_part(R, v, part, 6) + (size_t)a * v);
+  PAMD_WAVE_COMBINE(fk, vr_part(R, v, part, 7) + (size_t)a * v);
   const double t1i_s = block_sum_vr(t1i, sred);
   const double fvoi_s = block_sum_vr(fvoi, sred);
   if (tid == 0) {
-    R[10 * vv + a] = t1i_s;
-    R[10 * vv + v + a] = fvoi_s;
+    vr_part_vec(R, v, part, 0)[a] = t1i_s;
+    vr_part_vec(R, v, part, 1)[a] = fvoi_s;
   }
 #undef SYM
 }
 
+template <bool FVO>
 __global__ __launch_bounds__(64 * kVRW) void k_vr_b(VRArgs A) {
   __shared__ double red[kVRW * 512];
-  const int b = blockIdx.x;
+  const int b = blockIdx.x, part = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int o = A.o, v = A.v, i = A.i;
   const size_t vv = (size_t)v * v;
@@ -754,9 +793,9 @@ __global__ __launch_bounds__(64 * kVRW) void k_vr_b(VRArgs A) {
   double r5[kMaxVQ], r6[kMaxVQ];
 #pragma unroll
   for (int qq = 0; qq < kMaxVQ; ++qq) r5[qq] = r6[qq] = 0.0;
-  for (int a = wave; a < v; a += kVRW) {
+  for (int a = wave + kVRW * part; a < v; a += kVRW * kVRS) {
     const double t1a = A.t1[i * v + a];
-    const double fa = A.has_fvo ? A.fvo[a * o + i] : 0.0;
+    const double fa = FVO ? A.fvo[a * o + i] : 0.0;
     const double *row = A.Vb + (size_t)a * vv + (size_t)b * v;
 #pragma unroll
     for (int qq = 0; qq < kMaxVQ; ++qq) {
@@ -764,19 +803,25 @@ __global__ __launch_bounds__(64 * kVRW) void k_vr_b(VRArgs A) {
       if (qq < nq && c < v) {
         const double x = row[c];
         r5[qq] += x * t1a;
-        r6[qq] += x * fa;
+        if (FVO) r6[qq] += x * fa;
       }
     }
   }
-  PAMD_WAVE_COMBINE(r5, A.R + 4 * vv + (size_t)b * v);
-  PAMD_WAVE_COMBINE(r6, A.R + 5 * vv + (size_t)b * v);
+  PAMD_WAVE_COMBINE(r5, vr_part(A.R, v, part, 4) + (size_t)b * v);
+  if (FVO) PAMD_WAVE_COMBINE(r6, vr_part(A.R, v, part, 5) + (size_t)b * v);
 }
 #undef PAMD_WAVE_COMBINE
 
 hipError_t launch_vr(const VRArgs &a, hipStream_t s) {
   if (a.v > 64 * kMaxVQ) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_vr_a, dim3(a.v), dim3(64 * kVRW), 0, s, a);
-  hipLaunchKernelGGL(k_vr_b, dim3(a.v), dim3(64 * kVRW), 0, s, a);
+  const dim3 grid(a.v, kVRS), block(64 * kVRW);
+  if (a.has_fvo) {
+    hipLaunchKernelGGL(k_vr_a<true>, grid, block, 0, s, a);
+    hipLaunchKernelGGL(k_vr_b<true>, grid, block, 0, s, a);
+  } else {
+    hipLaunchKernelGGL(k_vr_a<false>, grid, block, 0, s, a);
+    hipLaunchKernelGGL(k_vr_b<false>, grid, block, 0, s, a);
+  }
   return hipGetLastError();
 }
 
@@ -799,10 +844,10 @@ __global__ void k_combine(CombineArgs A) {
     };
     ob(i, j) += 0.5 * R[0 * vv + pq];
     ob(j, i) += 0.5 * R[0 * vv + qp];
-    ob(i, k) += 0.5 * R[2 * vv + pq];
-    ob(k, i) += 0.5 * R[2 * vv + qp];
-    ob(j, k) += 0.5 * R[4 * vv + pq];
-    ob(k, j) += 0.5 * R[4 * vv + qp];
+    ob(i, k) += 0.5 * vr_sum(R, v, 2, pq);
+    ob(k, i) += 0.5 * vr_sum(R, v, 2, qp);
+    ob(j, k) += 0.5 * vr_sum(R, v, 4, pq);
+    ob(k, j) += 0.5 * vr_sum(R, v, 4, qp);
     // t2_bar[x,y,p,q]: particle split-K partials (problem order P1..P6) + V terms
     auto tb = [&](int x, int y) -> double & {
       return A.t2_bar[(((size_t)x * o + y) * v + p) * v + q];
@@ -813,10 +858,10 @@ __global__ void k_combine(CombineArgs A) {
       double s = A.part_p[(size_t)pr * A.S_p * vv + pq];  // split-K sum (k_splitk_reduce)
       if (A.has_fvo) {
         switch (pr) {
-          case 0: s += 0.5 * R[5 * vv + qp]; break;  // t2_bar[k,j,c,b] += R6[b,c]/2
-          case 1: s += 0.5 * R[5 * vv + pq]; break;  // t2_bar[j,k,b,c] += R6[b,c]/2
-          case 2: s += 0.5 * R[3 * vv + qp]; break;  // t2_bar[k,i,c,a] += R4[a,c]/2
-          case 3: s += 0.5 * R[3 * vv + pq]; break;  // t2_bar[i,k,a,c] += R4[a,c]/2
+          case 0: s += 0.5 * vr_sum(R, v, 5, qp); break;  // t2_bar[k,j,c,b] += R6[b,c]/2
+          case 1: s += 0.5 * vr_sum(R, v, 5, pq); break;  // t2_bar[j,k,b,c] += R6[b,c]/2
+          case 2: s += 0.5 * vr_sum(R, v, 3, qp); break;  // t2_bar[k,i,c,a] += R4[a,c]/2
+          case 3: s += 0.5 * vr_sum(R, v, 3, pq); break;  // t2_bar[i,k,a,c] += R4[a,c]/2
           case 4: s += 0.5 * R[1 * vv + qp]; break;  // t2_bar[j,i,b,a] += R2[a,b]/2
           case 5: s += 0.5 * R[1 * vv + pq]; break;  // t2_bar[i,j,a,b] += R2[a,b]/2
         }
@@ -850,9 +895,9 @@ __global__ __launch_bounds__(256) void k_vr_colsums(CombineArgs A) {
   if (c < v) {
     for (int a = wave; a < v; a += 4) {
       const size_t ac = (size_t)a * v + c;
-      sk += R[6 * vv + ac];
+      sk += vr_sum(R, v, 6, ac);
       sj += R[8 * vv + ac];
-      fk += R[7 * vv + ac];
+      fk += vr_sum(R, v, 7, ac);
       fj += R[9 * vv + ac];
     }
   }
@@ -867,10 +912,10 @@ __global__ __launch_bounds__(256) void k_vr_colsums(CombineArgs A) {
     for (int q = 0; q < 4; ++q) s[q] = red[q][0][lane] + red[q][1][lane] + red[q][2][lane] + red[q][3][lane];
     A.t1_bar[k * v + c] += 0.5 * s[0];
     A.t1_bar[j * v + c] += 0.5 * s[1];
-    A.t1_bar[i * v + c] += 0.5 * R[10 * vv + c];
+    A.t1_bar[i * v + c] += 0.5 * vr_sum_vec(R, v, 0, c);
     A.fvo_bar[(size_t)c * o + k] += 0.5 * s[2];
     A.fvo_bar[(size_t)c * o + j] += 0.5 * s[3];
-    A.fvo_bar[(size_t)c * o + i] += 0.5 * R[10 * vv + v + c];
+    A.fvo_bar[(size_t)c * o + i] += 0.5 * vr_sum_vec(R, v, 1, c);
   }
 }
 

@@ -10,7 +10,7 @@ namespace pamd {
 // MP2 amplitudes and pair energies (pyscfad/cc/ccsd.py:364-381).
 // One workgroup per unordered pair i <= j streams the two (a, b) planes
 // ovov[i,:,j,:] and ovov[j,:,i,:] once, forms both amplitude planes
-// t2[i,j] and t2[j,i] (same denominators), and reduces
+// t2[i,j] and t2[j,i], and reduces
 //   e_ij = sum_ab (2 t2[i,j,a,b] - t2[j,i,a,b]) ovov[i,a,j,b]
 //   e_ji = sum_ab (2 t2[j,i,a,b] - t2[i,j,a,b]) ovov[j,a,i,b]
 // in a fixed order (per-thread strided sums, then a tree over the workgroup).
@@ -32,12 +32,14 @@ __global__ void __launch_bounds__(kMp2Threads) k_mp2_pairs(int o, int v, const d
   for (size_t x = threadIdx.x; x < vv; x += kMp2Threads) {
     const int a = (int)(x / v), b = (int)(x - (size_t)a * v);
     const double ea = mo[o + a], eb = mo[o + b];
-    // eia[i,a] + eia[j,b] as the reference forms the denominator
-    const double d = (ei - ea) + (ej - eb);
+    // eia[x,a] + eia[y,b], grouped as the reference forms the denominator of
+    // t2[x,y,a,b] (the two planes differ in rounding, not in value)
+    const double dij = (ei - ea) + (ej - eb);
+    const double dji = (ej - ea) + (ei - eb);
     const double gij = ovov[((size_t)i * v + a) * ov + (size_t)j * v + b];
     const double gji = ovov[((size_t)j * v + a) * ov + (size_t)i * v + b];
-    const double tij = gij / d;
-    const double tji = gji / d;
+    const double tij = gij / dij;
+    const double tji = gji / dji;
     sij += (2.0 * tij - tji) * gij;
     sji += (2.0 * tji - tij) * gji;
     if (t2) {
