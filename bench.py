@@ -81,6 +81,38 @@ def engine_roofline(prof, precision):
     return roofline
 
 
+def df_measure(o, v, naux, seed, dev, inp, grads):
+    """Time pamd_df_eris / pamd_df_eris_vjp on the workload's own DF factors: the
+    forward rebuilds the (T) blocks, the adjoint folds the step's cotangents
+    into dE/dLpq.  Reported beside the (T) step, not part of it."""
+    import torch
+    from pyscfad_amd.cc import df as dfm
+    from pyscfad_amd.gen import make_L
+    L = make_L(o, v, naux, seed=seed, device=dev)
+    n = o + v
+    npair = v * (v + 1) // 2
+    f_fwd = 2.0 * naux * o * v * (npair + o * o + o * v)
+    f_vjp = 2.0 * naux * o * v * (o * o + (npair + o * o + 2 * o * v) + npair)
+    dfm.df_eris(L[:, :min(n, 8), :min(n, 8)].contiguous(), min(o, 2))  # load, warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = dfm.df_eris(L, o)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    diff = max(float((out[k] - ref).abs().max() / ref.abs().max())
+               for k, ref in (('ovvv', inp.ovvv), ('ovoo', inp.ovoo), ('ovov', inp.ovov)))
+    del out
+    t0 = time.perf_counter()
+    Lb = dfm.df_eris_vjp(L, o, grads['ovvv_bar'], grads['ovoo_bar'], grads['ovov_bar'])
+    torch.cuda.synchronize()
+    t_vjp = time.perf_counter() - t0
+    del Lb, L
+    return {'workload': f'Lpq ({naux}, {n}, {n}) -> packed ovvv, ovoo, ovov and back',
+            'fwd_s': t_fwd, 'vjp_s': t_vjp, 'fwd_tflops': f_fwd / t_fwd / 1e12,
+            'vjp_tflops': f_vjp / t_vjp / 1e12, 'frac_fp64_peak_fwd': f_fwd / t_fwd / 1e12 / PEAKS['fp64'],
+            'max_rel_diff_vs_generator': diff}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument('--gpus', type=int, default=1)
@@ -100,6 +132,8 @@ def main():
     ap.add_argument('--cpu-sample', default='12,120', help='nocc,nvir of the CPU baseline sample')
     ap.add_argument('--cpu-threads', type=int, default=16)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-df', action='store_true',
+                    help='skip the timing of the DF integral producer and its adjoint')
     ap.add_argument('--energy-only', action='store_true', help='time E(T) without the VJP')
     ap.add_argument('--out', default=None, help='also write the JSON line to this file')
     ap.add_argument('--calib-copy-mib', type=int, default=0,
@@ -205,6 +239,9 @@ def main():
                 k: float(torch.linalg.norm(grads[k] - g64[k]) / torch.linalg.norm(g64[k]))
                 for k in g64}
         del g64
+    df = None
+    if not args.no_df and not args.energy_only:
+        df = df_measure(o, v, naux, args.seed, dev, inp, grads)
     cpu = None
     if not args.no_cpu_baseline:
         so, sv = (int(x) for x in args.cpu_sample.split(','))
@@ -237,6 +274,7 @@ def main():
         'input_gen_s': t_gen,
         'roofline': roofline,
         'compare_fp64': compare,
+        'df_eris': df,
         'cpu_baseline': cpu,
         'kernels': {k: dict(x, avg_ms=(x['ms'] / x['launches'] if x['launches'] else None))
                     for k, x in prof.items()},

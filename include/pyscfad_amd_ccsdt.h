@@ -89,6 +89,26 @@ int pamd_ccsdt_pair_energy(const pamd_ccsdt_config *cfg,
                            void *workspace, size_t workspace_bytes,
                            double *e_pair, void *stream);
 
+/* Occupied-weighted E(T) for a diagonal occupied projector mat = diag(m) and
+ * its adjoint: the weighted (T) of lnoccsdt_contract (lno_ccsd_t.c:6-29,
+ * 96-138, with et *= 2/3 of pyscfad/lno/ccsd_t.py:104) and of its VJP
+ * (lno_ccsd_t_vjp.c:400-566) when mat is diagonal.
+ *   e_out = E_m = sum_i m_i E_i,   E_i = sum_j e_pair[i,j] (pamd_ccsdt_pair_energy)
+ * occ_weight: HOST array of nocc doubles (m).  occ_weight_bar: device, nocc
+ * doubles, overwritten with et_bar * E_i -- the derivative along the diagonal
+ * of mat (the off-diagonal part of the reference's mat_bar is not formed).
+ * The other cotangents are those of pamd_ccsdt_energy_vjp for E_m; with m = 1
+ * they equal pamd_ccsdt_energy_vjp's bit for bit. */
+int pamd_ccsdt_weighted_energy_vjp(const pamd_ccsdt_config *cfg,
+                                   const double *mo_energy, const double *t1, const double *t2,
+                                   const double *ovvv, const double *ovoo, const double *ovov,
+                                   const double *fvo, const double *occ_weight, double et_bar,
+                                   void *workspace, size_t workspace_bytes,
+                                   double *e_out, double *mo_energy_bar, double *t1_bar,
+                                   double *t2_bar, double *ovvv_bar, double *ovoo_bar,
+                                   double *ovov_bar, double *fvo_bar, double *occ_weight_bar,
+                                   void *stream);
+
 /* Library version string (static storage). */
 const char *pamd_version(void);
 

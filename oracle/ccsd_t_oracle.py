@@ -112,8 +112,15 @@ def pair_energy(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo):
     return np.einsum('ijkabc->ij', W * Z / D) / 3.0
 
 
-def energy_vjp(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, et_bar=1.0):
+def energy_vjp(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, et_bar=1.0, occ_weight=None):
     """(E, cotangents) with ``et_bar`` the cotangent of E.
+
+    ``occ_weight`` m (nocc,) gives the occupied-weighted energy of a diagonal
+    projector, E_m = 1/3 sum m_i W Z / D = sum_i m_i sum_j pair_energy[i, j]
+    (lnoccsdt_contract with mat = diag(m), lno_ccsd_t.c:16-26).  As
+    sum_abc W Z / D is symmetric in (i, j, k), E_m also equals the sum with the
+    symmetric weight mu_ijk = (m_i + m_j + m_k) / 3, and that form's seed
+    mu / 3 replaces 1/3 in the adjoint below.
 
     Returns the raw-convention gradient blocks in the reference's output layout
     (pyscfad/cc/ccsd_t.py:179-191): ``t1_bar`` (o,v), ``t2_bar`` (o,o,v,v),
@@ -123,8 +130,15 @@ def energy_vjp(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, et_bar=1.0):
     """
     nocc, nvir = t1.shape
     W, V, Z, D = _forward(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo)
-    e = float(np.sum(W * Z / D) / 3.0)
-    g = et_bar / 3.0
+    if occ_weight is None:
+        e = float(np.sum(W * Z / D) / 3.0)
+        g = et_bar / 3.0
+    else:
+        m = np.asarray(occ_weight, dtype=np.float64)
+        mu = (m[:, None, None] + m[None, :, None] + m[None, None, :]) / 3.0
+        mu = mu[:, :, :, None, None, None]
+        e = float(np.sum(mu * W * Z / D) / 3.0)
+        g = et_bar * mu / 3.0
     Y = W / D
     r3Y = r3(Y)
     Wbar = g * (Z / D + r3Y)

@@ -135,3 +135,38 @@ def energy_vjp(inp: Inputs, et_bar=1.0):
         _ptr(out['fvo_bar']), _stream(dev))
     _lib.check(st, 'pamd_ccsdt_energy_vjp')
     return e, out
+
+
+def weighted_energy_vjp(inp: Inputs, occ_weight, et_bar=1.0):
+    """(E_m, cotangents) for the diagonal occupied projector diag(m):
+    E_m = sum_i m_i E_i with E_i the row sums of ``pair_energy``; the cotangents
+    include ``occ_weight_bar`` = et_bar * E_i."""
+    lib = _lib.load()
+    o, v = inp.nocc, inp.nvir
+    m = np.ascontiguousarray(np.asarray(
+        occ_weight.detach().cpu() if isinstance(occ_weight, torch.Tensor) else occ_weight,
+        dtype=np.float64))
+    if m.shape != (o,):
+        raise ValueError(f'occ_weight must have {o} entries, got {m.shape}')
+    dev = inp.device
+    ws, nbytes = inp.workspace(True)
+    f64 = dict(dtype=torch.float64, device=dev)
+    e = torch.zeros((), **f64)
+    out = dict(
+        mo_energy_bar=torch.empty(o + v, **f64),
+        t1_bar=torch.empty((o, v), **f64),
+        t2_bar=torch.empty((o, o, v, v), **f64),
+        ovvv_bar=torch.empty(tuple(inp.ovvv.shape), **f64),
+        ovoo_bar=torch.empty((o, v, o, o), **f64),
+        ovov_bar=torch.empty((o, v, o, v), **f64),
+        fvo_bar=torch.empty((v, o), **f64),
+        occ_weight_bar=torch.empty(o, **f64),
+    )
+    st = lib.pamd_ccsdt_weighted_energy_vjp(
+        ctypes.byref(inp.cfg), *inp.args(), m.ctypes.data_as(ctypes.c_void_p),
+        ctypes.c_double(float(et_bar)), _ptr(ws), nbytes, _ptr(e), _ptr(out['mo_energy_bar']),
+        _ptr(out['t1_bar']), _ptr(out['t2_bar']), _ptr(out['ovvv_bar']), _ptr(out['ovoo_bar']),
+        _ptr(out['ovov_bar']), _ptr(out['fvo_bar']), _ptr(out['occ_weight_bar']), _stream(dev))
+    _lib.check(st, 'pamd_ccsdt_weighted_energy_vjp')
+    torch.cuda.current_stream(dev).synchronize()   # m is read from host memory
+    return e, out

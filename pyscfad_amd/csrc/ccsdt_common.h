@@ -120,5 +120,7 @@ hipError_t launch_triple_energy(const double *part_t, int G, double *dst, hipStr
 int orbit_grid(int v);
 hipError_t launch_pair_energy(const double *tri_e, int o, double *e_pair, hipStream_t s);
 hipError_t launch_total_energy(const double *tri_e, int o, double *e_out, hipStream_t s);
+hipError_t launch_weighted_energy(const double *tri_e, int o, const double *m, double ebar,
+                                  double *e_out, double *m_bar, hipStream_t s);
 
 }  // namespace pamd

@@ -140,7 +140,7 @@ static hipError_t run_gemm(const GemmDesc &d, hipStream_t s, int kind) {
 
 struct Layout {
   size_t off_Vp, off_T2m, off_OOt, off_OVS, off_T2S, off_orb, off_W, off_tri, off_part_t, off_T2g, off_W8;
-  size_t off_Vb, off_Vpb, off_T2mb, off_part_mo, off_R, off_part_p, off_part_h;
+  size_t off_Vb, off_Vpb, off_T2mb, off_part_mo, off_R, off_part_p, off_part_h, off_wt;
   size_t total;
   int S_p, S_h;
 };
@@ -158,14 +158,16 @@ static int device_cus() {
   return cus;
 }
 
-// Split-K factor for an M x N output with `chunks` K chunks of 16: three full
-// rounds of resident workgroups (two per CU for the FP64 engine, one for the
-// split-BF16x3 engine), rounded down so that the last round is not a tail.
+// Split-K factor for an M x N output with `chunks` K chunks of 16: one full
+// round of resident workgroups (two per CU for the FP64 engine, one for the
+// split-BF16x3 engine), rounded down so that no workgroup waits for a second
+// round.  (One round measured faster than two or three: fewer split-K
+// partials to write and reduce, same engine time.)
 static int splits_for(int M, int N, int chunks, int prec) {
   const bool b3 = prec == PAMD_PREC_BF16X3;
   const int tiles = std::max(1, b3 ? gemm_tiles_b3(M, N) : gemm_tiles(M, N));
   const int slots = (b3 ? 1 : 2) * device_cus();
-  int s = 3 * slots / tiles;
+  int s = slots / tiles;
   s = std::max(1, std::min(s, 512));
   return std::min(s, std::max(1, b3 ? chunks / 2 : chunks));
 }
@@ -195,6 +197,7 @@ static Layout make_layout(int o, int v, int vjp, int prec) {
   L.off_W = take(v3);
   L.off_tri = take((size_t)o * o * o);
   L.off_part_t = take(kGmax);
+  L.off_wt = take((size_t)o);
   if (prec == PAMD_PREC_BF16X3) {
     L.off_T2g = take((size_t)o * o * v2);
     if (vjp) L.off_W8 = take(v3);
@@ -226,6 +229,8 @@ struct Ctx {
   double *Vb, *Vpb, *T2mb, *part_mo, *R, *part_p, *part_h;
   int S_p, S_h, G;
   double ebar;
+  const double *wt_host;  // occupied weights m (host) of a diagonal projector, or null
+  double *wt;             // device copy of m
   // outputs
   double *t1_bar, *t2_bar, *ovoo_bar, *ovov_bar, *fvo_bar;
 };
@@ -492,7 +497,9 @@ static int run(Ctx &c) {
         oa.vjp = c.vjp;
         oa.norb = norb;
         oa.omega = (i == j || j == k) ? 1.0 : 2.0;
-        oa.ebar = c.ebar;
+        // a diagonal occupied projector diag(m) weighs the whole triple by
+        // (m_i + m_j + m_k) / 3 (the triple energy is symmetric in i, j, k)
+        oa.ebar = c.wt_host ? c.ebar * ((c.wt_host[i] + c.wt_host[j] + c.wt_host[k]) / 3.0) : c.ebar;
         oa.part_t = c.part_t;
         oa.part_mo = c.part_mo;
         {
@@ -542,6 +549,7 @@ static void bind_workspace(Ctx &c, const Layout &L, void *ws) {
   c.W = (double *)(b + L.off_W);
   c.tri = (double *)(b + L.off_tri);
   c.part_t = (double *)(b + L.off_part_t);
+  c.wt = (double *)(b + L.off_wt);
   c.gT2 = c.prec == PAMD_PREC_BF16X3 ? (const double *)(b + L.off_T2g) : c.t2;
   c.gW = (c.prec == PAMD_PREC_BF16X3 && c.vjp) ? (double *)(b + L.off_W8) : c.W;
   if (c.vjp) {
@@ -628,12 +636,12 @@ int pamd_ccsdt_pair_energy(const pamd_ccsdt_config *cfg, const double *mo_energy
   return PAMD_OK;
 }
 
-int pamd_ccsdt_energy_vjp(const pamd_ccsdt_config *cfg, const double *mo_energy, const double *t1,
-                          const double *t2, const double *ovvv, const double *ovoo,
-                          const double *ovov, const double *fvo, double et_bar, void *workspace,
-                          size_t workspace_bytes, double *e_out, double *mo_energy_bar,
-                          double *t1_bar, double *t2_bar, double *ovvv_bar, double *ovoo_bar,
-                          double *ovov_bar, double *fvo_bar, void *stream) {
+static int vjp_impl(const pamd_ccsdt_config *cfg, const double *mo_energy, const double *t1,
+                    const double *t2, const double *ovvv, const double *ovoo, const double *ovov,
+                    const double *fvo, const double *wt_host, double et_bar, void *workspace,
+                    size_t workspace_bytes, double *e_out, double *mo_energy_bar, double *t1_bar,
+                    double *t2_bar, double *ovvv_bar, double *ovoo_bar, double *ovov_bar,
+                    double *fvo_bar, double *wt_bar, void *stream) {
   int st = check_common(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo);
   if (st != PAMD_OK) return st;
   if (!e_out || !mo_energy_bar || !t1_bar || !t2_bar || !ovvv_bar || !ovoo_bar || !ovov_bar ||
@@ -659,6 +667,7 @@ int pamd_ccsdt_energy_vjp(const pamd_ccsdt_config *cfg, const double *mo_energy,
   c.ovov = ovov;
   c.fvo = fvo;
   c.ebar = et_bar;
+  c.wt_host = wt_host;
   c.t1_bar = t1_bar;
   c.t2_bar = t2_bar;
   c.ovoo_bar = ovoo_bar;
@@ -668,11 +677,42 @@ int pamd_ccsdt_energy_vjp(const pamd_ccsdt_config *cfg, const double *mo_energy,
   st = run(c);
   if (st != PAMD_OK) return st;
   const int o = c.o, v = c.v;
-  PAMD_CHECK(launch_total_energy(c.tri, o, e_out, c.s));
+  if (wt_host) {
+    PAMD_CHECK(hipMemcpyAsync(c.wt, wt_host, sizeof(double) * (size_t)o, hipMemcpyHostToDevice,
+                              c.s));
+    PAMD_CHECK(launch_weighted_energy(c.tri, o, c.wt, et_bar, e_out, wt_bar, c.s));
+  } else {
+    PAMD_CHECK(launch_total_energy(c.tri, o, e_out, c.s));
+  }
   PAMD_CHECK(launch_reduce_rows(c.part_mo, c.G, o + v, mo_energy_bar, 0, c.s));
   PAMD_CHECK(launch_fold_ovvv(c.Vpb, o, v, c.packed, ovvv_bar, c.s));
   PAMD_CHECK(launch_fold_t2m(c.T2mb, o, v, t2_bar, c.s));
   return PAMD_OK;
+}
+
+int pamd_ccsdt_energy_vjp(const pamd_ccsdt_config *cfg, const double *mo_energy, const double *t1,
+                          const double *t2, const double *ovvv, const double *ovoo,
+                          const double *ovov, const double *fvo, double et_bar, void *workspace,
+                          size_t workspace_bytes, double *e_out, double *mo_energy_bar,
+                          double *t1_bar, double *t2_bar, double *ovvv_bar, double *ovoo_bar,
+                          double *ovov_bar, double *fvo_bar, void *stream) {
+  return vjp_impl(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, nullptr, et_bar, workspace,
+                  workspace_bytes, e_out, mo_energy_bar, t1_bar, t2_bar, ovvv_bar, ovoo_bar,
+                  ovov_bar, fvo_bar, nullptr, stream);
+}
+
+int pamd_ccsdt_weighted_energy_vjp(const pamd_ccsdt_config *cfg, const double *mo_energy,
+                                   const double *t1, const double *t2, const double *ovvv,
+                                   const double *ovoo, const double *ovov, const double *fvo,
+                                   const double *occ_weight, double et_bar, void *workspace,
+                                   size_t workspace_bytes, double *e_out, double *mo_energy_bar,
+                                   double *t1_bar, double *t2_bar, double *ovvv_bar,
+                                   double *ovoo_bar, double *ovov_bar, double *fvo_bar,
+                                   double *occ_weight_bar, void *stream) {
+  if (!occ_weight || !occ_weight_bar) return PAMD_ERR_ARG;
+  return vjp_impl(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, occ_weight, et_bar, workspace,
+                  workspace_bytes, e_out, mo_energy_bar, t1_bar, t2_bar, ovvv_bar, ovoo_bar,
+                  ovov_bar, fvo_bar, occ_weight_bar, stream);
 }
 
 int pamd_debug_gemm_descriptors(int nocc, int nvir, int i, int j, int k, int with_vjp,

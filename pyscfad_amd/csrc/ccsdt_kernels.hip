@@ -1057,6 +1057,46 @@ hipError_t launch_total_energy(const double *tri_e, int o, double *e_out, hipStr
   return hipGetLastError();
 }
 
+// Weighted total for a diagonal occupied projector diag(m):
+//   E_m = sum_t w_t tri_e[t] (m_i + m_j + m_k) / 3 over i <= j <= k (w_t as k_total_energy),
+//   m_bar[x] = ebar * E_x, E_x = 1/3 sum_{y,z} tri_e[sort(x,y,z)] (row sums of e_pair).
+// One workgroup, fixed summation order.
+__global__ void k_weighted_energy(const double *tri_e, int o, const double *m, double ebar,
+                                  double *e_out, double *m_bar) {
+  __shared__ double red[4];
+  double s = 0.0;
+  const int n3 = o * o * o;
+  for (int t = threadIdx.x; t < n3; t += 256) {
+    const int i = t / (o * o), j = (t / o) % o, k = t % o;
+    if (i <= j && j <= k) {
+      const double w = (i == j && j == k) ? (1.0 / 3.0) : ((i == j || j == k) ? 1.0 : 2.0);
+      s += w * tri_e[t] * ((m[i] + m[j] + m[k]) / 3.0);
+    }
+  }
+  s = block_sum256(s, red);
+  if (threadIdx.x == 0) *e_out = s;
+  if (m_bar) {
+    for (int x = threadIdx.x; x < o; x += 256) {
+      double r = 0.0;
+      for (int y = 0; y < o; ++y)
+        for (int z = 0; z < o; ++z) {
+          int a = x, b = y, c = z, tmp;
+          if (a > b) { tmp = a; a = b; b = tmp; }
+          if (b > c) { tmp = b; b = c; c = tmp; }
+          if (a > b) { tmp = a; a = b; b = tmp; }
+          r += tri_e[((size_t)a * o + b) * o + c];
+        }
+      m_bar[x] = ebar * (r / 3.0);
+    }
+  }
+}
+
+hipError_t launch_weighted_energy(const double *tri_e, int o, const double *m, double ebar,
+                                  double *e_out, double *m_bar, hipStream_t s) {
+  hipLaunchKernelGGL(k_weighted_energy, dim3(1), dim3(256), 0, s, tri_e, o, m, ebar, e_out, m_bar);
+  return hipGetLastError();
+}
+
 // e_pair[x][y] = 1/3 sum_z tri_e[sort(x,y,z)]
 __global__ void k_pair_energy(const double *tri_e, int o, double *e_pair) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;

@@ -122,6 +122,19 @@ def make_problem(nocc, nvir, naux, seed=0, fvo_scale=0.0, device=None,
     return assemble_torch(L, mo_energy, fock, t1, nocc, nvir)
 
 
+def make_L(nocc, nvir, naux, seed=0, device=None):
+    """The symmetric DF factors L (naux, n, n) that ``make_problem`` draws for
+    the same arguments (numpy, or torch on ``device``)."""
+    (iu, ju, tri), *_ = _draw(nocc, nvir, naux, seed)
+    n = nocc + nvir
+    if device is None:
+        return _unpack_L(iu, ju, tri, naux, n, np)
+    import torch
+    dev = torch.device(device)
+    return unpack_L_torch(torch.from_numpy(iu).to(dev), torch.from_numpy(ju).to(dev),
+                          torch.from_numpy(tri).to(dev), naux, n)
+
+
 def unpack_L_torch(iu_t, ju_t, tri_t, naux, n):
     import torch
     L = torch.zeros((naux, n, n), dtype=torch.float64, device=tri_t.device)

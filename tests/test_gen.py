@@ -34,3 +34,13 @@ def test_fvo_and_packing():
     assert pk.shape == (3, 9, 45)
     idx, idy = np.tril_indices(9)
     assert np.array_equal(pk, p.ovvv[:, :, idx, idy])
+
+
+def test_make_L_is_the_factor_behind_make_problem():
+    from pyscfad_amd.gen import make_L
+    o, v, naux = 3, 7, 11
+    L = make_L(o, v, naux, seed=2)
+    p = make_problem(o, v, naux, seed=2)
+    assert np.array_equal(L, np.swapaxes(L, 1, 2))
+    Lov = L[:, :o, o:].reshape(naux, -1)
+    assert np.array_equal((Lov.T @ Lov).reshape(o, v, o, v), p.ovov)

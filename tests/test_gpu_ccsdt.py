@@ -85,3 +85,34 @@ def test_pair_energy(cuda):
     assert np.abs(ep.sum(axis=1) - z['e_occ']).max() <= E_TOL
     ref = O.pair_energy(p.mo_energy, p.t1, p.t2, p.ovvv, p.ovoo, p.ovov, p.fvo)
     assert np.abs(ep - ref).max() <= E_TOL
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'bf16x3'])
+def test_weighted_energy_vjp(precision, cuda):
+    """Diagonal occupied projector: energy and every cotangent against the
+    oracle's weighted adjoint; occ_weight_bar against the golden occupied-weighted
+    (T) of the reference; m = 1 against the plain adjoint, bit for bit."""
+    import torch
+    z = np.load(os.path.join(GOLDEN, 'ccsd_t_o6v20_fvo.npz'))
+    p = make_problem(6, 20, 80, seed=int(z['seed']), fvo_scale=float(z['fvo_scale']))
+    print(f'weighted vjp o6v20 {precision}', flush=True)
+    args = (p.mo_energy, p.t1, p.t2, p.ovvv, p.ovoo, p.ovov, p.fvo)
+    m = np.array([0.9, 0.1, 0.0, 0.5, 1.0, 0.3])
+    e_ref, g_ref = O.energy_vjp(*args, et_bar=0.7, occ_weight=m)
+    inp = _ops.Inputs(p.mo_energy, p.t1, p.t2, p.ovvv_packed(), p.ovoo, p.ovov, p.fvo,
+                      precision=precision, device=cuda)
+    e, g = _ops.weighted_energy_vjp(inp, m, et_bar=0.7)
+    etol, gtol = (E_TOL, G_RTOL) if precision == 'fp64' else (1e-9, 1e-5)
+    assert abs(float(e.item()) - e_ref) <= etol
+    for k in BLOCKS:
+        assert _rel(g[k].cpu().numpy(), g_ref[k]) <= gtol, k
+    assert _rel(g['ovvv_bar'].cpu().numpy(), O.fold_ovvv_packed(g_ref['ovvv_bar'])) <= gtol
+    assert np.abs(g['occ_weight_bar'].cpu().numpy() - 0.7 * z['e_occ']).max() <= etol
+    e1, g1 = _ops.weighted_energy_vjp(inp, np.ones(6), et_bar=0.7)
+    e0, g0 = _ops.energy_vjp(inp, et_bar=0.7)
+    assert abs(float(e1.item()) - float(e0.item())) <= 1e-14 * abs(float(e0.item()))
+    for k in g0:
+        if k == 'mo_energy_bar':   # virtual part summed with LDS atomics: order may vary
+            assert _rel(g1[k].cpu().numpy(), g0[k].cpu().numpy()) <= 1e-13
+        else:
+            assert torch.equal(g1[k], g0[k]), k

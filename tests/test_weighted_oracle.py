@@ -1,0 +1,66 @@
+"""The occupied-weighted (T) of a diagonal projector in the oracle: energy
+against the pair energies and the spin-orbital projector form, adjoint
+against central differences, and m = 1 against the plain adjoint."""
+import numpy as np
+
+from oracle import ccsd_t_oracle as O
+from oracle import gccsd_t_oracle as G
+from pyscfad_amd.gen import make_L, make_problem
+
+BLOCKS = ('t1', 't2', 'ovvv', 'ovoo', 'ovov', 'fvo')
+
+
+def _args(p):
+    return (p.mo_energy, p.t1, p.t2, p.ovvv, p.ovoo, p.ovov, p.fvo)
+
+
+def test_weighted_energy_is_the_weighted_pair_row_sum():
+    p = make_problem(4, 9, 30, seed=5, fvo_scale=1e-3)
+    m = np.array([0.3, 1.0, 0.0, 0.7])
+    e_m, _ = O.energy_vjp(*_args(p), occ_weight=m)
+    pairs = O.pair_energy(*_args(p))
+    assert np.isclose(e_m, (m * pairs.sum(1)).sum(), rtol=1e-13)
+    # the spin-orbital form with the projector rows sqrt(m_i) e_i
+    L = make_L(4, 9, 30, seed=5)
+    e_so = G.energy(L, p.mo_energy, p.fock, p.t1, p.t2, 4, prj=np.diag(np.sqrt(m)))
+    assert abs(e_so - e_m) <= 1e-12 * abs(e_m)
+
+
+def test_unit_weights_give_the_plain_adjoint():
+    p = make_problem(3, 8, 24, seed=6)
+    e1, g1 = O.energy_vjp(*_args(p))
+    e2, g2 = O.energy_vjp(*_args(p), occ_weight=np.ones(3))
+    assert np.isclose(e1, e2, rtol=1e-14)
+    for k in g1:
+        np.testing.assert_allclose(g2[k], g1[k], rtol=1e-13, atol=1e-18)
+
+
+def test_weighted_adjoint_central_differences():
+    p = make_problem(3, 7, 20, seed=8, fvo_scale=1e-3)
+    m = np.array([0.2, 0.9, 0.5])
+    _e, g = O.energy_vjp(*_args(p), occ_weight=m)
+    grads = dict(t1=g['t1_bar'], t2=g['t2_bar'], ovvv=g['ovvv_bar'], ovoo=g['ovoo_bar'],
+                 ovov=g['ovov_bar'], fvo=g['fvo_bar'])
+    rng = np.random.default_rng(1)
+    x = dict(t1=p.t1, t2=p.t2, ovvv=p.ovvv, ovoo=p.ovoo, ovov=p.ovov, fvo=p.fvo)
+    h = 1e-5
+    for _ in range(3):
+        d = {k: rng.normal(size=x[k].shape) for k in BLOCKS}
+        nrm = np.sqrt(sum((t * t).sum() for t in d.values()))
+        d = {k: t / nrm for k, t in d.items()}
+
+        def em(s):
+            y = {k: x[k] + s * h * d[k] for k in BLOCKS}
+            return O.energy_vjp(p.mo_energy, y['t1'], y['t2'], y['ovvv'], y['ovoo'], y['ovov'],
+                                y['fvo'], occ_weight=m)[0]
+        fd = (em(1) - em(-1)) / (2 * h)
+        ana = sum((grads[k] * d[k]).sum() for k in BLOCKS)
+        assert abs(fd - ana) <= 1e-6 * abs(fd), (fd, ana)
+    # mo_energy: the derivative along each orbital energy
+    for q in (0, 4):
+        e = p.mo_energy.copy()
+        e[q] += h
+        ep = O.energy_vjp(e, *_args(p)[1:], occ_weight=m)[0]
+        e[q] -= 2 * h
+        emn = O.energy_vjp(e, *_args(p)[1:], occ_weight=m)[0]
+        assert abs((ep - emn) / (2 * h) - g['mo_energy_bar'][q]) <= 1e-6 * abs(g['mo_energy_bar'][q])
