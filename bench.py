@@ -65,8 +65,8 @@ def engine_roofline(prof, precision):
     g_n = sum(x['launches'] for x in gemm)
     fp64_equiv = g_fl / (g_ms * 1e-3) / 1e12 if g_ms > 0 else None
     if precision == 'bf16x3':
-        # six bf16 32x32x16 MFMA products per FP64 multiply-add
-        kernel, mfma_factor, peak = 'k_gemm_b3', 6.0, PEAKS['bf16']
+        # six bf16 MFMA partial products per FP64-equivalent multiply-add
+        kernel, mfma_factor, peak = 'k_gemm_b3 / k_gemm_b3n', 6.0, PEAKS['bf16']
     else:
         kernel, mfma_factor, peak = 'k_gemm_f64', 1.0, PEAKS['fp64']
     achieved = fp64_equiv * mfma_factor if fp64_equiv else None
