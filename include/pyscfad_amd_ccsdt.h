@@ -109,6 +109,22 @@ int pamd_ccsdt_weighted_energy_vjp(const pamd_ccsdt_config *cfg,
                                    double *ovov_bar, double *fvo_bar, double *occ_weight_bar,
                                    void *stream);
 
+/* Occupied-weighted E(T) of a local-orbital projector, forward only:
+ * pyscfad/lno/ccsd_t.py:35-60 (mat = ulo^T ulo) with lnoccsdt_contract
+ * (lno_ccsd_t.c:6-29, 96-138) and et *= 2/3 (lno/ccsd_t.py:104):
+ *   E = 1/3 sum_{l,i,j,k,abc} mat[l,i] W_ljk[abc] Z_ijk[abc] / D_ijk[abc].
+ * ulo: HOST array (nfrag, nocc), 1 <= nfrag <= 8.  e_out: device scalar.
+ * The walk takes every unordered occupied pair (j, k) and all o triples
+ * (l, j, k), so it runs about three times the W builds of pamd_ccsdt_energy.
+ * Workspace: pamd_ccsdt_projected_workspace_size. */
+int pamd_ccsdt_projected_workspace_size(const pamd_ccsdt_config *cfg, int nfrag, size_t *bytes);
+int pamd_ccsdt_projected_energy(const pamd_ccsdt_config *cfg,
+                                const double *mo_energy, const double *t1, const double *t2,
+                                const double *ovvv, const double *ovoo, const double *ovov,
+                                const double *fvo, const double *ulo, int nfrag,
+                                void *workspace, size_t workspace_bytes, double *e_out,
+                                void *stream);
+
 /* Library version string (static storage). */
 const char *pamd_version(void);
 

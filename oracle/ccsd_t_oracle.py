@@ -112,6 +112,20 @@ def pair_energy(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo):
     return np.einsum('ijkabc->ij', W * Z / D) / 3.0
 
 
+def projected_energy(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, ulo):
+    """Occupied-weighted E(T) of the local-orbital projector mat = ulo^T ulo.
+
+    pyscfad/lno/ccsd_t.py:42 forms ``mat = ulo.T @ ulo``; lno_ccsd_t.c:16-26
+    contracts W over its first occupied index with mat before the energy
+    product: E = 1/3 sum_{l,i,j,k,abc} mat[l,i] W[l,j,k,abc] Z[i,j,k,abc] / D[i,j,k,abc]
+    (mat = I gives E(T); mat = e_i e_i^T gives the row sums of ``pair_energy``).
+    """
+    ulo = np.asarray(ulo, dtype=np.float64)
+    mat = ulo.T @ ulo
+    W, V, Z, D = _forward(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo)
+    return float(np.einsum('li,ljkabc,ijkabc->', mat, W, Z / D, optimize=True) / 3.0)
+
+
 def energy_vjp(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, et_bar=1.0, occ_weight=None):
     """(E, cotangents) with ``et_bar`` the cotangent of E.
 

@@ -170,3 +170,27 @@ def weighted_energy_vjp(inp: Inputs, occ_weight, et_bar=1.0):
     _lib.check(st, 'pamd_ccsdt_weighted_energy_vjp')
     torch.cuda.current_stream(dev).synchronize()   # m is read from host memory
     return e, out
+
+
+def projected_energy(inp: Inputs, ulo):
+    """Occupied-weighted E(T) of the local-orbital projector mat = ulo^T ulo
+    (ulo: (nfrag, nocc), nfrag <= 8), as a 0-d float64 device tensor."""
+    lib = _lib.load()
+    u = np.ascontiguousarray(np.asarray(
+        ulo.detach().cpu() if isinstance(ulo, torch.Tensor) else ulo, dtype=np.float64))
+    if u.ndim == 1:
+        u = u[None, :]
+    if u.ndim != 2 or u.shape[1] != inp.nocc:
+        raise ValueError(f'ulo must be (nfrag, {inp.nocc}), got {u.shape}')
+    nfrag = int(u.shape[0])
+    n = ctypes.c_size_t(0)
+    _lib.check(lib.pamd_ccsdt_projected_workspace_size(ctypes.byref(inp.cfg), nfrag,
+                                                       ctypes.byref(n)),
+               'pamd_ccsdt_projected_workspace_size')
+    ws = torch.empty(max(n.value, 1), dtype=torch.uint8, device=inp.device)
+    e = torch.zeros((), dtype=torch.float64, device=inp.device)
+    st = lib.pamd_ccsdt_projected_energy(ctypes.byref(inp.cfg), *inp.args(),
+                                         u.ctypes.data_as(ctypes.c_void_p), nfrag, _ptr(ws),
+                                         n.value, _ptr(e), _stream(inp.device))
+    _lib.check(st, 'pamd_ccsdt_projected_energy')
+    return e

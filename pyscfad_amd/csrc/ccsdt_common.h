@@ -56,6 +56,8 @@ struct GemmDesc {
   double *partial;               // kPartial: partial[(split * M + m) * N + n]
 };
 
+constexpr int kMaxFrag = 8;  // rows of the local-orbital projector ulo
+
 // Epilogue over orbits of virtual tiles for one occupied triple.
 struct OrbitArgs {
   double *W;            // v^3; W in (and Wbar out when Wb8 is null, vjp)
@@ -70,6 +72,11 @@ struct OrbitArgs {
   double ebar;          // cotangent of E
   double *part_t;       // [G] per-workgroup unweighted triple energy (overwritten)
   double *part_mo;      // [G][n] per-workgroup mo_energy_bar accumulators (vjp)
+  // projector accumulation (pamd_ccsdt_projected_energy): when accA is set,
+  // accA[f] += u[f] W and accB[f] += u[f] Z / D over the v^3 block, f < nfrag
+  double *accA, *accB;
+  int nfrag;
+  double u[kMaxFrag];
 };
 
 // V-bar reductions: the sums over one virtual index are split into kVRS parts
@@ -120,6 +127,9 @@ hipError_t launch_triple_energy(const double *part_t, int G, double *dst, hipStr
 int orbit_grid(int v);
 hipError_t launch_pair_energy(const double *tri_e, int o, double *e_pair, hipStream_t s);
 hipError_t launch_total_energy(const double *tri_e, int o, double *e_out, hipStream_t s);
+hipError_t launch_dot(const double *a, const double *b, size_t n, double *part, int G, double *dst,
+                      hipStream_t s);
+hipError_t launch_projected_total(const double *pair_e, int o, double *e_out, hipStream_t s);
 hipError_t launch_weighted_energy(const double *tri_e, int o, const double *m, double ebar,
                                   double *e_out, double *m_bar, hipStream_t s);
 

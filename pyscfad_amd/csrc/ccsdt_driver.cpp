@@ -442,7 +442,8 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
   return PAMD_OK;
 }
 
-static int run(Ctx &c) {
+// Layout changes at entry, shared by every walk; returns the orbit count.
+static int setup(Ctx &c, long long &norb) {
   const int o = c.o, v = c.v;
   g_prec = c.prec;
   const int bf = c.prec == PAMD_PREC_BF16X3;
@@ -453,9 +454,90 @@ static int run(Ctx &c) {
     PAMD_CHECK(launch_split_bf16x3(c.t2, (size_t)o * o * v * v, (unsigned long long *)c.gT2, c.s));
   PAMD_CHECK(launch_fill(c.tri, (size_t)o * o * o, 0.0, c.s));
   const int nt = (v + 7) / 8;
-  const long long norb = (long long)nt * (nt + 1) * (nt + 2) / 6;
+  norb = (long long)nt * (nt + 1) * (nt + 2) / 6;
   PAMD_CHECK(launch_orbit_table(norb, c.orb, c.s));
   c.G = std::min(orbit_grid(v), kGmax);
+  return PAMD_OK;
+}
+
+static OrbitArgs orbit_args(const Ctx &c, int i, int j, int k, long long norb) {
+  OrbitArgs oa;
+  memset(&oa, 0, sizeof(oa));
+  oa.W = c.W;
+  oa.tab = c.orb;
+  oa.Vb = c.Vb;
+  oa.t1 = c.t1;
+  oa.ovs = c.OVS;
+  oa.t2s = c.T2S;
+  oa.fvo = c.fvo;
+  oa.mo = c.mo;
+  oa.o = c.o;
+  oa.v = c.v;
+  oa.i = i;
+  oa.j = j;
+  oa.k = k;
+  oa.has_fvo = c.has_fvo;
+  oa.norb = norb;
+  oa.part_t = c.part_t;
+  oa.part_mo = c.part_mo;
+  return oa;
+}
+
+// Projected walk: for every unordered occupied pair (j, k), all o triples
+// (l, j, k) in order of l.  Each W_ljk is built, and the orbit epilogue adds
+// ulo[f, l] W and ulo[f, l] Z / D into accA[f], accB[f]; then
+// pair_e[j, k] = sum_f <accA[f], accB[f]> = sum_{l,i} mat[l,i] <W_ljk, Z_ijk / D_ijk>.
+static int run_projected(Ctx &c, const double *ulo, int nfrag, double *accA, double *accB,
+                         double *pair_e, double *dot_part) {
+  long long norb = 0;
+  int st = setup(c, norb);
+  if (st != PAMD_OK) return st;
+  const int o = c.o;
+  const size_t v3 = (size_t)c.v * c.v * c.v;
+  long long tcount = 0;
+  const long long nbuild = (long long)o * o * (o + 1) / 2;
+  const auto t_start = std::chrono::steady_clock::now();
+  for (int j = 0; j < o; ++j)
+    for (int k = j; k < o; ++k) {
+      PAMD_CHECK(launch_fill(accA, (size_t)nfrag * v3, 0.0, c.s));
+      PAMD_CHECK(launch_fill(accB, (size_t)nfrag * v3, 0.0, c.s));
+      for (int l = 0; l < o; ++l) {
+        g_prof_on = g_prof_every > 0 && (tcount % g_prof_every) == 0;
+        ++tcount;
+        st = build_w(c, l, j, k);
+        if (st != PAMD_OK) return st;
+        OrbitArgs oa = orbit_args(c, l, j, k, norb);
+        oa.accA = accA;
+        oa.accB = accB;
+        oa.nfrag = nfrag;
+        for (int f = 0; f < nfrag; ++f) oa.u[f] = ulo[(size_t)f * o + l];
+        const double ob = 8.0 * (double)v3 * (1.0 + 4.0 * nfrag);  // W read, 2 nfrag RMW
+        PAMD_CHECK(timed(c.s, kProfOrbit, 0.0, ob, [&] { return launch_orbit(oa, c.G, c.s); }));
+        if (g_progress_every > 0 && tcount % g_progress_every == 0) {
+          PAMD_CHECK(hipStreamSynchronize(c.s));
+          const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() -
+                                                           t_start).count();
+          fprintf(stderr, "pyscfad_amd: projected (T) triples %lld/%lld done, %.1f s\n", tcount,
+                  nbuild, el);
+          fflush(stderr);
+        }
+      }
+      PAMD_CHECK(launch_dot(accA, accB, (size_t)nfrag * v3, dot_part, kGmax, pair_e + j * o + k,
+                            c.s));
+    }
+  g_prof_on = false;
+  if (!g_prof_recs.empty()) PAMD_CHECK(prof_collect());
+  return PAMD_OK;
+}
+
+static int run(Ctx &c) {
+  const int o = c.o, v = c.v;
+  const int bf = c.prec == PAMD_PREC_BF16X3;
+  long long norb = 0;
+  {
+    const int st0 = setup(c, norb);
+    if (st0 != PAMD_OK) return st0;
+  }
   if (c.vjp) {
     const size_t v2 = (size_t)v * v;
     PAMD_CHECK(launch_fill(c.Vpb, (size_t)o * v2 * v, 0.0, c.s));
@@ -478,30 +560,13 @@ static int run(Ctx &c) {
         ++tcount;
         int st = build_w(c, i, j, k);
         if (st != PAMD_OK) return st;
-        OrbitArgs oa;
-        oa.W = c.W;
+        OrbitArgs oa = orbit_args(c, i, j, k, norb);
         oa.Wb8 = (bf && c.vjp) ? (unsigned long long *)c.gW : nullptr;
-        oa.tab = c.orb;
-        oa.Vb = c.Vb;
-        oa.t1 = c.t1;
-        oa.ovs = c.OVS;
-        oa.t2s = c.T2S;
-        oa.fvo = c.fvo;
-        oa.mo = c.mo;
-        oa.o = o;
-        oa.v = v;
-        oa.i = i;
-        oa.j = j;
-        oa.k = k;
-        oa.has_fvo = c.has_fvo;
         oa.vjp = c.vjp;
-        oa.norb = norb;
         oa.omega = (i == j || j == k) ? 1.0 : 2.0;
         // a diagonal occupied projector diag(m) weighs the whole triple by
         // (m_i + m_j + m_k) / 3 (the triple energy is symmetric in i, j, k)
         oa.ebar = c.wt_host ? c.ebar * ((c.wt_host[i] + c.wt_host[j] + c.wt_host[k]) / 3.0) : c.ebar;
-        oa.part_t = c.part_t;
-        oa.part_mo = c.part_mo;
         {
           const double v3 = (double)v * v * v;
           const double ob = 8.0 * v3 * (c.vjp ? 3.0 : 1.0);  // W read (+ Wbar, Vbar written)
@@ -713,6 +778,66 @@ int pamd_ccsdt_weighted_energy_vjp(const pamd_ccsdt_config *cfg, const double *m
   return vjp_impl(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, occ_weight, et_bar, workspace,
                   workspace_bytes, e_out, mo_energy_bar, t1_bar, t2_bar, ovvv_bar, ovoo_bar,
                   ovov_bar, fvo_bar, occ_weight_bar, stream);
+}
+
+// workspace of the projected walk: the forward layout, then accA, accB
+// (nfrag v^3 each), the pair energies (o^2) and the dot partials (kGmax)
+static size_t projected_extra(int o, int v, int nfrag, size_t *off_a, size_t *off_b,
+                              size_t *off_p, size_t *off_d, size_t base) {
+  const size_t blk = align256(sizeof(double) * (size_t)nfrag * v * v * v);
+  *off_a = base;
+  *off_b = *off_a + blk;
+  *off_p = *off_b + blk;
+  *off_d = *off_p + align256(sizeof(double) * (size_t)o * o);
+  return *off_d + align256(sizeof(double) * kGmax);
+}
+
+int pamd_ccsdt_projected_workspace_size(const pamd_ccsdt_config *cfg, int nfrag, size_t *bytes) {
+  if (!cfg || !bytes || cfg->nocc < 1 || cfg->nvir < 1 || nfrag < 1 || nfrag > kMaxFrag)
+    return PAMD_ERR_ARG;
+  size_t a, b, p, d;
+  *bytes = projected_extra(cfg->nocc, cfg->nvir, nfrag, &a, &b, &p, &d,
+                           make_layout(cfg->nocc, cfg->nvir, 0, cfg->precision).total);
+  return PAMD_OK;
+}
+
+int pamd_ccsdt_projected_energy(const pamd_ccsdt_config *cfg, const double *mo_energy,
+                                const double *t1, const double *t2, const double *ovvv,
+                                const double *ovoo, const double *ovov, const double *fvo,
+                                const double *ulo, int nfrag, void *workspace,
+                                size_t workspace_bytes, double *e_out, void *stream) {
+  int st = check_common(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo);
+  if (st != PAMD_OK) return st;
+  if (!ulo || !e_out || nfrag < 1 || nfrag > kMaxFrag) return PAMD_ERR_ARG;
+  Layout L = make_layout(cfg->nocc, cfg->nvir, 0, cfg->precision);
+  size_t off_a, off_b, off_p, off_d;
+  const size_t total = projected_extra(cfg->nocc, cfg->nvir, nfrag, &off_a, &off_b, &off_p,
+                                       &off_d, L.total);
+  if (!workspace || workspace_bytes < total) return PAMD_ERR_WORKSPACE;
+  Ctx c;
+  memset(&c, 0, sizeof(c));
+  c.o = cfg->nocc;
+  c.v = cfg->nvir;
+  c.packed = cfg->ovvv_packed ? 1 : 0;
+  c.has_fvo = cfg->has_fvo ? 1 : 0;
+  c.vjp = 0;
+  c.prec = cfg->precision;
+  c.s = (hipStream_t)stream;
+  c.mo = mo_energy;
+  c.t1 = t1;
+  c.t2 = t2;
+  c.ovvv = ovvv;
+  c.ovoo = ovoo;
+  c.ovov = ovov;
+  c.fvo = fvo;
+  bind_workspace(c, L, workspace);
+  char *b = (char *)workspace;
+  double *pair_e = (double *)(b + off_p);
+  st = run_projected(c, ulo, nfrag, (double *)(b + off_a), (double *)(b + off_b), pair_e,
+                     (double *)(b + off_d));
+  if (st != PAMD_OK) return st;
+  PAMD_CHECK(launch_projected_total(pair_e, c.o, e_out, c.s));
+  return PAMD_OK;
 }
 
 int pamd_debug_gemm_descriptors(int nocc, int nvir, int i, int j, int k, int with_vjp,

@@ -546,6 +546,15 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
         rD = fma(rD, fma(-D, rD, 1.0), rD);
         rD = fma(rD, fma(-D, rD, 1.0), rD);
         eacc += w0 * Z * rD;
+        if (A.accA) {
+          const size_t v3 = (size_t)v * v * v;
+          const long long idx = ((long long)A0 * v + B0) * v + C0 + thr_w;
+          const double y = Z * rD;
+          for (int f = 0; f < A.nfrag; ++f) {
+            A.accA[f * v3 + idx] += A.u[f] * w0;
+            A.accB[f * v3 + idx] += A.u[f] * y;
+          }
+        }
         if (A.vjp) {
           const double r3W = 4.0 * w0 + w1 + w2 - 2.0 * (w3 + w4 + w5);
           const long long idx = ((long long)A0 * v + B0) * v + C0 + thr_w;
@@ -1054,6 +1063,42 @@ __global__ void k_total_energy(const double *tri_e, int o, double *e_out) {
 
 hipError_t launch_total_energy(const double *tri_e, int o, double *e_out, hipStream_t s) {
   hipLaunchKernelGGL(k_total_energy, dim3(1), dim3(256), 0, s, tri_e, o, e_out);
+  return hipGetLastError();
+}
+
+// Dot product of two length-n arrays: per-workgroup strided partial sums, then
+// one workgroup sums the G partials (fixed order for a fixed G).
+__global__ void k_dot_partial(const double *a, const double *b, size_t n, double *part) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (size_t)gridDim.x * 256)
+    s += a[t] * b[t];
+  s = block_sum256(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+hipError_t launch_dot(const double *a, const double *b, size_t n, double *part, int G, double *dst,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_dot_partial, dim3(G), dim3(256), 0, s, a, b, n, part);
+  hipLaunchKernelGGL(k_triple_energy, dim3(1), dim3(256), 0, s, part, G, dst);
+  return hipGetLastError();
+}
+
+// E = 1/3 sum over ordered occupied pairs (j, k) of pair_e[j, k], from the
+// unordered entries j <= k (the (k, j) term equals the (j, k) one).
+__global__ void k_projected_total(const double *pair_e, int o, double *e_out) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int t = threadIdx.x; t < o * o; t += 256) {
+    const int j = t / o, k = t % o;
+    if (j <= k) s += (j == k ? 1.0 : 2.0) * pair_e[t];
+  }
+  s = block_sum256(s, red);
+  if (threadIdx.x == 0) *e_out = s / 3.0;
+}
+
+hipError_t launch_projected_total(const double *pair_e, int o, double *e_out, hipStream_t s) {
+  hipLaunchKernelGGL(k_projected_total, dim3(1), dim3(256), 0, s, pair_e, o, e_out);
   return hipGetLastError();
 }
 

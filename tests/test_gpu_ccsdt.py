@@ -116,3 +116,23 @@ def test_weighted_energy_vjp(precision, cuda):
             assert _rel(g1[k].cpu().numpy(), g0[k].cpu().numpy()) <= 1e-13
         else:
             assert torch.equal(g1[k], g0[k]), k
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'bf16x3'])
+def test_projected_energy(precision, cuda):
+    """Local-orbital projector mat = ulo^T ulo against the oracle; the identity
+    projector gives E(T) and one orbital the golden occupied-weighted (T)."""
+    z = np.load(os.path.join(GOLDEN, 'ccsd_t_o6v20_fvo.npz'))
+    p = make_problem(6, 20, 80, seed=int(z['seed']), fvo_scale=float(z['fvo_scale']))
+    print(f'projected energy o6v20 {precision}', flush=True)
+    args = (p.mo_energy, p.t1, p.t2, p.ovvv, p.ovoo, p.ovov, p.fvo)
+    inp = _ops.Inputs(p.mo_energy, p.t1, p.t2, p.ovvv_packed(), p.ovoo, p.ovov, p.fvo,
+                      precision=precision, device=cuda)
+    etol = E_TOL if precision == 'fp64' else 1e-9
+    ulo = np.random.default_rng(3).normal(size=(2, 6)) * 0.5
+    e = float(_ops.projected_energy(inp, ulo).item())
+    assert abs(e - O.projected_energy(*args, ulo)) <= etol
+    assert abs(float(_ops.projected_energy(inp, np.eye(6)[:4]).item())
+               - O.projected_energy(*args, np.eye(6)[:4])) <= etol
+    e2 = float(_ops.projected_energy(inp, np.eye(6)[2]).item())
+    assert abs(e2 - float(z['e_occ'][2])) <= etol

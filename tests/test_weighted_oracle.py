@@ -64,3 +64,22 @@ def test_weighted_adjoint_central_differences():
         e[q] -= 2 * h
         emn = O.energy_vjp(e, *_args(p)[1:], occ_weight=m)[0]
         assert abs((ep - emn) / (2 * h) - g['mo_energy_bar'][q]) <= 1e-6 * abs(g['mo_energy_bar'][q])
+
+
+def test_projected_energy_matches_spin_orbital_projector_and_limits():
+    p = make_problem(4, 8, 28, seed=9, fvo_scale=1e-3)
+    L = make_L(4, 8, 28, seed=9)
+    rng = np.random.default_rng(4)
+    ulo = rng.normal(size=(2, 4))
+    e = O.projected_energy(*_args(p), ulo)
+    e_so = G.energy(L, p.mo_energy, p.fock, p.t1, p.t2, 4, prj=ulo)
+    assert abs(e - e_so) <= 1e-12 * abs(e_so)
+    # identity projector -> E(T); one orbital -> pair-energy row sum
+    e_t = O.energy(*_args(p))
+    assert np.isclose(O.projected_energy(*_args(p), np.eye(4)), e_t, rtol=1e-13)
+    pairs = O.pair_energy(*_args(p))
+    assert np.isclose(O.projected_energy(*_args(p), np.eye(4)[1:2]), pairs[1].sum(), rtol=1e-12)
+    # a non-diagonal projector is not the diagonal-weighted energy of its diagonal
+    m = (ulo.T @ ulo).diagonal()
+    e_diag, _ = O.energy_vjp(*_args(p), occ_weight=m)
+    assert abs(e - e_diag) > 1e-6 * abs(e)
