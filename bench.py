@@ -70,8 +70,11 @@ def engine_roofline(prof, precision):
     else:
         kernel, mfma_factor, peak = 'k_gemm_f64', 1.0, PEAKS['fp64']
     achieved = fp64_equiv * mfma_factor if fp64_equiv else None
+    traffic, traffic_src = engine_traffic('k_gemm_b3') if precision == 'bf16x3' else (None, None)
     roofline = {'bound': 'mfma', 'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s',
-                'frac': (achieved / peak) if achieved else None, 'traffic': None,
+                'frac': (achieved / peak) if achieved else None, 'traffic': traffic,
+                'traffic_unit': 'bytes per launch' if traffic else None,
+                'traffic_source': traffic_src,
                 'kernel': kernel, 'mfma_flops_per_alg_flop': mfma_factor,
                 'fp64_equiv_tflops': fp64_equiv,
                 'frac_fp64_peak': (fp64_equiv / PEAKS['fp64']) if fp64_equiv else None,
@@ -79,6 +82,26 @@ def engine_roofline(prof, precision):
                 'avg_launch_ms': g_ms / g_n if g_n else None,
                 'flops_per_launch': g_fl * mfma_factor / g_n if g_n else None}
     return roofline
+
+
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                            'traffic_e_t_vjp_o10v400_bf16x3.json')
+
+
+def engine_traffic(prefix):
+    """HBM bytes per launch of the contraction engine (dispatch-weighted mean
+    over its tile shapes) from the committed rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE summary, or None when that file is absent."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None, None
+    with open(TRAFFIC_FILE) as f:
+        t = json.load(f)
+    ks = [x for k, x in t['kernels'].items() if k.startswith(prefix)]
+    n = sum(x['dispatches'] for x in ks)
+    if not n:
+        return None, None
+    b = sum(x['dispatches'] * (x['hbm_read_bytes'] + x['hbm_write_bytes']) for x in ks) / n
+    return b, f"rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, {t['workload']} (profiles/)"
 
 
 def df_measure(o, v, naux, seed, dev, inp, grads):
