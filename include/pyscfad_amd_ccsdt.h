@@ -109,21 +109,35 @@ int pamd_ccsdt_weighted_energy_vjp(const pamd_ccsdt_config *cfg,
                                    double *ovov_bar, double *fvo_bar, double *occ_weight_bar,
                                    void *stream);
 
-/* Occupied-weighted E(T) of a local-orbital projector, forward only:
+/* Occupied-weighted E(T) of a local-orbital projector and its adjoint:
  * pyscfad/lno/ccsd_t.py:35-60 (mat = ulo^T ulo) with lnoccsdt_contract
- * (lno_ccsd_t.c:6-29, 96-138) and et *= 2/3 (lno/ccsd_t.py:104):
+ * (lno_ccsd_t.c:6-29, 96-138) and et *= 2/3 (lno/ccsd_t.py:104), and the VJP
+ * of _ccsd_t_energy (lno/ccsd_t.py:124-237, lno_ccsd_t_vjp.c:400-566) chained
+ * through mat = ulo^T ulo:
  *   E = 1/3 sum_{l,i,j,k,abc} mat[l,i] W_ljk[abc] Z_ijk[abc] / D_ijk[abc].
  * ulo: HOST array (nfrag, nocc), 1 <= nfrag <= 8.  e_out: device scalar.
  * The walk takes every unordered occupied pair (j, k) and all o triples
- * (l, j, k), so it runs about three times the W builds of pamd_ccsdt_energy.
- * Workspace: pamd_ccsdt_projected_workspace_size. */
-int pamd_ccsdt_projected_workspace_size(const pamd_ccsdt_config *cfg, int nfrag, size_t *bytes);
+ * (l, j, k): about three times the W builds of pamd_ccsdt_energy, and the
+ * adjoint builds each W a second time.  Cotangents as pamd_ccsdt_energy_vjp,
+ * plus ulo_bar (device, nfrag x nocc, overwritten) = ulo (mat_bar + mat_bar^T).
+ * Workspace: pamd_ccsdt_projected_workspace_size(cfg, nfrag, with_vjp, ...). */
+int pamd_ccsdt_projected_workspace_size(const pamd_ccsdt_config *cfg, int nfrag, int with_vjp,
+                                        size_t *bytes);
 int pamd_ccsdt_projected_energy(const pamd_ccsdt_config *cfg,
                                 const double *mo_energy, const double *t1, const double *t2,
                                 const double *ovvv, const double *ovoo, const double *ovov,
                                 const double *fvo, const double *ulo, int nfrag,
                                 void *workspace, size_t workspace_bytes, double *e_out,
                                 void *stream);
+int pamd_ccsdt_projected_energy_vjp(const pamd_ccsdt_config *cfg,
+                                    const double *mo_energy, const double *t1, const double *t2,
+                                    const double *ovvv, const double *ovoo, const double *ovov,
+                                    const double *fvo, const double *ulo, int nfrag,
+                                    double et_bar, void *workspace, size_t workspace_bytes,
+                                    double *e_out, double *mo_energy_bar, double *t1_bar,
+                                    double *t2_bar, double *ovvv_bar, double *ovoo_bar,
+                                    double *ovov_bar, double *fvo_bar, double *ulo_bar,
+                                    void *stream);
 
 /* Library version string (static storage). */
 const char *pamd_version(void);

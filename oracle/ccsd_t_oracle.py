@@ -158,7 +158,15 @@ def energy_vjp(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, et_bar=1.0, occ_weight=
     Wbar = g * (Z / D + r3Y)
     Vbar = g * r3Y
     Dbar = -g * W * Z / D ** 2
-    # D = e_i + e_j + e_k - e_a - e_b - e_c   (ccsd_t_vjp.c:98-117)
+    mo_bar, g = _backprop(nocc, t1, t2, ovvv, ovoo, ovov, fvo, Wbar, Vbar, Dbar)
+    return e, dict(g, mo_energy_bar=mo_bar)
+
+
+def _backprop(nocc, t1, t2, ovvv, ovoo, ovov, fvo, Wbar, Vbar, Dbar):
+    """Operand cotangents from the seeds dE/dW, dE/dV, dE/dD of the full arrays
+    (ccsd_t_vjp.c:98-269): D = e_i + e_j + e_k - e_a - e_b - e_c, W = P6(w),
+    V = P6(v) with w, v the raw terms of ``_w_raw`` / ``_v_raw``."""
+    nvir = t1.shape[1]
     mo_bar = np.zeros(nocc + nvir)
     mo_bar[:nocc] = (np.einsum('ijkabc->i', Dbar) + np.einsum('ijkabc->j', Dbar)
                      + np.einsum('ijkabc->k', Dbar))
@@ -174,8 +182,31 @@ def energy_vjp(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, et_bar=1.0, occ_weight=
     t1_bar = 0.5 * np.einsum('ijkabc,iajb->kc', vbar, ovov, optimize=True)
     t2_bar += 0.5 * np.einsum('ijkabc,ck->jiba', vbar, fvo, optimize=True)
     fvo_bar = 0.5 * np.einsum('ijkabc,jiba->ck', vbar, t2, optimize=True)
-    return e, dict(t1_bar=t1_bar, t2_bar=t2_bar, fvo_bar=fvo_bar, mo_energy_bar=mo_bar,
-                   ovoo_bar=ovoo_bar, ovov_bar=ovov_bar, ovvv_bar=ovvv_bar)
+    return mo_bar, dict(t1_bar=t1_bar, t2_bar=t2_bar, fvo_bar=fvo_bar, ovoo_bar=ovoo_bar,
+                        ovov_bar=ovov_bar, ovvv_bar=ovvv_bar)
+
+
+def projected_energy_vjp(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, ulo, et_bar=1.0):
+    """(E, cotangents) of ``projected_energy``, plus ``ulo_bar``.
+
+    With Y = Z / D: E = 1/3 sum mat[l,i] W[l,...] Y[i,...], so the seeds are
+    dE/dW = P + r3(Q / D), dE/dV = r3(Q / D), dE/dD = -Q Z / D^2, with
+    P = 1/3 sum_i mat[l,i] Y[i,...] and Q = 1/3 sum_l mat[l,i] W[l,...]
+    (r3 is its own adjoint).  mat_bar[l,i] = 1/3 <W[l,...], Y[i,...]> and
+    ulo_bar = ulo (mat_bar + mat_bar^T) for mat = ulo^T ulo.
+    """
+    nocc = t1.shape[0]
+    ulo = np.asarray(ulo, dtype=np.float64)
+    mat = ulo.T @ ulo
+    W, V, Z, D = _forward(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo)
+    Y = Z / D
+    e = float(np.einsum('li,ljkabc,ijkabc->', mat, W, Y, optimize=True) / 3.0)
+    P = et_bar * np.einsum('li,ijkabc->ljkabc', mat, Y, optimize=True) / 3.0
+    Q = et_bar * np.einsum('li,ljkabc->ijkabc', mat, W, optimize=True) / 3.0
+    Xbar = r3(Q / D)
+    mo_bar, g = _backprop(nocc, t1, t2, ovvv, ovoo, ovov, fvo, P + Xbar, Xbar, -Q * Z / D ** 2)
+    mat_bar = et_bar * np.einsum('ljkabc,ijkabc->li', W, Y, optimize=True) / 3.0
+    return e, dict(g, mo_energy_bar=mo_bar, ulo_bar=ulo @ (mat_bar + mat_bar.T))
 
 
 def fold_ovvv_packed(ovvv_bar):

@@ -172,25 +172,63 @@ def weighted_energy_vjp(inp: Inputs, occ_weight, et_bar=1.0):
     return e, out
 
 
-def projected_energy(inp: Inputs, ulo):
-    """Occupied-weighted E(T) of the local-orbital projector mat = ulo^T ulo
-    (ulo: (nfrag, nocc), nfrag <= 8), as a 0-d float64 device tensor."""
-    lib = _lib.load()
+def _ulo_array(inp, ulo):
     u = np.ascontiguousarray(np.asarray(
         ulo.detach().cpu() if isinstance(ulo, torch.Tensor) else ulo, dtype=np.float64))
     if u.ndim == 1:
         u = u[None, :]
     if u.ndim != 2 or u.shape[1] != inp.nocc:
         raise ValueError(f'ulo must be (nfrag, {inp.nocc}), got {u.shape}')
-    nfrag = int(u.shape[0])
+    return u
+
+
+def _projected_ws(inp, nfrag, with_vjp):
+    lib = _lib.load()
     n = ctypes.c_size_t(0)
     _lib.check(lib.pamd_ccsdt_projected_workspace_size(ctypes.byref(inp.cfg), nfrag,
-                                                       ctypes.byref(n)),
+                                                       int(with_vjp), ctypes.byref(n)),
                'pamd_ccsdt_projected_workspace_size')
-    ws = torch.empty(max(n.value, 1), dtype=torch.uint8, device=inp.device)
+    return torch.empty(max(n.value, 1), dtype=torch.uint8, device=inp.device), n.value
+
+
+def projected_energy(inp: Inputs, ulo):
+    """Occupied-weighted E(T) of the local-orbital projector mat = ulo^T ulo
+    (ulo: (nfrag, nocc), nfrag <= 8), as a 0-d float64 device tensor."""
+    lib = _lib.load()
+    u = _ulo_array(inp, ulo)
+    ws, nb = _projected_ws(inp, u.shape[0], False)
     e = torch.zeros((), dtype=torch.float64, device=inp.device)
     st = lib.pamd_ccsdt_projected_energy(ctypes.byref(inp.cfg), *inp.args(),
-                                         u.ctypes.data_as(ctypes.c_void_p), nfrag, _ptr(ws),
-                                         n.value, _ptr(e), _stream(inp.device))
+                                         u.ctypes.data_as(ctypes.c_void_p), int(u.shape[0]),
+                                         _ptr(ws), nb, _ptr(e), _stream(inp.device))
     _lib.check(st, 'pamd_ccsdt_projected_energy')
     return e
+
+
+def projected_energy_vjp(inp: Inputs, ulo, et_bar=1.0):
+    """(E, cotangents) of ``projected_energy``; the cotangents include
+    ``ulo_bar`` (nfrag, nocc)."""
+    lib = _lib.load()
+    o, v = inp.nocc, inp.nvir
+    u = _ulo_array(inp, ulo)
+    dev = inp.device
+    ws, nb = _projected_ws(inp, u.shape[0], True)
+    f64 = dict(dtype=torch.float64, device=dev)
+    e = torch.zeros((), **f64)
+    out = dict(
+        mo_energy_bar=torch.empty(o + v, **f64),
+        t1_bar=torch.empty((o, v), **f64),
+        t2_bar=torch.empty((o, o, v, v), **f64),
+        ovvv_bar=torch.empty(tuple(inp.ovvv.shape), **f64),
+        ovoo_bar=torch.empty((o, v, o, o), **f64),
+        ovov_bar=torch.empty((o, v, o, v), **f64),
+        fvo_bar=torch.empty((v, o), **f64),
+        ulo_bar=torch.empty(tuple(u.shape), **f64),
+    )
+    st = lib.pamd_ccsdt_projected_energy_vjp(
+        ctypes.byref(inp.cfg), *inp.args(), u.ctypes.data_as(ctypes.c_void_p), int(u.shape[0]),
+        ctypes.c_double(float(et_bar)), _ptr(ws), nb, _ptr(e), _ptr(out['mo_energy_bar']),
+        _ptr(out['t1_bar']), _ptr(out['t2_bar']), _ptr(out['ovvv_bar']), _ptr(out['ovoo_bar']),
+        _ptr(out['ovov_bar']), _ptr(out['fvo_bar']), _ptr(out['ulo_bar']), _stream(dev))
+    _lib.check(st, 'pamd_ccsdt_projected_energy_vjp')
+    return e, out

@@ -77,7 +77,10 @@ struct OrbitArgs {
   double *accA, *accB;
   int nfrag;
   double u[kMaxFrag];
+  double *part_u;       // [G][kMaxFrag] per-workgroup dE/du_f sums (projected adjoint)
 };
+
+enum OrbitMode { kOrbitPlain = 0, kOrbitProjAcc = 1, kOrbitProjVjp = 2 };
 
 // V-bar reductions: the sums over one virtual index are split into kVRS parts
 // (more workgroups in flight); the scratch R holds every part.
@@ -116,7 +119,7 @@ hipError_t launch_gemm_bf16x3(const GemmDesc &d, hipStream_t s);
 hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s);
 int gemm_tiles(int M, int N);     // FP64 engine: workgroup tiles of one split
 int gemm_tiles_b3(int M, int N);  // split-BF16x3 engine: workgroup tiles of one split
-hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s);
+hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s, int mode = kOrbitPlain);
 hipError_t launch_vr(const VRArgs &a, hipStream_t s);
 hipError_t launch_combine(const CombineArgs &a, hipStream_t s);
 hipError_t launch_fold_ovvv(const double *Vpb, int o, int v, int packed, double *out, hipStream_t s);
@@ -129,6 +132,8 @@ hipError_t launch_pair_energy(const double *tri_e, int o, double *e_pair, hipStr
 hipError_t launch_total_energy(const double *tri_e, int o, double *e_out, hipStream_t s);
 hipError_t launch_dot(const double *a, const double *b, size_t n, double *part, int G, double *dst,
                       hipStream_t s);
+hipError_t launch_reduce_ulo(const double *part_u, int G, int nfrag, double scale, double *dst,
+                             int stride, hipStream_t s);
 hipError_t launch_projected_total(const double *pair_e, int o, double *e_out, hipStream_t s);
 hipError_t launch_weighted_energy(const double *tri_e, int o, const double *m, double ebar,
                                   double *e_out, double *m_bar, hipStream_t s);

@@ -442,6 +442,21 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
   return PAMD_OK;
 }
 
+// Zero the adjoint accumulators and the cotangent outputs of one call.
+static int zero_vjp(const Ctx &c) {
+  const int o = c.o, v = c.v;
+  const size_t v2 = (size_t)v * v;
+  PAMD_CHECK(launch_fill(c.Vpb, (size_t)o * v2 * v, 0.0, c.s));
+  PAMD_CHECK(launch_fill(c.T2mb, (size_t)o * o * v2, 0.0, c.s));
+  PAMD_CHECK(launch_fill(c.part_mo, (size_t)kGmax * (o + v), 0.0, c.s));
+  PAMD_CHECK(launch_fill(c.t1_bar, (size_t)o * v, 0.0, c.s));
+  PAMD_CHECK(launch_fill(c.t2_bar, (size_t)o * o * v2, 0.0, c.s));
+  PAMD_CHECK(launch_fill(c.ovoo_bar, (size_t)o * v * o * o, 0.0, c.s));
+  PAMD_CHECK(launch_fill(c.ovov_bar, (size_t)o * v * o * v, 0.0, c.s));
+  PAMD_CHECK(launch_fill(c.fvo_bar, (size_t)v * o, 0.0, c.s));
+  return PAMD_OK;
+}
+
 // Layout changes at entry, shared by every walk; returns the orbit count.
 static int setup(Ctx &c, long long &norb) {
   const int o = c.o, v = c.v;
@@ -487,16 +502,38 @@ static OrbitArgs orbit_args(const Ctx &c, int i, int j, int k, long long norb) {
 // (l, j, k) in order of l.  Each W_ljk is built, and the orbit epilogue adds
 // ulo[f, l] W and ulo[f, l] Z / D into accA[f], accB[f]; then
 // pair_e[j, k] = sum_f <accA[f], accB[f]> = sum_{l,i} mat[l,i] <W_ljk, Z_ijk / D_ijk>.
+// With c.vjp, a second pass over l rebuilds W_ljk and runs the projected
+// adjoint epilogue (seeds from the finished accA, accB, cotangent
+// g = ebar w_jk / 3 with w_jk = 2 off the diagonal) and the plain adjoint
+// contractions of that triple; dE/dulo[f, l] gathers g (<W, accB[f]> +
+// <accA[f], Z/D>) into ulo_bar.
 static int run_projected(Ctx &c, const double *ulo, int nfrag, double *accA, double *accB,
-                         double *pair_e, double *dot_part) {
+                         double *pair_e, double *dot_part, double *part_u, double *ulo_bar) {
   long long norb = 0;
   int st = setup(c, norb);
   if (st != PAMD_OK) return st;
   const int o = c.o;
   const size_t v3 = (size_t)c.v * c.v * c.v;
+  if (c.vjp) {
+    st = zero_vjp(c);
+    if (st != PAMD_OK) return st;
+    PAMD_CHECK(launch_fill(ulo_bar, (size_t)nfrag * o, 0.0, c.s));
+  }
   long long tcount = 0;
-  const long long nbuild = (long long)o * o * (o + 1) / 2;
+  const long long nbuild = (long long)o * o * (o + 1) / 2 * (c.vjp ? 2 : 1);
   const auto t_start = std::chrono::steady_clock::now();
+  auto progress = [&]() -> int {
+    if (g_progress_every > 0 && tcount % g_progress_every == 0) {
+      PAMD_CHECK(hipStreamSynchronize(c.s));
+      const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() -
+                                                       t_start).count();
+      fprintf(stderr, "pyscfad_amd: projected (T)%s triples %lld/%lld done, %.1f s\n",
+              c.vjp ? "+VJP" : "", tcount, nbuild, el);
+      fflush(stderr);
+    }
+    return PAMD_OK;
+  };
+  const int bf = c.prec == PAMD_PREC_BF16X3;
   for (int j = 0; j < o; ++j)
     for (int k = j; k < o; ++k) {
       PAMD_CHECK(launch_fill(accA, (size_t)nfrag * v3, 0.0, c.s));
@@ -512,18 +549,38 @@ static int run_projected(Ctx &c, const double *ulo, int nfrag, double *accA, dou
         oa.nfrag = nfrag;
         for (int f = 0; f < nfrag; ++f) oa.u[f] = ulo[(size_t)f * o + l];
         const double ob = 8.0 * (double)v3 * (1.0 + 4.0 * nfrag);  // W read, 2 nfrag RMW
-        PAMD_CHECK(timed(c.s, kProfOrbit, 0.0, ob, [&] { return launch_orbit(oa, c.G, c.s); }));
-        if (g_progress_every > 0 && tcount % g_progress_every == 0) {
-          PAMD_CHECK(hipStreamSynchronize(c.s));
-          const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() -
-                                                           t_start).count();
-          fprintf(stderr, "pyscfad_amd: projected (T) triples %lld/%lld done, %.1f s\n", tcount,
-                  nbuild, el);
-          fflush(stderr);
-        }
+        PAMD_CHECK(timed(c.s, kProfOrbit, 0.0, ob,
+                         [&] { return launch_orbit(oa, c.G, c.s, kOrbitProjAcc); }));
+        st = progress();
+        if (st != PAMD_OK) return st;
       }
       PAMD_CHECK(launch_dot(accA, accB, (size_t)nfrag * v3, dot_part, kGmax, pair_e + j * o + k,
                             c.s));
+      if (!c.vjp) continue;
+      const double g = c.ebar * (j == k ? 1.0 : 2.0) / 3.0;
+      for (int l = 0; l < o; ++l) {
+        g_prof_on = g_prof_every > 0 && (tcount % g_prof_every) == 0;
+        ++tcount;
+        st = build_w(c, l, j, k);
+        if (st != PAMD_OK) return st;
+        OrbitArgs oa = orbit_args(c, l, j, k, norb);
+        oa.Wb8 = bf ? (unsigned long long *)c.gW : nullptr;
+        oa.vjp = 1;
+        oa.ebar = g;
+        oa.accA = accA;
+        oa.accB = accB;
+        oa.nfrag = nfrag;
+        oa.part_u = part_u;
+        for (int f = 0; f < nfrag; ++f) oa.u[f] = ulo[(size_t)f * o + l];
+        const double ob = 8.0 * (double)v3 * (3.0 + 2.0 * nfrag);  // W, accA, accB read; Wbar, Vbar
+        PAMD_CHECK(timed(c.s, kProfOrbit, 0.0, ob,
+                         [&] { return launch_orbit(oa, c.G, c.s, kOrbitProjVjp); }));
+        PAMD_CHECK(launch_reduce_ulo(part_u, c.G, nfrag, g, ulo_bar + l, o, c.s));
+        st = adjoint(c, l, j, k);
+        if (st != PAMD_OK) return st;
+        st = progress();
+        if (st != PAMD_OK) return st;
+      }
     }
   g_prof_on = false;
   if (!g_prof_recs.empty()) PAMD_CHECK(prof_collect());
@@ -539,15 +596,8 @@ static int run(Ctx &c) {
     if (st0 != PAMD_OK) return st0;
   }
   if (c.vjp) {
-    const size_t v2 = (size_t)v * v;
-    PAMD_CHECK(launch_fill(c.Vpb, (size_t)o * v2 * v, 0.0, c.s));
-    PAMD_CHECK(launch_fill(c.T2mb, (size_t)o * o * v2, 0.0, c.s));
-    PAMD_CHECK(launch_fill(c.part_mo, (size_t)kGmax * (o + v), 0.0, c.s));
-    PAMD_CHECK(launch_fill(c.t1_bar, (size_t)o * v, 0.0, c.s));
-    PAMD_CHECK(launch_fill(c.t2_bar, (size_t)o * o * v2, 0.0, c.s));
-    PAMD_CHECK(launch_fill(c.ovoo_bar, (size_t)o * v * o * o, 0.0, c.s));
-    PAMD_CHECK(launch_fill(c.ovov_bar, (size_t)o * v * o * v, 0.0, c.s));
-    PAMD_CHECK(launch_fill(c.fvo_bar, (size_t)v * o, 0.0, c.s));
+    const int st0 = zero_vjp(c);
+    if (st0 != PAMD_OK) return st0;
   }
   long long tcount = 0;
   const long long ntrip = (long long)o * (o + 1) * (o + 2) / 6 - o;
@@ -780,47 +830,58 @@ int pamd_ccsdt_weighted_energy_vjp(const pamd_ccsdt_config *cfg, const double *m
                   ovov_bar, fvo_bar, occ_weight_bar, stream);
 }
 
-// workspace of the projected walk: the forward layout, then accA, accB
-// (nfrag v^3 each), the pair energies (o^2) and the dot partials (kGmax)
-static size_t projected_extra(int o, int v, int nfrag, size_t *off_a, size_t *off_b,
-                              size_t *off_p, size_t *off_d, size_t base) {
+// workspace of the projected walk: the plain layout (forward or adjoint), then
+// accA, accB (nfrag v^3 each), the pair energies (o^2), the dot partials
+// (kGmax) and the per-workgroup dE/dulo sums (kGmax x kMaxFrag)
+struct ProjLayout {
+  Layout L;
+  size_t off_a, off_b, off_p, off_d, off_u, total;
+};
+
+static ProjLayout projected_layout(int o, int v, int nfrag, int vjp, int prec) {
+  ProjLayout P;
+  P.L = make_layout(o, v, vjp, prec);
   const size_t blk = align256(sizeof(double) * (size_t)nfrag * v * v * v);
-  *off_a = base;
-  *off_b = *off_a + blk;
-  *off_p = *off_b + blk;
-  *off_d = *off_p + align256(sizeof(double) * (size_t)o * o);
-  return *off_d + align256(sizeof(double) * kGmax);
+  P.off_a = P.L.total;
+  P.off_b = P.off_a + blk;
+  P.off_p = P.off_b + blk;
+  P.off_d = P.off_p + align256(sizeof(double) * (size_t)o * o);
+  P.off_u = P.off_d + align256(sizeof(double) * kGmax);
+  P.total = P.off_u + align256(sizeof(double) * kGmax * kMaxFrag);
+  return P;
 }
 
-int pamd_ccsdt_projected_workspace_size(const pamd_ccsdt_config *cfg, int nfrag, size_t *bytes) {
+int pamd_ccsdt_projected_workspace_size(const pamd_ccsdt_config *cfg, int nfrag, int with_vjp,
+                                        size_t *bytes) {
   if (!cfg || !bytes || cfg->nocc < 1 || cfg->nvir < 1 || nfrag < 1 || nfrag > kMaxFrag)
     return PAMD_ERR_ARG;
-  size_t a, b, p, d;
-  *bytes = projected_extra(cfg->nocc, cfg->nvir, nfrag, &a, &b, &p, &d,
-                           make_layout(cfg->nocc, cfg->nvir, 0, cfg->precision).total);
+  *bytes = projected_layout(cfg->nocc, cfg->nvir, nfrag, with_vjp ? 1 : 0, cfg->precision).total;
   return PAMD_OK;
 }
 
-int pamd_ccsdt_projected_energy(const pamd_ccsdt_config *cfg, const double *mo_energy,
-                                const double *t1, const double *t2, const double *ovvv,
-                                const double *ovoo, const double *ovov, const double *fvo,
-                                const double *ulo, int nfrag, void *workspace,
-                                size_t workspace_bytes, double *e_out, void *stream) {
+static int projected_impl(const pamd_ccsdt_config *cfg, const double *mo_energy, const double *t1,
+                          const double *t2, const double *ovvv, const double *ovoo,
+                          const double *ovov, const double *fvo, const double *ulo, int nfrag,
+                          int vjp, double et_bar, void *workspace, size_t workspace_bytes,
+                          double *e_out, double *mo_energy_bar, double *t1_bar, double *t2_bar,
+                          double *ovvv_bar, double *ovoo_bar, double *ovov_bar, double *fvo_bar,
+                          double *ulo_bar, void *stream) {
   int st = check_common(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo);
   if (st != PAMD_OK) return st;
   if (!ulo || !e_out || nfrag < 1 || nfrag > kMaxFrag) return PAMD_ERR_ARG;
-  Layout L = make_layout(cfg->nocc, cfg->nvir, 0, cfg->precision);
-  size_t off_a, off_b, off_p, off_d;
-  const size_t total = projected_extra(cfg->nocc, cfg->nvir, nfrag, &off_a, &off_b, &off_p,
-                                       &off_d, L.total);
-  if (!workspace || workspace_bytes < total) return PAMD_ERR_WORKSPACE;
+  if (vjp && (!mo_energy_bar || !t1_bar || !t2_bar || !ovvv_bar || !ovoo_bar || !ovov_bar ||
+              !fvo_bar || !ulo_bar))
+    return PAMD_ERR_ARG;
+  if (vjp && cfg->nvir > 512) return PAMD_ERR_UNSUPPORTED;
+  const ProjLayout P = projected_layout(cfg->nocc, cfg->nvir, nfrag, vjp, cfg->precision);
+  if (!workspace || workspace_bytes < P.total) return PAMD_ERR_WORKSPACE;
   Ctx c;
   memset(&c, 0, sizeof(c));
   c.o = cfg->nocc;
   c.v = cfg->nvir;
   c.packed = cfg->ovvv_packed ? 1 : 0;
   c.has_fvo = cfg->has_fvo ? 1 : 0;
-  c.vjp = 0;
+  c.vjp = vjp;
   c.prec = cfg->precision;
   c.s = (hipStream_t)stream;
   c.mo = mo_energy;
@@ -830,14 +891,49 @@ int pamd_ccsdt_projected_energy(const pamd_ccsdt_config *cfg, const double *mo_e
   c.ovoo = ovoo;
   c.ovov = ovov;
   c.fvo = fvo;
-  bind_workspace(c, L, workspace);
+  c.ebar = et_bar;
+  c.t1_bar = t1_bar;
+  c.t2_bar = t2_bar;
+  c.ovoo_bar = ovoo_bar;
+  c.ovov_bar = ovov_bar;
+  c.fvo_bar = fvo_bar;
+  bind_workspace(c, P.L, workspace);
   char *b = (char *)workspace;
-  double *pair_e = (double *)(b + off_p);
-  st = run_projected(c, ulo, nfrag, (double *)(b + off_a), (double *)(b + off_b), pair_e,
-                     (double *)(b + off_d));
+  double *pair_e = (double *)(b + P.off_p);
+  st = run_projected(c, ulo, nfrag, (double *)(b + P.off_a), (double *)(b + P.off_b), pair_e,
+                     (double *)(b + P.off_d), (double *)(b + P.off_u), ulo_bar);
   if (st != PAMD_OK) return st;
   PAMD_CHECK(launch_projected_total(pair_e, c.o, e_out, c.s));
+  if (vjp) {
+    const int o = c.o, v = c.v;
+    PAMD_CHECK(launch_reduce_rows(c.part_mo, c.G, o + v, mo_energy_bar, 0, c.s));
+    PAMD_CHECK(launch_fold_ovvv(c.Vpb, o, v, c.packed, ovvv_bar, c.s));
+    PAMD_CHECK(launch_fold_t2m(c.T2mb, o, v, t2_bar, c.s));
+  }
   return PAMD_OK;
+}
+
+int pamd_ccsdt_projected_energy(const pamd_ccsdt_config *cfg, const double *mo_energy,
+                                const double *t1, const double *t2, const double *ovvv,
+                                const double *ovoo, const double *ovov, const double *fvo,
+                                const double *ulo, int nfrag, void *workspace,
+                                size_t workspace_bytes, double *e_out, void *stream) {
+  return projected_impl(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, ulo, nfrag, 0, 0.0,
+                        workspace, workspace_bytes, e_out, nullptr, nullptr, nullptr, nullptr,
+                        nullptr, nullptr, nullptr, nullptr, stream);
+}
+
+int pamd_ccsdt_projected_energy_vjp(const pamd_ccsdt_config *cfg, const double *mo_energy,
+                                    const double *t1, const double *t2, const double *ovvv,
+                                    const double *ovoo, const double *ovov, const double *fvo,
+                                    const double *ulo, int nfrag, double et_bar, void *workspace,
+                                    size_t workspace_bytes, double *e_out, double *mo_energy_bar,
+                                    double *t1_bar, double *t2_bar, double *ovvv_bar,
+                                    double *ovoo_bar, double *ovov_bar, double *fvo_bar,
+                                    double *ulo_bar, void *stream) {
+  return projected_impl(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, ulo, nfrag, 1, et_bar,
+                        workspace, workspace_bytes, e_out, mo_energy_bar, t1_bar, t2_bar,
+                        ovvv_bar, ovoo_bar, ovov_bar, fvo_bar, ulo_bar, stream);
 }
 
 int pamd_debug_gemm_descriptors(int nocc, int nvir, int i, int j, int k, int with_vjp,

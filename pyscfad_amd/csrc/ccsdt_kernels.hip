@@ -461,11 +461,21 @@ hipError_t launch_orbit_table(long long norb, unsigned short *tab, hipStream_t s
 // denominators' adjoint.  Thread t handles position t of every tile, so wave w
 // holds the slab a = 8 P1 + w: the dE/d(e_virtual) sums over two of the three
 // tile axes reduce within the wave, with a few LDS atomics per wave.
+// Modes: kOrbitPlain -- energy, and for the adjoint the Wbar/Vbar/dE/de seeds;
+// kOrbitProjAcc -- accA[f] += u_f W, accB[f] += u_f Z/D (projected walk);
+// kOrbitProjVjp -- seeds of the projected energy from the finished accA, accB
+// of the pair: Q = g sum_f u_f accA[f] (dE/dY), P = g sum_f u_f accB[f]
+// (dE/dW direct); Wbar = P + r3(Q)/D, Vbar = r3(Q)/D, Dbar = -Q Z/D^2, and
+// per-workgroup sums of <W, accB[f]> + <accA[f], Z/D> for dE/du_f.
+template <int MODE>
 __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
   __shared__ double Ws[6][OT3];
   __shared__ double Xs[6][OT3];
+  __shared__ double Qs[MODE == kOrbitProjVjp ? 6 : 1][MODE == kOrbitProjVjp ? OT3 : 1];
   __shared__ double red[8];
+  __shared__ double ured[MODE == kOrbitProjVjp ? 8 * kMaxFrag : 1];
   extern __shared__ double vir[];  // v doubles: dE/de_virtual accumulated by this workgroup
+  constexpr bool kVjp = MODE != kOrbitProjAcc;  // seeds (plain: when A.vjp)
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int o = A.o, v = A.v, n = o + v;
@@ -477,6 +487,10 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
   __syncthreads();
 
   double eacc = 0.0, occ = 0.0;
+  double udot[MODE == kOrbitProjVjp ? kMaxFrag : 1];
+#pragma unroll
+  for (int f = 0; f < (MODE == kOrbitProjVjp ? kMaxFrag : 1); ++f) udot[f] = 0.0;
+  const size_t v3 = (size_t)v * v * v;
   // Element (a, b, c) = 8 (Pa, Pb, Pc) + (al, be, ga): every address splits into a
   // part that is uniform per tile and a part that is fixed per thread.
   const int al = tid >> 6, be = (tid >> 3) & 7, ga = tid & 7;
@@ -519,6 +533,13 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
         if (A.has_fvo)
           vv_ += t2s_ij[ab] * fvo[c * o + k] + t2s_ik[ac] * fvo[b * o + j] + t2s_jk[bc] * fvo[a * o + i];
         xv = wv + 0.5 * vv_;
+        if constexpr (MODE == kOrbitProjVjp) {
+          double qv = 0.0;
+          for (int f = 0; f < A.nfrag; ++f) qv += A.u[f] * A.accA[f * v3 + tile_w + thr_w];
+          Qs[p][l_id] = A.ebar * qv;
+        }
+      } else if constexpr (MODE == kOrbitProjVjp) {
+        Qs[p][l_id] = 0.0;
       }
       Ws[p][l_id] = wv;
       Xs[p][l_id] = xv;
@@ -546,30 +567,52 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
         rD = fma(rD, fma(-D, rD, 1.0), rD);
         rD = fma(rD, fma(-D, rD, 1.0), rD);
         eacc += w0 * Z * rD;
-        if (A.accA) {
-          const size_t v3 = (size_t)v * v * v;
-          const long long idx = ((long long)A0 * v + B0) * v + C0 + thr_w;
+        const long long idx = ((long long)A0 * v + B0) * v + C0 + thr_w;
+        if constexpr (MODE == kOrbitProjAcc) {
           const double y = Z * rD;
           for (int f = 0; f < A.nfrag; ++f) {
             A.accA[f * v3 + idx] += A.u[f] * w0;
             A.accB[f * v3 + idx] += A.u[f] * y;
           }
-        }
-        if (A.vjp) {
-          const double r3W = 4.0 * w0 + w1 + w2 - 2.0 * (w3 + w4 + w5);
-          const long long idx = ((long long)A0 * v + B0) * v + C0 + thr_w;
-          const double vbar = g * r3W * rD;
-          const double wbar = g * Z * rD + vbar;
+        } else if constexpr (MODE == kOrbitProjVjp) {
+          const double q0 = Qs[p][l_id];
+          const double r3Q = 4.0 * q0 + Qs[p0][l_bca] + Qs[p1][l_cab] -
+                             2.0 * (Qs[p2][l_cba] + Qs[p3][l_acb] + Qs[p4][l_bac]);
+          const double y = Z * rD;
+          double pw = 0.0;
+#pragma unroll
+          for (int f = 0; f < kMaxFrag; ++f) {
+            if (f < A.nfrag) {
+              const double bf = A.accB[f * v3 + idx];
+              pw += A.u[f] * bf;
+              udot[f] += w0 * bf + A.accA[f * v3 + idx] * y;
+            }
+          }
+          const double vbar = r3Q * rD;
+          const double wbar = A.ebar * pw + vbar;
           if (A.Wb8)
             A.Wb8[idx] = split3(wbar);
           else
             A.W[idx] = wbar;
           A.Vb[idx] = vbar;
-          dbar = -g * w0 * Z * rD * rD;
+          dbar = -q0 * Z * rD * rD;
           occ += dbar;
+        } else {
+          if (A.vjp) {
+            const double r3W = 4.0 * w0 + w1 + w2 - 2.0 * (w3 + w4 + w5);
+            const double vbar = g * r3W * rD;
+            const double wbar = g * Z * rD + vbar;
+            if (A.Wb8)
+              A.Wb8[idx] = split3(wbar);
+            else
+              A.W[idx] = wbar;
+            A.Vb[idx] = vbar;
+            dbar = -g * w0 * Z * rD * rD;
+            occ += dbar;
+          }
         }
       }
-      if (A.vjp) {
+      if (kVjp && (MODE == kOrbitProjVjp || A.vjp)) {
         // axis c: sum over be (lane bits 3-5); axis b: sum over ga (bits 0-2); axis a: all
         double sc = dbar;
         sc += __shfl_xor(sc, 8, 64);
@@ -603,7 +646,23 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
   double otot = 0.0;
   for (int w = 0; w < 8; ++w) otot += red[w];
   if (tid == 0) A.part_t[blockIdx.x] = etot;
-  if (A.vjp) {
+  if constexpr (MODE == kOrbitProjVjp) {
+    // per-workgroup sums of <W, accB[f]> + <accA[f], Z/D>, wave order fixed
+#pragma unroll
+    for (int f = 0; f < kMaxFrag; ++f) {
+      if (f < A.nfrag) {
+        const double s_ = wave_sum(udot[f]);
+        if (lane == 0) ured[f * 8 + wave] = s_;
+      }
+    }
+    __syncthreads();
+    if (tid < A.nfrag) {
+      double s_ = 0.0;
+      for (int w = 0; w < 8; ++w) s_ += ured[tid * 8 + w];
+      A.part_u[(size_t)blockIdx.x * kMaxFrag + tid] = s_;
+    }
+  }
+  if (MODE == kOrbitProjVjp || (MODE == kOrbitPlain && A.vjp)) {
     double *pm = A.part_mo + (size_t)blockIdx.x * n;
     for (int x = tid; x < v; x += 512) pm[o + x] += vir[x];
     if (tid == 0) {
@@ -614,8 +673,13 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
   }
 }
 
-hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_orbit, dim3(grid), dim3(512), sizeof(double) * a.v, s, a);
+hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s, int mode) {
+  if (mode == kOrbitProjAcc)
+    hipLaunchKernelGGL(k_orbit<kOrbitProjAcc>, dim3(grid), dim3(512), sizeof(double) * a.v, s, a);
+  else if (mode == kOrbitProjVjp)
+    hipLaunchKernelGGL(k_orbit<kOrbitProjVjp>, dim3(grid), dim3(512), sizeof(double) * a.v, s, a);
+  else
+    hipLaunchKernelGGL(k_orbit<kOrbitPlain>, dim3(grid), dim3(512), sizeof(double) * a.v, s, a);
   return hipGetLastError();
 }
 
@@ -1081,6 +1145,22 @@ hipError_t launch_dot(const double *a, const double *b, size_t n, double *part, 
                       hipStream_t s) {
   hipLaunchKernelGGL(k_dot_partial, dim3(G), dim3(256), 0, s, a, b, n, part);
   hipLaunchKernelGGL(k_triple_energy, dim3(1), dim3(256), 0, s, part, G, dst);
+  return hipGetLastError();
+}
+
+// dst[f * stride] += scale * sum_g part_u[g][f] for f < nfrag (fixed order)
+__global__ void k_reduce_ulo(const double *part_u, int G, int nfrag, double scale, double *dst,
+                             int stride) {
+  const int f = threadIdx.x;
+  if (f >= nfrag) return;
+  double s = 0.0;
+  for (int g = 0; g < G; ++g) s += part_u[(size_t)g * kMaxFrag + f];
+  dst[(size_t)f * stride] += scale * s;
+}
+
+hipError_t launch_reduce_ulo(const double *part_u, int G, int nfrag, double scale, double *dst,
+                             int stride, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_ulo, dim3(1), dim3(64), 0, s, part_u, G, nfrag, scale, dst, stride);
   return hipGetLastError();
 }
 

@@ -136,3 +136,26 @@ def test_projected_energy(precision, cuda):
                - O.projected_energy(*args, np.eye(6)[:4])) <= etol
     e2 = float(_ops.projected_energy(inp, np.eye(6)[2]).item())
     assert abs(e2 - float(z['e_occ'][2])) <= etol
+
+
+@pytest.mark.parametrize('name,precision', [('o4v12', 'fp64'), ('o6v20_fvo', 'fp64'),
+                                            ('o6v20_fvo', 'bf16x3')])
+def test_projected_energy_vjp(name, precision, cuda):
+    """Adjoint of the local-orbital projected (T): every cotangent block and
+    ulo_bar against the oracle's projected adjoint."""
+    z = np.load(os.path.join(GOLDEN, f'ccsd_t_{name}.npz'))
+    o, v = int(z['nocc']), int(z['nvir'])
+    p = make_problem(o, v, int(z['naux']), seed=int(z['seed']), fvo_scale=float(z['fvo_scale']))
+    print(f'projected vjp {name} {precision}', flush=True)
+    args = (p.mo_energy, p.t1, p.t2, p.ovvv, p.ovoo, p.ovov, p.fvo)
+    ulo = np.random.default_rng(8).normal(size=(2, o)) * 0.5
+    e_ref, g_ref = O.projected_energy_vjp(*args, ulo, et_bar=0.9)
+    fvo = p.fvo if np.any(p.fvo) else None
+    inp = _ops.Inputs(p.mo_energy, p.t1, p.t2, p.ovvv_packed(), p.ovoo, p.ovov, fvo,
+                      precision=precision, device=cuda)
+    e, g = _ops.projected_energy_vjp(inp, ulo, et_bar=0.9)
+    etol, gtol = (E_TOL, G_RTOL) if precision == 'fp64' else (1e-9, 1e-5)
+    assert abs(float(e.item()) - e_ref) <= etol
+    for k in BLOCKS + ('ulo_bar',):
+        assert _rel(g[k].cpu().numpy(), g_ref[k]) <= gtol, k
+    assert _rel(g['ovvv_bar'].cpu().numpy(), O.fold_ovvv_packed(g_ref['ovvv_bar'])) <= gtol

@@ -83,3 +83,46 @@ def test_projected_energy_matches_spin_orbital_projector_and_limits():
     m = (ulo.T @ ulo).diagonal()
     e_diag, _ = O.energy_vjp(*_args(p), occ_weight=m)
     assert abs(e - e_diag) > 1e-6 * abs(e)
+
+
+def test_projected_adjoint_central_differences_and_identity():
+    p = make_problem(3, 7, 20, seed=12, fvo_scale=1e-3)
+    rng = np.random.default_rng(5)
+    ulo = rng.normal(size=(2, 3))
+    e, g = O.projected_energy_vjp(*_args(p), ulo, et_bar=1.0)
+    assert np.isclose(e, O.projected_energy(*_args(p), ulo), rtol=1e-14)
+    x = dict(t1=p.t1, t2=p.t2, ovvv=p.ovvv, ovoo=p.ovoo, ovov=p.ovov, fvo=p.fvo)
+    grads = dict(t1=g['t1_bar'], t2=g['t2_bar'], ovvv=g['ovvv_bar'], ovoo=g['ovoo_bar'],
+                 ovov=g['ovov_bar'], fvo=g['fvo_bar'])
+    h = 1e-5
+    for _ in range(3):
+        d = {k: rng.normal(size=x[k].shape) for k in BLOCKS}
+        nrm = np.sqrt(sum((t * t).sum() for t in d.values()))
+        d = {k: t / nrm for k, t in d.items()}
+
+        def ep(s):
+            y = {k: x[k] + s * h * d[k] for k in BLOCKS}
+            return O.projected_energy(p.mo_energy, y['t1'], y['t2'], y['ovvv'], y['ovoo'],
+                                      y['ovov'], y['fvo'], ulo)
+        fd = (ep(1) - ep(-1)) / (2 * h)
+        ana = sum((grads[k] * d[k]).sum() for k in BLOCKS)
+        assert abs(fd - ana) <= 1e-6 * abs(fd), (fd, ana)
+    for q in (1, 5):
+        em = p.mo_energy.copy()
+        em[q] += h
+        a = O.projected_energy(em, *_args(p)[1:], ulo)
+        em[q] -= 2 * h
+        b = O.projected_energy(em, *_args(p)[1:], ulo)
+        assert abs((a - b) / (2 * h) - g['mo_energy_bar'][q]) <= 1e-6 * abs(g['mo_energy_bar'][q])
+    for f, l in ((0, 0), (1, 2)):
+        u = ulo.copy()
+        u[f, l] += h
+        a = O.projected_energy(*_args(p), u)
+        u[f, l] -= 2 * h
+        b = O.projected_energy(*_args(p), u)
+        assert abs((a - b) / (2 * h) - g['ulo_bar'][f, l]) <= 1e-6 * abs(g['ulo_bar'][f, l])
+    # ulo = I: the plain adjoint
+    _e0, g0 = O.energy_vjp(*_args(p), et_bar=1.0)
+    _e1, g1 = O.projected_energy_vjp(*_args(p), np.eye(3), et_bar=1.0)
+    for k in g0:
+        np.testing.assert_allclose(g1[k], g0[k], rtol=1e-12, atol=1e-18)
