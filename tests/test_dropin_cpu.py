@@ -13,3 +13,13 @@ def test_kernel_fails_loudly_without_gpu():
     mycc = SimpleCCSD(t1=p.t1, t2=p.t2, verbose=0)
     with pytest.raises(RuntimeError, match='no GPU'):
         ccsd_t.kernel(mycc, eris_from_problem(p))
+
+
+@pytest.mark.skipif(torch.cuda.is_available(), reason='checks the no-GPU error path')
+def test_lno_kernel_fails_loudly_without_gpu():
+    import numpy as np
+    from pyscfad_amd.lno import ccsd_t as lno_t
+    p = make_problem(3, 6, 24, seed=0)
+    mycc = SimpleCCSD(t1=p.t1, t2=p.t2, verbose=0)
+    with pytest.raises(RuntimeError, match='no GPU'):
+        lno_t.kernel(mycc, eris_from_problem(p), np.eye(3)[:1], verbose=0)

@@ -159,3 +159,32 @@ def test_projected_energy_vjp(name, precision, cuda):
     for k in BLOCKS + ('ulo_bar',):
         assert _rel(g[k].cpu().numpy(), g_ref[k]) <= gtol, k
     assert _rel(g['ovvv_bar'].cpu().numpy(), O.fold_ovvv_packed(g_ref['ovvv_bar'])) <= gtol
+
+
+def test_lno_kernel_drop_in_autograd(cuda):
+    """pyscfad_amd.lno.ccsd_t.kernel(mycc, eris, ulo): energy and torch autograd
+    gradients (ulo, t1, t2, packed ovvv, fock) against the oracle."""
+    import torch
+    from pyscfad_amd.cc.eris import ChemistsERIs, SimpleCCSD
+    from pyscfad_amd.lno import ccsd_t as lno_t
+    p = make_problem(4, 12, 48, seed=2, fvo_scale=1e-3)
+    print('lno drop-in o4v12', flush=True)
+    idx, idy = np.tril_indices(12)
+    leaves = dict(fock=p.fock, mo_energy=p.mo_energy, ovoo=p.ovoo, ovov=p.ovov,
+                  ovvv=p.ovvv[:, :, idx, idy])
+    tl = {k: torch.tensor(v, device=cuda, requires_grad=True) for k, v in leaves.items()}
+    t1 = torch.tensor(p.t1, device=cuda, requires_grad=True)
+    t2 = torch.tensor(p.t2, device=cuda, requires_grad=True)
+    ulo_np = np.random.default_rng(1).normal(size=(1, 4))
+    ulo = torch.tensor(ulo_np, device=cuda, requires_grad=True)
+    e = lno_t.kernel(SimpleCCSD(t1=t1, t2=t2, verbose=0), ChemistsERIs(**tl), ulo, verbose=0)
+    e.backward()
+    e_ref, g = O.projected_energy_vjp(p.mo_energy, p.t1, p.t2, p.ovvv, p.ovoo, p.ovov, p.fvo,
+                                      ulo_np)
+    assert abs(float(e.detach()) - e_ref) <= E_TOL
+    assert _rel(ulo.grad.cpu().numpy(), g['ulo_bar']) <= G_RTOL
+    assert _rel(t1.grad.cpu().numpy(), g['t1_bar']) <= G_RTOL
+    assert _rel(t2.grad.cpu().numpy(), g['t2_bar']) <= G_RTOL
+    assert _rel(tl['ovvv'].grad.cpu().numpy(), O.fold_ovvv_packed(g['ovvv_bar'])) <= G_RTOL
+    fb = tl['fock'].grad.cpu().numpy()
+    assert _rel(fb[4:, :4], g['fvo_bar']) <= G_RTOL and not np.any(fb[:4, :])
