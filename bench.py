@@ -70,7 +70,7 @@ def engine_roofline(prof, precision):
     else:
         kernel, mfma_factor, peak = 'k_gemm_f64', 1.0, PEAKS['fp64']
     achieved = fp64_equiv * mfma_factor if fp64_equiv else None
-    traffic, traffic_src = engine_traffic('k_gemm_b3') if precision == 'bf16x3' else (None, None)
+    traffic, traffic_src = engine_traffic(precision)
     roofline = {'bound': 'mfma', 'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s',
                 'frac': (achieved / peak) if achieved else None, 'traffic': traffic,
                 'traffic_unit': 'bytes per launch' if traffic else None,
@@ -84,17 +84,19 @@ def engine_roofline(prof, precision):
     return roofline
 
 
-TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
-                            'traffic_e_t_vjp_o10v400_bf16x3.json')
+TRAFFIC_FILES = {p: os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                                  f'traffic_e_t_vjp_o10v400_{p}.json') for p in ('bf16x3', 'fp64')}
 
 
-def engine_traffic(prefix):
+def engine_traffic(precision):
     """HBM bytes per launch of the contraction engine (dispatch-weighted mean
     over its tile shapes) from the committed rocprofv3 --pmc FETCH_SIZE /
     WRITE_SIZE summary, or None when that file is absent."""
-    if not os.path.exists(TRAFFIC_FILE):
+    path = TRAFFIC_FILES[precision]
+    prefix = 'k_gemm_b3' if precision == 'bf16x3' else 'k_gemm_f64'
+    if not os.path.exists(path):
         return None, None
-    with open(TRAFFIC_FILE) as f:
+    with open(path) as f:
         t = json.load(f)
     ks = [x for k, x in t['kernels'].items() if k.startswith(prefix)]
     n = sum(x['dispatches'] for x in ks)

@@ -515,7 +515,11 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
   const int l_acb = slot((al << 6) | (ga << 3) | be);
   const int l_bac = slot((be << 6) | (al << 3) | ga);
 
-  for (long long q = blockIdx.x; q < A.norb; q += gridDim.x) {
+  // consecutive orbits (neighbouring tiles along c, which share cache lines of
+  // W) go to workgroups on the same XCD, so the halves of a line that two
+  // workgroups need meet in one L2
+  const int first = xcd_remap(blockIdx.x, gridDim.x);
+  for (long long q = first; q < A.norb; q += gridDim.x) {
     const unsigned short *ent = A.tab + 4 * q;
     const int val[3] = {ent[0], ent[1], ent[2]};
     const OrbitPattern &pt = kOrbitPat[ent[3]];
