@@ -78,6 +78,21 @@ int pamd_ccsdt_energy_vjp(const pamd_ccsdt_config *cfg,
                           double *t2_bar, double *ovvv_bar, double *ovoo_bar,
                           double *ovov_bar, double *fvo_bar, void *stream);
 
+/* Number of occupied triples i <= j <= k (i == j == k excluded) of the walk. */
+int pamd_ccsdt_num_triples(int nocc, long long *n);
+
+/* One shard of E(T): the triples whose walk index t (i outer, then j, then k,
+ * i <= j <= k, i == j == k skipped) has t % nshards == shard.  The shards'
+ * results add up to pamd_ccsdt_energy's; this is the per-GPU share of a
+ * sharded run (round-robin keeps the shares balanced).  Same workspace as
+ * pamd_ccsdt_energy. */
+int pamd_ccsdt_energy_shard(const pamd_ccsdt_config *cfg,
+                            const double *mo_energy, const double *t1, const double *t2,
+                            const double *ovvv, const double *ovoo, const double *ovov,
+                            const double *fvo, int shard, int nshards,
+                            void *workspace, size_t workspace_bytes,
+                            double *e_out, void *stream);
+
 /* Occupied-pair resolved E(T): e_pair[i*o + j] = 1/3 sum_{k,abc} W Z / D,
  * sum(e_pair) = E(T).  Row sums are the occupied-weighted (T) of
  * lnoccsdt_contract with mat = e_i e_i^T (lno_ccsd_t.c:16-26); the labels of the

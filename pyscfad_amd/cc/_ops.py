@@ -100,6 +100,19 @@ def energy(inp: Inputs):
     return e
 
 
+def energy_shard(inp: Inputs, shard, nshards):
+    """E(T) of the round-robin shard ``shard`` of ``nshards`` over the occupied
+    triples (the shards add up to ``energy``)."""
+    lib = _lib.load()
+    ws, nbytes = inp.workspace(False)
+    e = torch.zeros((), dtype=torch.float64, device=inp.device)
+    st = lib.pamd_ccsdt_energy_shard(ctypes.byref(inp.cfg), *inp.args(), int(shard),
+                                     int(nshards), _ptr(ws), nbytes, _ptr(e),
+                                     _stream(inp.device))
+    _lib.check(st, 'pamd_ccsdt_energy_shard')
+    return e
+
+
 def pair_energy(inp: Inputs):
     """(o, o) occupied-pair resolved E(T); its sum is E(T)."""
     lib = _lib.load()

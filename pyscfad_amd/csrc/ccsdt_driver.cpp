@@ -229,6 +229,7 @@ struct Ctx {
   double *Vb, *Vpb, *T2mb, *part_mo, *R, *part_p, *part_h;
   int S_p, S_h, G;
   double ebar;
+  int shard, nshards;     // this call covers walk triples with index % nshards == shard
   const double *wt_host;  // occupied weights m (host) of a diagonal projector, or null
   double *wt;             // device copy of m
   // outputs
@@ -599,13 +600,16 @@ static int run(Ctx &c) {
     const int st0 = zero_vjp(c);
     if (st0 != PAMD_OK) return st0;
   }
-  long long tcount = 0;
+  long long tcount = 0, windex = -1;
   const long long ntrip = (long long)o * (o + 1) * (o + 2) / 6 - o;
   const auto t_start = std::chrono::steady_clock::now();
   for (int i = 0; i < o; ++i)
     for (int j = i; j < o; ++j)
       for (int k = j; k < o; ++k) {
         if (i == j && j == k) continue;
+        // round-robin shard of the walk: triple number windex goes to shard windex % nshards
+        ++windex;
+        if (c.nshards > 1 && windex % c.nshards != c.shard) continue;
         g_prof_on = g_prof_every > 0 && (tcount % g_prof_every) == 0;
         ++tcount;
         int st = build_w(c, i, j, k);
@@ -699,7 +703,7 @@ int pamd_ccsdt_workspace_size(const pamd_ccsdt_config *cfg, int with_vjp, size_t
 static int energy_impl(const pamd_ccsdt_config *cfg, const double *mo, const double *t1,
                        const double *t2, const double *ovvv, const double *ovoo,
                        const double *ovov, const double *fvo, void *ws, size_t ws_bytes,
-                       hipStream_t s, Ctx &c) {
+                       hipStream_t s, Ctx &c, int shard = 0, int nshards = 1) {
   int st = check_common(cfg, mo, t1, t2, ovvv, ovoo, ovov, fvo);
   if (st != PAMD_OK) return st;
   Layout L = make_layout(cfg->nocc, cfg->nvir, 0, cfg->precision);
@@ -719,6 +723,8 @@ static int energy_impl(const pamd_ccsdt_config *cfg, const double *mo, const dou
   c.ovoo = ovoo;
   c.ovov = ovov;
   c.fvo = fvo;
+  c.shard = shard;
+  c.nshards = nshards;
   bind_workspace(c, L, ws);
   return run(c);
 }
@@ -732,6 +738,27 @@ int pamd_ccsdt_energy(const pamd_ccsdt_config *cfg, const double *mo_energy, con
   hipStream_t s = (hipStream_t)stream;
   int st = energy_impl(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, workspace, workspace_bytes,
                        s, c);
+  if (st != PAMD_OK) return st;
+  PAMD_CHECK(launch_total_energy(c.tri, c.o, e_out, s));
+  return PAMD_OK;
+}
+
+int pamd_ccsdt_num_triples(int nocc, long long *n) {
+  if (nocc < 1 || !n) return PAMD_ERR_ARG;
+  *n = (long long)nocc * (nocc + 1) * (nocc + 2) / 6 - nocc;
+  return PAMD_OK;
+}
+
+int pamd_ccsdt_energy_shard(const pamd_ccsdt_config *cfg, const double *mo_energy,
+                            const double *t1, const double *t2, const double *ovvv,
+                            const double *ovoo, const double *ovov, const double *fvo, int shard,
+                            int nshards, void *workspace, size_t workspace_bytes, double *e_out,
+                            void *stream) {
+  if (!e_out || nshards < 1 || shard < 0 || shard >= nshards) return PAMD_ERR_ARG;
+  Ctx c;
+  hipStream_t s = (hipStream_t)stream;
+  int st = energy_impl(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, workspace, workspace_bytes,
+                       s, c, shard, nshards);
   if (st != PAMD_OK) return st;
   PAMD_CHECK(launch_total_energy(c.tri, c.o, e_out, s));
   return PAMD_OK;

@@ -188,3 +188,15 @@ def test_lno_kernel_drop_in_autograd(cuda):
     assert _rel(tl['ovvv'].grad.cpu().numpy(), O.fold_ovvv_packed(g['ovvv_bar'])) <= G_RTOL
     fb = tl['fock'].grad.cpu().numpy()
     assert _rel(fb[4:, :4], g['fvo_bar']) <= G_RTOL and not np.any(fb[:4, :])
+
+
+def test_energy_shards_add_up(cuda):
+    """Round-robin shards of the triple walk sum to E(T)."""
+    p = make_problem(6, 24, 96, seed=4, fvo_scale=1e-3)
+    print('energy shards o6v24', flush=True)
+    inp = _inputs(p, True, cuda)
+    e = float(_ops.energy(inp).item())
+    parts = [float(_ops.energy_shard(inp, s, 3).item()) for s in range(3)]
+    assert all(x != 0.0 for x in parts)
+    assert abs(sum(parts) - e) <= 1e-13 * abs(e)
+    assert float(_ops.energy_shard(inp, 0, 1).item()) == e
