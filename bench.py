@@ -67,6 +67,9 @@ def engine_roofline(prof, precision):
     if precision == 'bf16x3':
         # six bf16 MFMA partial products per FP64-equivalent multiply-add
         kernel, mfma_factor, peak = 'k_gemm_b3 / k_gemm_b3n', 6.0, PEAKS['bf16']
+    elif precision == 'fp32x2':
+        # three fp32 MFMA products (hi*hi, hi*lo, lo*hi) per multiply-add
+        kernel, mfma_factor, peak = 'k_gemm_f64 (FP32 hi/lo mode)', 3.0, PEAKS['fp32']
     else:
         kernel, mfma_factor, peak = 'k_gemm_f64', 1.0, PEAKS['fp64']
     achieved = fp64_equiv * mfma_factor if fp64_equiv else None
@@ -92,7 +95,9 @@ def engine_traffic(precision):
     """HBM bytes per launch of the contraction engine (dispatch-weighted mean
     over its tile shapes) from the committed rocprofv3 --pmc FETCH_SIZE /
     WRITE_SIZE summary, or None when that file is absent."""
-    path = TRAFFIC_FILES[precision]
+    path = TRAFFIC_FILES.get(precision)
+    if path is None:
+        return None, None
     prefix = 'k_gemm_b3' if precision == 'bf16x3' else 'k_gemm_f64'
     if not os.path.exists(path):
         return None, None
@@ -143,7 +148,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=1)
     ap.add_argument('--warmup', type=int, default=0)
-    ap.add_argument('--precision', default='bf16x3', choices=['fp64', 'bf16x3'])
+    ap.add_argument('--precision', default='bf16x3', choices=['fp64', 'bf16x3', 'fp32x2'])
     ap.add_argument('--compare', default='fp64', choices=['fp64', 'none'],
                     help='after the timed steps, run one plain FP64 step on the same inputs '
                          'and report its time and the |dE| / gradient errors against it')

@@ -32,7 +32,9 @@ extern "C" {
 
 #define PAMD_PREC_FP64 0        /* plain form: FP64 MFMA                       */
 #define PAMD_PREC_BF16X3 1      /* split-BF16x3 MFMA, FP32 tile accumulation   */
-#define PAMD_PREC_FP32X2 2      /* FP32 hi/lo split on FP32 MFMA (comparison)  */
+#define PAMD_PREC_FP32X2 2      /* FP32 hi/lo split on FP32 MFMA (comparison form:
+                                   three products per multiply-add, FP32 tile sums
+                                   added into FP64 every 128 k)                 */
 
 typedef struct {
     int nocc;

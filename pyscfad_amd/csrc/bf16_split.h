@@ -25,8 +25,23 @@ __device__ __forceinline__ unsigned long long split3(double x) {
   return (unsigned long long)h | ((unsigned long long)m << 16) | ((unsigned long long)l << 32);
 }
 
+// FP64 -> FP32 hi/lo pair in one 8-byte element: bits 0-31 hi = fp32(x),
+// 32-63 lo = fp32(x - hi) (the comparison form of the FP32 MFMA engine).
+struct F2 {
+  unsigned long long u;
+};
+
+__device__ __forceinline__ F2 split2(double x) {
+  const float h = (float)x;
+  const float l = (float)(x - (double)h);
+  F2 r;
+  r.u = (unsigned long long)__float_as_uint(h) | ((unsigned long long)__float_as_uint(l) << 32);
+  return r;
+}
+
 // store policy of the layout kernels: FP64 copy or packed split
 __device__ __forceinline__ void put(double *p, size_t t, double x) { p[t] = x; }
 __device__ __forceinline__ void put(unsigned long long *p, size_t t, double x) { p[t] = split3(x); }
+__device__ __forceinline__ void put(F2 *p, size_t t, double x) { p[t] = split2(x); }
 
 }  // namespace pamd

@@ -58,6 +58,9 @@ struct GemmDesc {
 
 constexpr int kMaxFrag = 8;  // rows of the local-orbital projector ulo
 
+// packed 8-byte operand formats of the GEMM engines
+enum SplitFormat { kSplitNone = 0, kSplitBf16x3 = 1, kSplitF32x2 = 2 };
+
 // Epilogue over orbits of virtual tiles for one occupied triple.
 struct OrbitArgs {
   double *W;            // v^3; W in (and Wbar out when Wb8 is null, vjp)
@@ -78,6 +81,7 @@ struct OrbitArgs {
   int nfrag;
   double u[kMaxFrag];
   double *part_u;       // [G][kMaxFrag] per-workgroup dE/du_f sums (projected adjoint)
+  int wsplit;           // format of Wb8: kSplitBf16x3 or kSplitF32x2
 };
 
 enum OrbitMode { kOrbitPlain = 0, kOrbitProjAcc = 1, kOrbitProjVjp = 2 };
@@ -112,8 +116,10 @@ hipError_t launch_fill(double *p, size_t n, double val, hipStream_t s);
 hipError_t launch_orbit_table(long long norb, unsigned short *tab, hipStream_t s);
 hipError_t launch_prep_sym(const double *src, int o, int v, int kind, double *dst, hipStream_t s);
 hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, const double *ovoo,
-                          int o, int v, double *Vp, double *T2m, double *OOt, int split_bf16,
+                          int o, int v, double *Vp, double *T2m, double *OOt, int split,
                           hipStream_t s);
+hipError_t launch_split_f32x2(const double *x, size_t n, unsigned long long *out, hipStream_t s);
+hipError_t launch_gemm_f32x2(const GemmDesc &d, hipStream_t s);
 hipError_t launch_split_bf16x3(const double *x, size_t n, unsigned long long *out, hipStream_t s);
 hipError_t launch_gemm_bf16x3(const GemmDesc &d, hipStream_t s);
 hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s);

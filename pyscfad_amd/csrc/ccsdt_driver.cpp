@@ -135,6 +135,8 @@ static hipError_t run_gemm(const GemmDesc &d, hipStream_t s, int kind) {
   }
   if (g_prec == PAMD_PREC_BF16X3)
     return timed(s, kind, gemm_flops(d), 0.0, [&] { return launch_gemm_bf16x3(d, s); });
+  if (g_prec == PAMD_PREC_FP32X2)
+    return timed(s, kind, gemm_flops(d), 0.0, [&] { return launch_gemm_f32x2(d, s); });
   return timed(s, kind, gemm_flops(d), 0.0, [&] { return launch_gemm_f64(d, s); });
 }
 
@@ -198,7 +200,7 @@ static Layout make_layout(int o, int v, int vjp, int prec) {
   L.off_tri = take((size_t)o * o * o);
   L.off_part_t = take(kGmax);
   L.off_wt = take((size_t)o);
-  if (prec == PAMD_PREC_BF16X3) {
+  if (prec == PAMD_PREC_BF16X3 || prec == PAMD_PREC_FP32X2) {
     L.off_T2g = take((size_t)o * o * v2);
     if (vjp) L.off_W8 = take(v3);
   }
@@ -443,6 +445,12 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
   return PAMD_OK;
 }
 
+// packed operand format of a precision (FP64 operands are not packed)
+static int split_format(int prec) {
+  return prec == PAMD_PREC_BF16X3 ? kSplitBf16x3
+                                  : (prec == PAMD_PREC_FP32X2 ? kSplitF32x2 : kSplitNone);
+}
+
 // Zero the adjoint accumulators and the cotangent outputs of one call.
 static int zero_vjp(const Ctx &c) {
   const int o = c.o, v = c.v;
@@ -462,12 +470,14 @@ static int zero_vjp(const Ctx &c) {
 static int setup(Ctx &c, long long &norb) {
   const int o = c.o, v = c.v;
   g_prec = c.prec;
-  const int bf = c.prec == PAMD_PREC_BF16X3;
-  PAMD_CHECK(launch_prepare(c.ovvv, c.packed, c.t2, c.ovoo, o, v, c.Vp, c.T2m, c.OOt, bf, c.s));
+  const int fmt = split_format(c.prec);
+  PAMD_CHECK(launch_prepare(c.ovvv, c.packed, c.t2, c.ovoo, o, v, c.Vp, c.T2m, c.OOt, fmt, c.s));
   PAMD_CHECK(launch_prep_sym(c.ovov, o, v, 0, c.OVS, c.s));
   PAMD_CHECK(launch_prep_sym(c.t2, o, v, 1, c.T2S, c.s));
-  if (bf)
+  if (fmt == kSplitBf16x3)
     PAMD_CHECK(launch_split_bf16x3(c.t2, (size_t)o * o * v * v, (unsigned long long *)c.gT2, c.s));
+  if (fmt == kSplitF32x2)
+    PAMD_CHECK(launch_split_f32x2(c.t2, (size_t)o * o * v * v, (unsigned long long *)c.gT2, c.s));
   PAMD_CHECK(launch_fill(c.tri, (size_t)o * o * o, 0.0, c.s));
   const int nt = (v + 7) / 8;
   norb = (long long)nt * (nt + 1) * (nt + 2) / 6;
@@ -534,7 +544,7 @@ static int run_projected(Ctx &c, const double *ulo, int nfrag, double *accA, dou
     }
     return PAMD_OK;
   };
-  const int bf = c.prec == PAMD_PREC_BF16X3;
+  const int fmt = split_format(c.prec);
   for (int j = 0; j < o; ++j)
     for (int k = j; k < o; ++k) {
       PAMD_CHECK(launch_fill(accA, (size_t)nfrag * v3, 0.0, c.s));
@@ -565,7 +575,8 @@ static int run_projected(Ctx &c, const double *ulo, int nfrag, double *accA, dou
         st = build_w(c, l, j, k);
         if (st != PAMD_OK) return st;
         OrbitArgs oa = orbit_args(c, l, j, k, norb);
-        oa.Wb8 = bf ? (unsigned long long *)c.gW : nullptr;
+        oa.Wb8 = fmt != kSplitNone ? (unsigned long long *)c.gW : nullptr;
+        oa.wsplit = fmt;
         oa.vjp = 1;
         oa.ebar = g;
         oa.accA = accA;
@@ -590,7 +601,7 @@ static int run_projected(Ctx &c, const double *ulo, int nfrag, double *accA, dou
 
 static int run(Ctx &c) {
   const int o = c.o, v = c.v;
-  const int bf = c.prec == PAMD_PREC_BF16X3;
+  const int fmt = split_format(c.prec);
   long long norb = 0;
   {
     const int st0 = setup(c, norb);
@@ -615,7 +626,8 @@ static int run(Ctx &c) {
         int st = build_w(c, i, j, k);
         if (st != PAMD_OK) return st;
         OrbitArgs oa = orbit_args(c, i, j, k, norb);
-        oa.Wb8 = (bf && c.vjp) ? (unsigned long long *)c.gW : nullptr;
+        oa.Wb8 = (fmt != kSplitNone && c.vjp) ? (unsigned long long *)c.gW : nullptr;
+        oa.wsplit = fmt;
         oa.vjp = c.vjp;
         oa.omega = (i == j || j == k) ? 1.0 : 2.0;
         // a diagonal occupied projector diag(m) weighs the whole triple by
@@ -651,7 +663,8 @@ static int check_common(const pamd_ccsdt_config *cfg, const double *mo, const do
   if (!cfg || cfg->nocc < 1 || cfg->nvir < 1) return PAMD_ERR_ARG;
   if (!mo || !t1 || !t2 || !ovvv || !ovoo || !ovov) return PAMD_ERR_ARG;
   if (cfg->has_fvo && !fvo) return PAMD_ERR_ARG;
-  if (cfg->precision != PAMD_PREC_FP64 && cfg->precision != PAMD_PREC_BF16X3)
+  if (cfg->precision != PAMD_PREC_FP64 && cfg->precision != PAMD_PREC_BF16X3 &&
+      cfg->precision != PAMD_PREC_FP32X2)
     return PAMD_ERR_UNSUPPORTED;
   if (cfg->nvir > 4096 || cfg->nocc > 1024) return PAMD_ERR_ARG;
   return PAMD_OK;
@@ -669,8 +682,9 @@ static void bind_workspace(Ctx &c, const Layout &L, void *ws) {
   c.tri = (double *)(b + L.off_tri);
   c.part_t = (double *)(b + L.off_part_t);
   c.wt = (double *)(b + L.off_wt);
-  c.gT2 = c.prec == PAMD_PREC_BF16X3 ? (const double *)(b + L.off_T2g) : c.t2;
-  c.gW = (c.prec == PAMD_PREC_BF16X3 && c.vjp) ? (double *)(b + L.off_W8) : c.W;
+  const bool packed_ops = c.prec == PAMD_PREC_BF16X3 || c.prec == PAMD_PREC_FP32X2;
+  c.gT2 = packed_ops ? (const double *)(b + L.off_T2g) : c.t2;
+  c.gW = (packed_ops && c.vjp) ? (double *)(b + L.off_W8) : c.W;
   if (c.vjp) {
     c.Vb = (double *)(b + L.off_Vb);
     c.Vpb = (double *)(b + L.off_Vpb);

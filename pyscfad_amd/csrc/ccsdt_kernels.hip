@@ -13,6 +13,7 @@
 // * k_vr_a/k_vr_b -- contractions of Vbar with t1/fvo/ovov/t2 (the V-term adjoint).
 // * k_combine    -- per-triple gather of the adjoint pieces into the cotangents.
 // * prepare/fold -- layout changes at entry/exit.
+#include <type_traits>
 #include "ccsdt_common.h"
 #include "engine_staging.h"
 #include "bf16_split.h"
@@ -21,6 +22,7 @@ namespace pamd {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef double d2 __attribute__((ext_vector_type(2)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // generic helpers
@@ -132,13 +134,29 @@ static hipError_t prepare_as(const double *ovvv, int packed, const double *t2, c
 
 // split_bf16: write packed (hi, mid, lo) bf16 triplets instead of FP64 copies
 hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, const double *ovoo,
-                          int o, int v, double *Vp, double *T2m, double *OOt, int split_bf16,
+                          int o, int v, double *Vp, double *T2m, double *OOt, int split,
                           hipStream_t s) {
-  if (split_bf16) {
+  if (split == kSplitBf16x3) {
     typedef unsigned long long u64;
     return prepare_as<u64>(ovvv, packed, t2, ovoo, o, v, (u64 *)Vp, (u64 *)T2m, (u64 *)OOt, s);
   }
+  if (split == kSplitF32x2)
+    return prepare_as<F2>(ovvv, packed, t2, ovoo, o, v, (F2 *)Vp, (F2 *)T2m, (F2 *)OOt, s);
   return prepare_as<double>(ovvv, packed, t2, ovoo, o, v, Vp, T2m, OOt, s);
+}
+
+__global__ void k_split_f32x2(const double *__restrict__ x, size_t n, F2 *__restrict__ out) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (size_t)gridDim.x * blockDim.x)
+    out[t] = split2(x[t]);
+}
+
+hipError_t launch_split_f32x2(const double *x, size_t n, unsigned long long *out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  size_t b = (n + 255) / 256;
+  if (b > 262144) b = 262144;
+  hipLaunchKernelGGL(k_split_f32x2, dim3((unsigned)b), dim3(256), 0, s, x, n, (F2 *)out);
+  return hipGetLastError();
 }
 
 // Pair-symmetrised blocks, dst[x][y][p][q] = src(x,p,y,q) + src(y,q,x,p), for
@@ -194,9 +212,15 @@ hipError_t launch_prep_sym(const double *src, int o, int v, int kind, double *ds
 // Tile MT x NT with WM x WN waves; each wave (MT/WM) x (NT/WN).  The grid is
 // 1-D over (split, tile) with the N tiles innermost when n_fast (the A panel
 // is then read once and reused from L2), else the M tiles innermost.
-template <int MT, int NT, int WM, int WN>
+// FX2: the FP32 hi/lo comparison form.  Elements are F2 pairs (hi, lo); each
+// MFMA step runs three v_mfma_f32_16x16x4f32 (hi*hi, hi*lo, lo*hi) into FP32
+// tile sums, added into the FP64 accumulators every kFlush2 chunks (128 k).
+// The FP32 MFMA's D layout differs from the FP64 one (rows 4*(l/16) + r).
+template <int MT, int NT, int WM, int WN, bool FX2>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, int n_fast) {
   constexpr int NTHR = 64 * WM * WN;
+  using E = typename std::conditional<FX2, F2, double>::type;
+  constexpr int kFlush2 = 8;
   using SA = OpStage<MT, NTHR>;
   using SB = OpStage<NT, NTHR>;
   constexpr int WTM = MT / WM, WTN = NT / WN;
@@ -219,10 +243,18 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, int n_fas
   const int g1 = min(total, g0 + cps);
 
   d4 acc[TM][TN];
+  f4 facc[FX2 ? TM : 1][FX2 ? TN : 1];
 #pragma unroll
   for (int a = 0; a < TM; ++a)
 #pragma unroll
     for (int b = 0; b < TN; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  if constexpr (FX2) {
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int b = 0; b < TN; ++b) facc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  }
+  int nfl = 0;
 
   // fetch state: the segment of the next chunk to load, held in registers
   int fs = 0;
@@ -235,7 +267,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, int n_fas
   KPos kb{kappa / FB.kdiv, kappa - (kappa / FB.kdiv) * FB.kdiv};
   int ra_off[SA::E], rb_off[SB::E];
   bool ra_ok[SA::E], rb_ok[SB::E];
-  double va[SA::E], vb[SB::E];
+  E va[SA::E], vb[SB::E];
   bool oka[SA::E], okb[SB::E];
   op_rows<MT, NTHR>(FA, d.M, m0, tid, ra_off, ra_ok);
   op_rows<NT, NTHR>(FB, d.N, n0, tid, rb_off, rb_ok);
@@ -247,8 +279,8 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, int n_fas
   double alpha = Falpha;             // and its segment's scale
   int buf = 0;
   for (int g = g0; g < g1; ++g) {
-    op_stash<MT, NTHR>(la, alpha, tid, va, oka, As[buf]);
-    op_stash<NT, NTHR>(lb, 1.0, tid, vb, okb, Bs[buf]);
+    op_stash<MT, NTHR>(la, alpha, tid, va, oka, reinterpret_cast<E *>(As[buf]));
+    op_stash<NT, NTHR>(lb, 1.0, tid, vb, okb, reinterpret_cast<E *>(Bs[buf]));
     __syncthreads();
     const int cla = la, clb = lb;
     if (g + 1 < g1) {
@@ -328,9 +360,34 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, int n_fas
 #pragma unroll
         for (int x = 0; x < TM; ++x)
 #pragma unroll
+          for (int y = 0; y < TN; ++y) {
+            if constexpr (FX2) {
+              const unsigned long long ua = __double_as_longlong(fa[ks][x]);
+              const unsigned long long ub = __double_as_longlong(fb[ks][y]);
+              const float ah = __uint_as_float((unsigned)ua), al = __uint_as_float((unsigned)(ua >> 32));
+              const float bh = __uint_as_float((unsigned)ub), bl = __uint_as_float((unsigned)(ub >> 32));
+              facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, facc[x][y], 0, 0, 0);
+              facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, facc[x][y], 0, 0, 0);
+              facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, facc[x][y], 0, 0, 0);
+            } else {
+              acc[x][y] =
+                  __builtin_amdgcn_mfma_f64_16x16x4f64(fa[ks][x], fb[ks][y], acc[x][y], 0, 0, 0);
+            }
+          }
+    }
+    if constexpr (FX2) {
+      if (++nfl == kFlush2 || g + 1 == g1) {
+        nfl = 0;
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+#pragma unroll
           for (int y = 0; y < TN; ++y)
-            acc[x][y] =
-                __builtin_amdgcn_mfma_f64_16x16x4f64(fa[ks][x], fb[ks][y], acc[x][y], 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              acc[x][y][r] += (double)facc[x][y][r];
+              facc[x][y][r] = 0.f;
+            }
+      }
     }
     buf ^= 1;
   }
@@ -343,7 +400,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, int n_fas
     for (int y = 0; y < TN; ++y)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int ml = wm * WTM + x * 16 + 4 * r + (lane >> 4);
+        const int ml = wm * WTM + x * 16 + (FX2 ? 4 * (lane >> 4) + r : 4 * r + (lane >> 4));
         const int nl = wn * WTN + y * 16 + (lane & 15);
         const bool part = d.mode == kPartial;
         const OutPos pm = out_pos<MT>(tm, ml, part ? (1 << 30) : d.C.mdiv);
@@ -370,10 +427,26 @@ hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s) {
   // larger output dimension, whose operand panel is then read once (n_fast).
   if (d.M >= d.N) {
     const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 63) / 64) * d.splits;
-    hipLaunchKernelGGL((k_gemm_f64<128, 64, 4, 2>), dim3((unsigned)tiles), dim3(512), 0, s, d, 1);
+    hipLaunchKernelGGL((k_gemm_f64<128, 64, 4, 2, false>), dim3((unsigned)tiles), dim3(512), 0, s,
+                       d, 1);
   } else {
     const long long tiles = (long long)((d.M + 63) / 64) * ((d.N + 127) / 128) * d.splits;
-    hipLaunchKernelGGL((k_gemm_f64<64, 128, 2, 4>), dim3((unsigned)tiles), dim3(512), 0, s, d, 0);
+    hipLaunchKernelGGL((k_gemm_f64<64, 128, 2, 4, false>), dim3((unsigned)tiles), dim3(512), 0, s,
+                       d, 0);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm_f32x2(const GemmDesc &d, hipStream_t s) {
+  if (d.M <= 0 || d.N <= 0) return hipSuccess;
+  if (d.M >= d.N) {
+    const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 63) / 64) * d.splits;
+    hipLaunchKernelGGL((k_gemm_f64<128, 64, 4, 2, true>), dim3((unsigned)tiles), dim3(512), 0, s,
+                       d, 1);
+  } else {
+    const long long tiles = (long long)((d.M + 63) / 64) * ((d.N + 127) / 128) * d.splits;
+    hipLaunchKernelGGL((k_gemm_f64<64, 128, 2, 4, true>), dim3((unsigned)tiles), dim3(512), 0, s,
+                       d, 0);
   }
   return hipGetLastError();
 }
@@ -595,7 +668,7 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
           const double vbar = r3Q * rD;
           const double wbar = A.ebar * pw + vbar;
           if (A.Wb8)
-            A.Wb8[idx] = split3(wbar);
+            A.Wb8[idx] = A.wsplit == kSplitF32x2 ? split2(wbar).u : split3(wbar);
           else
             A.W[idx] = wbar;
           A.Vb[idx] = vbar;
@@ -607,7 +680,7 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
             const double vbar = g * r3W * rD;
             const double wbar = g * Z * rD + vbar;
             if (A.Wb8)
-              A.Wb8[idx] = split3(wbar);
+              A.Wb8[idx] = A.wsplit == kSplitF32x2 ? split2(wbar).u : split3(wbar);
             else
               A.W[idx] = wbar;
             A.Vb[idx] = vbar;

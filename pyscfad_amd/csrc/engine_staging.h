@@ -8,6 +8,7 @@
 //   K-contiguous operand   ("kfast") -> [r][k], row stride 18 elements
 // Elements are 8 bytes: a double, or a packed bf16 (hi, mid, lo) triplet.
 #pragma once
+#include "bf16_split.h"
 #include "ccsdt_common.h"
 
 namespace pamd {
@@ -133,6 +134,13 @@ __device__ __forceinline__ double staged(double v, bool ok, double alpha) {
 }
 __device__ __forceinline__ unsigned long long staged(unsigned long long v, bool ok, double alpha) {
   return ok ? (alpha < 0.0 ? v ^ 0x0000800080008000ull : v) : 0ull;
+}
+
+// FP32 hi/lo pairs (alpha = +-1): the two sign bits are flipped
+__device__ __forceinline__ F2 staged(F2 v, bool ok, double alpha) {
+  F2 r;
+  r.u = ok ? (alpha < 0.0 ? v.u ^ 0x8000000080000000ull : v.u) : 0ull;
+  return r;
 }
 
 template <int R, int NTHR, class T>

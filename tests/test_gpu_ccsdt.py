@@ -200,3 +200,19 @@ def test_energy_shards_add_up(cuda):
     assert all(x != 0.0 for x in parts)
     assert abs(sum(parts) - e) <= 1e-13 * abs(e)
     assert float(_ops.energy_shard(inp, 0, 1).item()) == e
+
+
+@pytest.mark.parametrize('o,v,fvo', [(4, 16, 0.0), (6, 24, 1e-3)])
+def test_fp32x2_comparison_form(o, v, fvo, cuda):
+    """FP32 hi/lo form (comparison): energy and every cotangent vs the oracle."""
+    p = make_problem(o, v, 4 * v, seed=5, fvo_scale=fvo)
+    print(f'fp32x2 o{o}v{v}', flush=True)
+    args = (p.mo_energy, p.t1, p.t2, p.ovvv, p.ovoo, p.ovov, p.fvo)
+    e_ref, g_ref = O.energy_vjp(*args, et_bar=1.0)
+    inp = _ops.Inputs(p.mo_energy, p.t1, p.t2, p.ovvv_packed(), p.ovoo, p.ovov,
+                      p.fvo if fvo else None, precision='fp32x2', device=cuda)
+    e, g = _ops.energy_vjp(inp, et_bar=1.0)
+    assert abs(float(e.item()) - e_ref) <= 1e-9 * abs(e_ref) + 1e-12
+    for k in BLOCKS:
+        assert _rel(g[k].cpu().numpy(), g_ref[k]) <= 1e-5, k
+    assert _rel(g['ovvv_bar'].cpu().numpy(), O.fold_ovvv_packed(g_ref['ovvv_bar'])) <= 1e-5
