@@ -448,8 +448,11 @@ __device__ __forceinline__ void b3n_products(bf16x8 ah, bf16x8 am, bf16x8 al, bf
   acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
 }
 
-template <int MT, int NT, int WM, int WN>
-__global__ __launch_bounds__(512) void k_gemm_b3n(GemmDesc d, int n_fast) {
+// MINW = 4 (with F32ACC): two workgroups per CU.  The FP32 tile sums then
+// cover the workgroup's whole K range (no FP64 accumulators, which would not
+// fit 128 VGPRs); the cross-workgroup and split-K sums stay FP64.
+template <int MT, int NT, int WM, int WN, int MINW = 2, bool F32ACC = false>
+__global__ __launch_bounds__(512, MINW) void k_gemm_b3n(GemmDesc d, int n_fast) {
   static_assert(WM * WN == 8, "8 waves");
   constexpr int WTM = MT / WM, WTN = NT / WN;
   constexpr int TM = WTM / 16, TN = WTN / 16;
@@ -473,7 +476,7 @@ __global__ __launch_bounds__(512) void k_gemm_b3n(GemmDesc d, int n_fast) {
   const int g0 = split * cps;
   const int g1 = min(total, g0 + cps);
 
-  double acc[TM][TN][4];
+  double acc[F32ACC ? 1 : TM][F32ACC ? 1 : TN][4];
   f32x4 fa[TM][TN];
 #pragma unroll
   for (int a = 0; a < TM; ++a)
@@ -481,7 +484,7 @@ __global__ __launch_bounds__(512) void k_gemm_b3n(GemmDesc d, int n_fast) {
     for (int b = 0; b < TN; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        acc[a][b][r] = 0.0;
+        if constexpr (!F32ACC) acc[a][b][r] = 0.0;
         fa[a][b][r] = 0.f;
       }
 
@@ -537,15 +540,17 @@ __global__ __launch_bounds__(512) void k_gemm_b3n(GemmDesc d, int n_fast) {
         for (int b = 0; b < TN; ++b) b3n_products(ah, am, al, bh[b], bm[b], bl[b], fa[a][b]);
       }
     }
-    if (++nfl == kFlush || g + 1 == g1) {
+    if (!F32ACC && (++nfl == kFlush || g + 1 == g1)) {
 #pragma unroll
       for (int a = 0; a < TM; ++a)
 #pragma unroll
         for (int b = 0; b < TN; ++b)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            acc[a][b][r] += (double)fa[a][b][r];
-            fa[a][b][r] = 0.f;
+            if constexpr (!F32ACC) {
+              acc[a][b][r] += (double)fa[a][b][r];
+              fa[a][b][r] = 0.f;
+            }
           }
       nfl = 0;
     }
@@ -566,7 +571,11 @@ __global__ __launch_bounds__(512) void k_gemm_b3n(GemmDesc d, int n_fast) {
         const OutPos pn = out_pos<NT>(tn, nl, part ? (1 << 30) : d.C.ndiv);
         const int m = pm.idx, n = pn.idx;
         if (m >= d.M || n >= d.N) continue;
-        const double val = acc[a][b][r];
+        double val;
+        if constexpr (F32ACC)
+          val = (double)fa[a][b][r];
+        else
+          val = acc[a][b][r];
         if (part) {
           d.partial[((size_t)split * d.M + m) * d.N + n] = val;
         } else {
@@ -586,15 +595,26 @@ static double tile_eff(int M, int N, int MT, int NT) {
   return (double)M * N / (pm * pn);
 }
 
-// Tile shapes of the split-BF16x3 engine.  The score is the padded tiling's
-// efficiency times the panel's flops per staged byte (MT NT / (MT + NT)).
-enum B3Cfg { kB3_256x64 = 0, kB3_64x256, kB3_256x80, kB3_80x256, kB3_256x48, kB3_48x256, kB3Num };
-static const int kB3Shape[kB3Num][2] = {{256, 64}, {64, 256}, {256, 80}, {80, 256}, {256, 48}, {48, 256}};
+// Tile shapes of the split-BF16x3 engine: {MT, NT, workgroups per CU}.  The
+// 16x16-MFMA shapes run two workgroups per CU (FP32 tile sums over the whole
+// K range of a workgroup), which hides one workgroup's operand staging and
+// epilogue behind the other's MFMA work; they are used whenever one of them
+// covers the output at least half-efficiently.  Among the candidates the
+// score is the padded tiling's efficiency times the tile's flops per staged
+// byte, MT NT / (MT + NT).
+enum B3Cfg { kB3_128x80 = 0, kB3_80x128, kB3_128x48, kB3_48x128, kB3_256x64, kB3_64x256, kB3Num };
+static const int kB3Shape[kB3Num][3] = {{128, 80, 2}, {80, 128, 2}, {128, 48, 2},
+                                        {48, 128, 2}, {256, 64, 1}, {64, 256, 1}};
+constexpr int kB3Wide = 4;  // configurations 0 .. 3 are the 16x16-MFMA ones
 
 static int b3_pick(int M, int N) {
+  double best_eff = 0.0;
+  for (int c = 0; c < kB3Wide; ++c)
+    best_eff = std::max(best_eff, tile_eff(M, N, kB3Shape[c][0], kB3Shape[c][1]));
+  const int ncand = best_eff >= 0.5 ? kB3Wide : kB3Num;
   int best = 0;
   double bs = -1.0;
-  for (int c = 0; c < kB3Num; ++c) {
+  for (int c = 0; c < ncand; ++c) {
     const int MT = kB3Shape[c][0], NT = kB3Shape[c][1];
     const double sc = tile_eff(M, N, MT, NT) * (double)MT * NT / (MT + NT);
     if (sc > bs + 1e-9) {
@@ -611,6 +631,8 @@ int gemm_tiles_b3(int M, int N) {
   return ((M + MT - 1) / MT) * ((N + NT - 1) / NT);
 }
 
+int gemm_wgs_per_cu_b3(int M, int N) { return kB3Shape[b3_pick(M, N)][2]; }
+
 hipError_t launch_gemm_b3(const GemmDesc &d0, hipStream_t s) {
   if (d0.M <= 0 || d0.N <= 0) return hipSuccess;
   GemmDesc d = d0;  // chunks of BKC
@@ -624,10 +646,10 @@ hipError_t launch_gemm_b3(const GemmDesc &d0, hipStream_t s) {
   switch (c) {
     case kB3_256x64: hipLaunchKernelGGL((k_gemm_b3<256, 64, 4, 2>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
     case kB3_64x256: hipLaunchKernelGGL((k_gemm_b3<64, 256, 2, 4>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
-    case kB3_256x80: hipLaunchKernelGGL((k_gemm_b3n<256, 80, 8, 1>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
-    case kB3_80x256: hipLaunchKernelGGL((k_gemm_b3n<80, 256, 1, 8>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
-    case kB3_256x48: hipLaunchKernelGGL((k_gemm_b3n<256, 48, 8, 1>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
-    default: hipLaunchKernelGGL((k_gemm_b3n<48, 256, 1, 8>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
+    case kB3_128x80: hipLaunchKernelGGL((k_gemm_b3n<128, 80, 8, 1, 4, true>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
+    case kB3_80x128: hipLaunchKernelGGL((k_gemm_b3n<80, 128, 1, 8, 4, true>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
+    case kB3_128x48: hipLaunchKernelGGL((k_gemm_b3n<128, 48, 8, 1, 4, true>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
+    default: hipLaunchKernelGGL((k_gemm_b3n<48, 128, 1, 8, 4, true>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
   }
   return hipGetLastError();
 }

@@ -125,6 +125,7 @@ hipError_t launch_gemm_bf16x3(const GemmDesc &d, hipStream_t s);
 hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s);
 int gemm_tiles(int M, int N);     // FP64 engine: workgroup tiles of one split
 int gemm_tiles_b3(int M, int N);  // split-BF16x3 engine: workgroup tiles of one split
+int gemm_wgs_per_cu_b3(int M, int N);  // split-BF16x3 engine: resident workgroups per CU
 hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s, int mode = kOrbitPlain);
 hipError_t launch_vr(const VRArgs &a, hipStream_t s);
 hipError_t launch_combine(const CombineArgs &a, hipStream_t s);

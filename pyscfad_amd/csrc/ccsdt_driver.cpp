@@ -161,14 +161,14 @@ static int device_cus() {
 }
 
 // Split-K factor for an M x N output with `chunks` K chunks of 16: one full
-// round of resident workgroups (two per CU for the FP64 engine, one for the
-// split-BF16x3 engine), rounded down so that no workgroup waits for a second
-// round.  (One round measured faster than two or three: fewer split-K
+// round of resident workgroups (two per CU for the FP64 engine and for the
+// split-BF16x3 16x16-MFMA tiles, one for its 32x32 tiles), rounded down so
+// that no workgroup waits for a second round.  (One round measured faster than two or three: fewer split-K
 // partials to write and reduce, same engine time.)
 static int splits_for(int M, int N, int chunks, int prec) {
   const bool b3 = prec == PAMD_PREC_BF16X3;
   const int tiles = std::max(1, b3 ? gemm_tiles_b3(M, N) : gemm_tiles(M, N));
-  const int slots = (b3 ? 1 : 2) * device_cus();
+  const int slots = (b3 ? gemm_wgs_per_cu_b3(M, N) : 2) * device_cus();
   int s = slots / tiles;
   s = std::max(1, std::min(s, 512));
   return std::min(s, std::max(1, b3 ? chunks / 2 : chunks));
