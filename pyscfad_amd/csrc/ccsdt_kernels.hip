@@ -491,10 +491,13 @@ int orbit_grid(int v) {
 // the p-th distinct tile from (A, B, C) (candidates in the order abc, acb, bac,
 // bca, cab, cba, first occurrence kept); pmap[s][p] is the tile that holds the
 // bca, cab, cba, acb, bac permutations (s = 0..4) of a position in tile p.
+// (int fields: gfx950 has no scalar byte loads, and a vector load here would
+// make every wait on it also wait for the kernel's outstanding W loads and
+// Wbar/Vbar stores, which share the vector-memory counter)
 struct OrbitPattern {
   int np;
-  unsigned char sel[6][3];
-  unsigned char pmap[5][6];
+  int sel[6][3];
+  int pmap[5][6];
 };
 __constant__ OrbitPattern kOrbitPat[4] = {
     {6, {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}},
@@ -506,6 +509,8 @@ __constant__ OrbitPattern kOrbitPat[4] = {
     {1, {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
      {{0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}}},
 };
+
+typedef const __attribute__((address_space(4))) unsigned long long *OrbitEntryPtr;
 
 // Orbit table: entry q = (A, B, C, pattern) of the q-th tile triple A >= B >= C;
 // pattern = (A == B) + 2 (B == C) selects kOrbitPat.
@@ -546,6 +551,7 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
   __shared__ double Xs[6][OT3];
   __shared__ double Qs[MODE == kOrbitProjVjp ? 6 : 1][MODE == kOrbitProjVjp ? OT3 : 1];
   __shared__ double red[8];
+  __shared__ double evs[3 * OT];  // virtual orbital energies of the orbit's tile ranges
   __shared__ double ured[MODE == kOrbitProjVjp ? 8 * kMaxFrag : 1];
   extern __shared__ double vir[];  // v doubles: dE/de_virtual accumulated by this workgroup
   constexpr bool kVjp = MODE != kOrbitProjAcc;  // seeds (plain: when A.vjp)
@@ -593,33 +599,72 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
   // workgroups need meet in one L2
   const int first = xcd_remap(blockIdx.x, gridDim.x);
   for (long long q = first; q < A.norb; q += gridDim.x) {
-    const unsigned short *ent = A.tab + 4 * q;
-    const int val[3] = {ent[0], ent[1], ent[2]};
-    const OrbitPattern &pt = kOrbitPat[ent[3]];
+    // The entry is read through the constant address space (the table is not
+    // written while the kernel runs), so it and the pattern it selects are
+    // scalar loads: they do not count against the vector-memory counter, and
+    // so no wait for them also waits for the previous tile loads.
+    const unsigned long long ent = ((const OrbitEntryPtr)A.tab)[q];
+    const int val[3] = {(int)(ent & 0xffff), (int)((ent >> 16) & 0xffff), (int)((ent >> 32) & 0xffff)};
+    const OrbitPattern &pt = kOrbitPat[(int)(ent >> 48)];
     const int np = pt.np;
-    // phase 1: W and X = W + V of the orbit's tiles
-    for (int p = 0; p < np; ++p) {
-      const int A0 = val[pt.sel[p][0]] * OT, B0 = val[pt.sel[p][1]] * OT, C0 = val[pt.sel[p][2]] * OT;
-      const int a = A0 + al, b = B0 + be, c = C0 + ga;
-      double wv = 0.0, xv = 0.0;
-      if (a < v && b < v && c < v) {
-        const long long tile_w = ((long long)A0 * v + B0) * v + C0;
-        wv = A.W[tile_w + thr_w];
-        const int ab = A0 * v + B0 + t_ab, ac = A0 * v + C0 + t_ac, bc = B0 * v + C0 + t_bc;
-        double vv_ = ovs_ij[ab] * t1k[c] + ovs_ik[ac] * t1j[b] + ovs_jk[bc] * t1i[a];
-        if (A.has_fvo)
-          vv_ += t2s_ij[ab] * fvo[c * o + k] + t2s_ik[ac] * fvo[b * o + j] + t2s_jk[bc] * fvo[a * o + i];
-        xv = wv + 0.5 * vv_;
-        if constexpr (MODE == kOrbitProjVjp) {
-          double qv = 0.0;
-          for (int f = 0; f < A.nfrag; ++f) qv += A.u[f] * A.accA[f * v3 + tile_w + thr_w];
-          Qs[p][l_id] = A.ebar * qv;
-        }
-      } else if constexpr (MODE == kOrbitProjVjp) {
-        Qs[p][l_id] = 0.0;
+    // phase 1: W and X = W + V of the orbit's tiles, and the orbital energies
+    // of the orbit's three virtual tile ranges.  The W loads of every tile are
+    // issued before anything waits on them (slots past np repeat tile 0 and are
+    // not stored; indices past v are clamped and their values zeroed); the V
+    // terms then follow tile by tile.
+    if (tid < 3 * OT) evs[tid] = ev[min(val[tid >> 3] * OT + (tid & 7), v - 1)];
+    double wreg[6], xreg[6];
+    // clamped element coordinates of tile slot p (recomputed where used: they
+    // are a few integer operations, cheaper than keeping them in registers)
+    auto coords = [&](int p, int &ar, int &br, int &cr) {
+      const int ps = p < np ? p : 0;
+      const int a = val[pt.sel[ps][0]] * OT + al, b = val[pt.sel[ps][1]] * OT + be,
+                c = val[pt.sel[ps][2]] * OT + ga;
+      ar = min(a, v - 1);
+      br = min(b, v - 1);
+      cr = min(c, v - 1);
+      return a < v && b < v && c < v;
+    };
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+      int ar, br, cr;
+      coords(p, ar, br, cr);
+      wreg[p] = A.W[((long long)ar * v + br) * v + cr];
+    }
+    auto v_terms = [&](auto with_fvo) {
+#pragma unroll
+      for (int p = 0; p < 6; ++p) {
+        int ar, br, cr;
+        const bool in = coords(p, ar, br, cr);
+        const int ab = ar * v + br, ac = ar * v + cr, bc = br * v + cr;
+        double vv_ = ovs_ij[ab] * t1k[cr] + ovs_ik[ac] * t1j[br] + ovs_jk[bc] * t1i[ar];
+        if constexpr (decltype(with_fvo)::value)
+          vv_ += t2s_ij[ab] * fvo[cr * o + k] + t2s_ik[ac] * fvo[br * o + j] + t2s_jk[bc] * fvo[ar * o + i];
+        xreg[p] = in ? wreg[p] + 0.5 * vv_ : 0.0;
+        wreg[p] = in ? wreg[p] : 0.0;
       }
-      Ws[p][l_id] = wv;
-      Xs[p][l_id] = xv;
+    };
+    if (A.has_fvo)
+      v_terms(std::true_type{});
+    else
+      v_terms(std::false_type{});
+    if constexpr (MODE == kOrbitProjVjp) {
+      for (int p = 0; p < np; ++p) {
+        const int A0 = val[pt.sel[p][0]] * OT, B0 = val[pt.sel[p][1]] * OT, C0 = val[pt.sel[p][2]] * OT;
+        double qv = 0.0;
+        if (A0 + al < v && B0 + be < v && C0 + ga < v) {
+          const long long tile_w = ((long long)A0 * v + B0) * v + C0;
+          for (int f = 0; f < A.nfrag; ++f) qv += A.u[f] * A.accA[f * v3 + tile_w + thr_w];
+        }
+        Qs[p][l_id] = A.ebar * qv;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+      if (p < np) {
+        Ws[p][l_id] = wreg[p];
+        Xs[p][l_id] = xreg[p];
+      }
     }
     __syncthreads();
     // phase 2
@@ -637,7 +682,8 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
         const double x1 = Xs[p0][l_bca], x2 = Xs[p1][l_cab], x3 = Xs[p2][l_cba],
                      x4 = Xs[p3][l_acb], x5 = Xs[p4][l_bac];
         const double Z = 4.0 * x0 + x1 + x2 - 2.0 * (x3 + x4 + x5);
-        const double D = eo - ev[a] - ev[b] - ev[c];
+        const double D = eo - evs[pt.sel[p][0] * OT + al] - evs[pt.sel[p][1] * OT + be] -
+                         evs[pt.sel[p][2] * OT + ga];
         // 1/D: hardware reciprocal estimate refined by two Newton-Raphson steps
         // (D <= -(HOMO-LUMO gap) < 0 is never near 0 or a denormal)
         double rD = __builtin_amdgcn_rcp(D);
