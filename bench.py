@@ -87,8 +87,13 @@ def engine_roofline(prof, precision):
     return roofline
 
 
-TRAFFIC_FILES = {p: os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
-                                  f'traffic_e_t_vjp_o10v400_{p}.json') for p in ('bf16x3', 'fp64')}
+# rocprofv3 FETCH_SIZE / WRITE_SIZE summaries of the engine launches: the
+# split-BF16x3 one at the default nocc=40 (its tile shapes for the occupied-side
+# contractions depend on nocc), the FP64 one at nocc=10 (the FP64 engine runs the
+# same two tile shapes at every size)
+TRAFFIC_FILES = {p: os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles', f)
+                 for p, f in (('bf16x3', 'traffic_e_t_vjp_o40v400_bf16x3.json'),
+                              ('fp64', 'traffic_e_t_vjp_o10v400_fp64.json'))}
 
 
 def engine_traffic(precision):
