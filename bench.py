@@ -176,9 +176,10 @@ def main():
                          '(reference dispatch for HBM byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
-        print('bench.py: this build runs on one GPU only; the (T) path would shard over '
-              'occupied triples across GPUs (see DESIGN.md), which is not part of it. '
-              f'--gpus {args.gpus} is not supported.', file=sys.stderr)
+        print('bench.py: this benchmark runs on one GPU only; the (T) path would shard the '
+              'occupied-triple walk round-robin across GPUs (pamd_ccsdt_energy_shard computes '
+              f'one GPU\'s share), which is not run here. --gpus {args.gpus} is not supported.',
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -282,7 +283,7 @@ def main():
         so, sv = (int(x) for x in args.cpu_sample.split(','))
         try:
             cpu = cpu_baseline(o, v, (so, sv), args.cpu_threads)
-        except Exception as exc:  # the checker library may be absent on a host
+        except Exception as exc:  # the compiled reference library may be absent on a host
             cpu = {'value': None, 'unit': 's', 'cores': args.cpu_threads, 'kind': 'reference',
                    'sample': f'not run: {type(exc).__name__}: {exc}'}
         if cpu is not None and args.energy_only and cpu['value']:

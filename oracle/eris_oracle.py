@@ -1,6 +1,6 @@
 """CPU oracle for the MP2 pair energies and the DF (T) integrals -- TEST INFRASTRUCTURE.
 
-This module is the checker, never the product: only ``tests/``,
+This module is test infrastructure (the oracle), never the product: only ``tests/``,
 ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may import
 it.  The product path (``pyscfad_amd``) never calls it.
 

@@ -1,6 +1,6 @@
 """Spin-orbital (T) from DF factors -- TEST INFRASTRUCTURE (cross-check only).
 
-This module is the checker, never the product: only ``tests/`` may import it.
+This module is test infrastructure (the oracle), never the product: only ``tests/`` may import it.
 
 An independent statement of the (T) energy in the spin-orbital basis, from the
 same spatial DF factors the closed-shell path starts from, used to cross-check
