@@ -11,6 +11,17 @@ energy is exact to ~1e-10 relative but not smooth at that level, so its own
 difference quotient would measure rounding, not the derivative.)  Also
 reports the peak device memory of each VJP call against what storing W and V
 of every occupied triple would take.
+
+Error measures per precision, over the directions (fd = difference quotient,
+ana = <g, d> of that precision's VJP, rms = root mean square of the FP64 VJP's
+directional derivatives over the sampled directions):
+  rel_err        |fd - ana| / |fd|, relative to each direction's own derivative
+                 (large where that derivative happens to be small);
+  err_vs_rms     |fd - ana| / rms;
+  vs_fp64_rms    (precisions other than fp64) |ana - ana_fp64| / rms: the error
+                 against the FP64 VJP, free of the difference quotient's own
+                 truncation and rounding error.
+The per-direction values are kept in the report.
 """
 from __future__ import annotations
 
@@ -62,6 +73,7 @@ def run(nocc, nvir, naux, directions, h, precisions, seed=0, dirseed=1234, log=p
                        'fvo': g['fvo_bar']}
         del inp, g
     rel = {prec: [] for prec in precisions}
+    vals = {'fd': [], **{prec: [] for prec in precisions}}
     t0 = time.perf_counter()
     for q in range(directions):
         d = {kk: torch.randn(x[kk].shape, generator=gen, dtype=torch.float64, device=dev)
@@ -71,22 +83,48 @@ def run(nocc, nvir, naux, directions, h, precisions, seed=0, dirseed=1234, log=p
         ep = float(_ops.energy(_inputs({kk: x[kk] + h * d[kk] for kk in BLOCKS}, 'fp64', dev)).item())
         em = float(_ops.energy(_inputs({kk: x[kk] - h * d[kk] for kk in BLOCKS}, 'fp64', dev)).item())
         fd = (ep - em) / (2 * h)
+        vals['fd'].append(fd)
         msg = []
+        ana = {prec: float(sum((grads[prec][kk] * d[kk]).sum() for kk in BLOCKS))
+               for prec in precisions}
         for prec in precisions:
-            ana = float(sum((grads[prec][kk] * d[kk]).sum() for kk in BLOCKS))
-            rel[prec].append(abs(fd - ana) / max(abs(fd), 1e-300))
-            msg.append(f'{prec} {ana:.10e} (rel {rel[prec][-1]:.2e})')
+            vals[prec].append(ana[prec])
+            rel[prec].append(abs(fd - ana[prec]) / max(abs(fd), 1e-300))
+            msg.append(f'{prec} {ana[prec]:.10e} (rel {rel[prec][-1]:.2e})')
         if q % 8 == 0 or q == directions - 1:
             log(f'direction {q + 1}/{directions}: fd {fd:.10e}; ' + '; '.join(msg))
     t_fd = time.perf_counter() - t0
+    summ = summarise(vals, precisions)
     for prec in precisions:
         m = meta[prec]
         m.update({'rel_err_max': max(rel[prec]), 'rel_err_median': float(np.median(rel[prec])),
                   'fd_s': t_fd,
                   'memory_ratio': m['store_everything_bytes'] / max(m['peak_device_bytes_vjp'], 1)})
+        m.update(summ[prec])
         report['precisions'][prec] = m
         log(json.dumps({prec: m}))
+    report['directional_derivatives'] = vals
     return report
+
+
+def summarise(vals, precisions):
+    """Error measures from the per-direction values {'fd': [...], prec: [...]}:
+    err_vs_rms and, for precisions other than fp64 when fp64 was run,
+    vs_fp64_rms (max and median of each; definitions in the module docstring)."""
+    fd = np.asarray(vals['fd'], dtype=float)
+    ref = np.asarray(vals['fp64' if 'fp64' in vals else precisions[0]], dtype=float)
+    rms = float(np.sqrt(np.mean(ref * ref)))
+    out = {}
+    for prec in precisions:
+        a = np.asarray(vals[prec], dtype=float)
+        e = np.abs(fd - a) / rms
+        m = {'rms_derivative': rms, 'err_vs_rms_max': float(e.max()),
+             'err_vs_rms_median': float(np.median(e))}
+        if prec != 'fp64' and 'fp64' in vals:
+            e64 = np.abs(a - ref) / rms
+            m.update({'vs_fp64_rms_max': float(e64.max()), 'vs_fp64_rms_median': float(np.median(e64))})
+        out[prec] = m
+    return out
 
 
 def main(argv=None):
