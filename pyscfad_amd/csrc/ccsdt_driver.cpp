@@ -165,11 +165,19 @@ static int device_cus() {
 // split-BF16x3 16x16-MFMA tiles, one for its 32x32 tiles), rounded down so
 // that no workgroup waits for a second round.  (One round measured faster than two or three: fewer split-K
 // partials to write and reduce, same engine time.)
+// The split-BF16x3 two-workgroup tiles sum each output tile in FP32 over the
+// workgroup's whole K range, so the split count is also raised until no part
+// spans more than kB3MaxK terms: the FP32 rounding of those sums then stays
+// bounded as v grows (K = v^2 for the particle-side t2 adjoint).  At v = 400 one
+// round of workgroups already gives 25 parts of 6400 terms, so this bound only
+// acts at larger v.
+constexpr int kB3MaxK = 8192;
 static int splits_for(int M, int N, int chunks, int prec) {
   const bool b3 = prec == PAMD_PREC_BF16X3;
   const int tiles = std::max(1, b3 ? gemm_tiles_b3(M, N) : gemm_tiles(M, N));
   const int slots = (b3 ? gemm_wgs_per_cu_b3(M, N) : 2) * device_cus();
   int s = slots / tiles;
+  if (b3) s = std::max(s, (int)(((long long)chunks * KC + kB3MaxK - 1) / kB3MaxK));
   s = std::max(1, std::min(s, 512));
   return std::min(s, std::max(1, b3 ? chunks / 2 : chunks));
 }
