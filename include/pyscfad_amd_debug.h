@@ -22,6 +22,10 @@ int pamd_debug_gemm_descriptors(int nocc, int nvir, int i, int j, int k, int wit
 /* Split-K factors the adjoint uses for (particle, hole) partial buffers. */
 int pamd_debug_split_factors(int nocc, int nvir, int *S_p, int *S_h);
 
+/* The same for a given precision (PAMD_PREC_*): the split-BF16x3 engine sizes
+ * its split-K for its own tile shapes and bounds the K range of one part. */
+int pamd_debug_split_factors_prec(int nocc, int nvir, int precision, int *S_p, int *S_h);
+
 #ifdef __cplusplus
 }
 #endif

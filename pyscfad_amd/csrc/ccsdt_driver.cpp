@@ -26,6 +26,7 @@
 #include "ccsdt_common.h"
 #include "gemm_desc.h"
 #include "../../include/pyscfad_amd_ccsdt.h"
+#include "../../include/pyscfad_amd_debug.h"
 
 namespace pamd {
 
@@ -1058,8 +1059,14 @@ int pamd_profile_read(int kind, long long *launches, double *ms, double *flops, 
 }
 
 int pamd_debug_split_factors(int nocc, int nvir, int *S_p, int *S_h) {
+  return pamd_debug_split_factors_prec(nocc, nvir, PAMD_PREC_FP64, S_p, S_h);
+}
+
+int pamd_debug_split_factors_prec(int nocc, int nvir, int precision, int *S_p, int *S_h) {
   if (nocc < 1 || nvir < 1 || !S_p || !S_h) return PAMD_ERR_ARG;
-  Layout L = make_layout(nocc, nvir, 1, PAMD_PREC_FP64);
+  if (precision != PAMD_PREC_FP64 && precision != PAMD_PREC_BF16X3 && precision != PAMD_PREC_FP32X2)
+    return PAMD_ERR_ARG;
+  Layout L = make_layout(nocc, nvir, 1, precision);
   *S_p = L.S_p;
   *S_h = L.S_h;
   return PAMD_OK;

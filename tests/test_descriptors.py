@@ -85,3 +85,26 @@ def test_adjoint_descriptors(o, v, i, j, k):
     for pr, (x, y) in enumerate([(i, j), (i, k), (j, i), (j, k), (k, i), (k, j)]):
         oob[x, :, :, y] += ph[pr]
     assert np.abs(oob - g_ovoo).max() <= 1e-12 * max(1.0, np.abs(g_ovoo).max())
+
+
+def _split_factors_prec(o, v, prec):
+    import ctypes
+    from pyscfad_amd import _lib
+    lib = _lib.load()
+    lib.pamd_debug_split_factors_prec.argtypes = [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_int)] * 2
+    a, b = ctypes.c_int(), ctypes.c_int()
+    assert lib.pamd_debug_split_factors_prec(o, v, prec, ctypes.byref(a), ctypes.byref(b)) == 0
+    return a.value, b.value
+
+
+@pytest.mark.parametrize('o,v', [(40, 400), (60, 600), (40, 800), (24, 240), (8, 40)])
+def test_bf16x3_split_k_bounds_fp32_sum_length(o, v):
+    """Every split-K part of the split-BF16x3 adjoint (K = v^2 terms) spans at
+    most 8192 terms, so the FP32 tile sums of one part stay short at any size;
+    and at most half the number of K stages of 32."""
+    PREC_BF16X3 = 1
+    S_p, S_h = _split_factors_prec(o, v, PREC_BF16X3)
+    K = v * v
+    for S in (S_p, S_h):
+        assert 1 <= S <= max(1, K // 32)
+        assert -(-K // S) <= 8192 or S == max(1, K // 32)
