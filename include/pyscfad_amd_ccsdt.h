@@ -14,6 +14,9 @@
  *   fvo       (v, o)         fock[o:, :o]; may be NULL when cfg->has_fvo == 0
  * The stream argument is a hipStream_t (0 = the null stream).  Calls on one
  * workspace must not overlap.  Every entry point returns a PAMD_* status.
+ * Entry points keep no per-call state outside their arguments, so calls on
+ * different workspaces may run from different host threads; only the sampled
+ * timing switches of pyscfad_amd_profile.h are process-global.
  */
 #ifndef PYSCFAD_AMD_CCSDT_H
 #define PYSCFAD_AMD_CCSDT_H
@@ -94,6 +97,24 @@ int pamd_ccsdt_energy_shard(const pamd_ccsdt_config *cfg,
                             const double *fvo, int shard, int nshards,
                             void *workspace, size_t workspace_bytes,
                             double *e_out, void *stream);
+
+/* One shard of E(T) and of its VJP: the triples whose walk index t has
+ * t % nshards == shard (the walk and rounding order of
+ * pamd_ccsdt_energy_shard), with the outputs of pamd_ccsdt_energy_vjp for that
+ * share of the walk (overwritten; e_out is the shard's energy).  The shards'
+ * cotangents add up to pamd_ccsdt_energy_vjp's.  Plays the role of the
+ * reference's block-callable VJP, ccsd_t_energy_vjp(..., a0, a1, b0, b1, ...)
+ * (pyscfadlib/pyscfadlib/vjp/cc/ccsd_t_vjp.c:332-346), which its host driver
+ * calls once per virtual block (pyscfad/cc/ccsd_t.py:145-177); here the unit of
+ * work is the occupied triple.  Same workspace as pamd_ccsdt_energy_vjp. */
+int pamd_ccsdt_energy_vjp_shard(const pamd_ccsdt_config *cfg,
+                                const double *mo_energy, const double *t1, const double *t2,
+                                const double *ovvv, const double *ovoo, const double *ovov,
+                                const double *fvo, double et_bar, int shard, int nshards,
+                                void *workspace, size_t workspace_bytes,
+                                double *e_out, double *mo_energy_bar, double *t1_bar,
+                                double *t2_bar, double *ovvv_bar, double *ovoo_bar,
+                                double *ovov_bar, double *fvo_bar, void *stream);
 
 /* Occupied-pair resolved E(T): e_pair[i*o + j] = 1/3 sum_{k,abc} W Z / D,
  * sum(e_pair) = E(T).  Row sums are the occupied-weighted (T) of

@@ -24,7 +24,8 @@ PRECISIONS = {'fp64': 0, 'bf16x3': 1, 'fp32x2': 2}
 # exported symbols of include/pyscfad_amd_ccsdt.h, _profile.h, _debug.h, _mlp.h and _eris.h
 EXPORTS = ('pamd_version', 'pamd_ccsdt_workspace_size', 'pamd_ccsdt_energy',
            'pamd_ccsdt_energy_vjp', 'pamd_ccsdt_pair_energy', 'pamd_ccsdt_num_triples',
-           'pamd_ccsdt_energy_shard', 'pamd_ccsdt_weighted_energy_vjp',
+           'pamd_ccsdt_energy_shard', 'pamd_ccsdt_energy_vjp_shard',
+           'pamd_ccsdt_weighted_energy_vjp',
            'pamd_ccsdt_projected_workspace_size', 'pamd_ccsdt_projected_energy',
            'pamd_ccsdt_projected_energy_vjp', 'pamd_profile_enable',
            'pamd_profile_reset', 'pamd_profile_progress', 'pamd_profile_num_kinds', 'pamd_profile_kind_name',
@@ -69,6 +70,9 @@ def load():
     lib.pamd_ccsdt_pair_energy.argtypes = [cfgp] + [vp] * 7 + [vp, sz, vp, vp]
     lib.pamd_ccsdt_energy_vjp.restype = i
     lib.pamd_ccsdt_energy_vjp.argtypes = [cfgp] + [vp] * 7 + [d, vp, sz] + [vp] * 8 + [vp]
+    lib.pamd_ccsdt_energy_vjp_shard.restype = i
+    lib.pamd_ccsdt_energy_vjp_shard.argtypes = ([cfgp] + [vp] * 7 + [d, i, i, vp, sz] +
+                                                [vp] * 8 + [vp])
     lib.pamd_ccsdt_projected_workspace_size.restype = i
     lib.pamd_ccsdt_projected_workspace_size.argtypes = [cfgp, i, i, ctypes.POINTER(sz)]
     lib.pamd_ccsdt_projected_energy_vjp.restype = i

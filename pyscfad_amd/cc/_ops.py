@@ -124,15 +124,10 @@ def pair_energy(inp: Inputs):
     return e
 
 
-def energy_vjp(inp: Inputs, et_bar=1.0):
-    """(E(T), cotangents) with the cotangents in the reference's output layout."""
-    lib = _lib.load()
+def _cotangent_outputs(inp: Inputs):
     o, v = inp.nocc, inp.nvir
-    dev = inp.device
-    ws, nbytes = inp.workspace(True)
-    f64 = dict(dtype=torch.float64, device=dev)
-    e = torch.zeros((), **f64)
-    out = dict(
+    f64 = dict(dtype=torch.float64, device=inp.device)
+    return dict(
         mo_energy_bar=torch.empty(o + v, **f64),
         t1_bar=torch.empty((o, v), **f64),
         t2_bar=torch.empty((o, o, v, v), **f64),
@@ -141,12 +136,40 @@ def energy_vjp(inp: Inputs, et_bar=1.0):
         ovov_bar=torch.empty((o, v, o, v), **f64),
         fvo_bar=torch.empty((v, o), **f64),
     )
+
+
+def _cotangent_ptrs(out):
+    return [_ptr(out[k]) for k in ('mo_energy_bar', 't1_bar', 't2_bar', 'ovvv_bar', 'ovoo_bar',
+                                   'ovov_bar', 'fvo_bar')]
+
+
+def energy_vjp(inp: Inputs, et_bar=1.0, workspace=None):
+    """(E(T), cotangents) with the cotangents in the reference's output layout.
+    ``workspace``: an optional (tensor, nbytes) from ``inp.workspace(True)``,
+    reused across calls."""
+    lib = _lib.load()
+    ws, nbytes = workspace if workspace is not None else inp.workspace(True)
+    e = torch.zeros((), dtype=torch.float64, device=inp.device)
+    out = _cotangent_outputs(inp)
     st = lib.pamd_ccsdt_energy_vjp(
         ctypes.byref(inp.cfg), *inp.args(), ctypes.c_double(float(et_bar)), _ptr(ws), nbytes,
-        _ptr(e), _ptr(out['mo_energy_bar']), _ptr(out['t1_bar']), _ptr(out['t2_bar']),
-        _ptr(out['ovvv_bar']), _ptr(out['ovoo_bar']), _ptr(out['ovov_bar']),
-        _ptr(out['fvo_bar']), _stream(dev))
+        _ptr(e), *_cotangent_ptrs(out), _stream(inp.device))
     _lib.check(st, 'pamd_ccsdt_energy_vjp')
+    return e, out
+
+
+def energy_vjp_shard(inp: Inputs, shard, nshards, et_bar=1.0, workspace=None):
+    """(E, cotangents) of the round-robin shard ``shard`` of ``nshards`` over the
+    occupied triples; the shards' energies and cotangents add up to
+    ``energy_vjp``'s."""
+    lib = _lib.load()
+    ws, nbytes = workspace if workspace is not None else inp.workspace(True)
+    e = torch.zeros((), dtype=torch.float64, device=inp.device)
+    out = _cotangent_outputs(inp)
+    st = lib.pamd_ccsdt_energy_vjp_shard(
+        ctypes.byref(inp.cfg), *inp.args(), ctypes.c_double(float(et_bar)), int(shard),
+        int(nshards), _ptr(ws), nbytes, _ptr(e), *_cotangent_ptrs(out), _stream(inp.device))
+    _lib.check(st, 'pamd_ccsdt_energy_vjp_shard')
     return e, out
 
 

@@ -1,7 +1,8 @@
 """Adjoint check of the (T) kernels by central finite differences.
 
     python -m pyscfad_amd.cc.adjoint_check [--nocc 24 --nvir 240 --naux 960]
-        [--directions 64] [--h 1e-5] [--precision fp64 bf16x3] [--out report.json]
+        [--directions 64] [--h 1e-5] [--precision fp64 bf16x3] [--nshards 1]
+        [--out report.json]
 
 For each seeded unit direction d in the joint input space (mo_energy, t1, t2,
 packed ovvv, ovoo, ovov, fvo), compares the directional derivative from one
@@ -22,6 +23,11 @@ directional derivatives over the sampled directions):
                  against the FP64 VJP, free of the difference quotient's own
                  truncation and rounding error.
 The per-direction values are kept in the report.
+
+With ``nshards`` > 1 the energy and its VJP are those of round-robin shard 0
+of the occupied triples (pamd_ccsdt_energy_shard / _energy_vjp_shard): a
+function of the same inputs with its own exact gradient, so the same check
+runs at full input size for 1/nshards of the cost.
 """
 from __future__ import annotations
 
@@ -44,7 +50,15 @@ def _inputs(x, precision, dev):
                        x['fvo'], precision=precision, device=dev)
 
 
-def run(nocc, nvir, naux, directions, h, precisions, seed=0, dirseed=1234, log=print):
+def _energy(x, nshards, dev):
+    inp = _inputs(x, 'fp64', dev)
+    if nshards > 1:
+        return float(_ops.energy_shard(inp, 0, nshards).item())
+    return float(_ops.energy(inp).item())
+
+
+def run(nocc, nvir, naux, directions, h, precisions, seed=0, dirseed=1234, log=print,
+        nshards=1):
     dev = torch.device('cuda', 0)
     p = make_problem(nocc, nvir, naux, seed=seed, fvo_scale=1e-3, device=dev)
     iu, ju = torch.tril_indices(nvir, nvir, device=dev)
@@ -53,7 +67,8 @@ def run(nocc, nvir, naux, directions, h, precisions, seed=0, dirseed=1234, log=p
     del p
     gen = torch.Generator(device=dev).manual_seed(dirseed)
     report = {'workload': f'o{nocc}/v{nvir}/naux{naux}', 'h': h, 'directions': directions,
-              'fd': 'central difference of the FP64 energy', 'precisions': {}}
+              'fd': 'central difference of the FP64 energy', 'nshards': nshards,
+              'precisions': {}}
     grads, meta = {}, {}
     o, v = nocc, nvir
     ntrip = o * (o + 1) * (o + 2) // 6 - o
@@ -63,7 +78,10 @@ def run(nocc, nvir, naux, directions, h, precisions, seed=0, dirseed=1234, log=p
         base = torch.cuda.memory_allocated(dev)
         inp = _inputs(x, prec, dev)
         t0 = time.perf_counter()
-        e0, g = _ops.energy_vjp(inp, et_bar=1.0)
+        if nshards > 1:
+            e0, g = _ops.energy_vjp_shard(inp, 0, nshards, et_bar=1.0)
+        else:
+            e0, g = _ops.energy_vjp(inp, et_bar=1.0)
         torch.cuda.synchronize()
         meta[prec] = {'e_t': float(e0.item()), 'vjp_s': time.perf_counter() - t0,
                       'peak_device_bytes_vjp': int(torch.cuda.max_memory_allocated(dev) - base),
@@ -80,8 +98,8 @@ def run(nocc, nvir, naux, directions, h, precisions, seed=0, dirseed=1234, log=p
              for kk in BLOCKS}
         nrm = torch.sqrt(sum((t * t).sum() for t in d.values()))
         d = {kk: t / nrm for kk, t in d.items()}
-        ep = float(_ops.energy(_inputs({kk: x[kk] + h * d[kk] for kk in BLOCKS}, 'fp64', dev)).item())
-        em = float(_ops.energy(_inputs({kk: x[kk] - h * d[kk] for kk in BLOCKS}, 'fp64', dev)).item())
+        ep = _energy({kk: x[kk] + h * d[kk] for kk in BLOCKS}, nshards, dev)
+        em = _energy({kk: x[kk] - h * d[kk] for kk in BLOCKS}, nshards, dev)
         fd = (ep - em) / (2 * h)
         vals['fd'].append(fd)
         msg = []
@@ -135,13 +153,14 @@ def main(argv=None):
     ap.add_argument('--directions', type=int, default=64)
     ap.add_argument('--h', type=float, default=1e-5)
     ap.add_argument('--precision', nargs='+', default=['fp64', 'bf16x3'])
+    ap.add_argument('--nshards', type=int, default=1)
     ap.add_argument('--out', default=None)
     args = ap.parse_args(argv)
     if not torch.cuda.is_available():
         print('adjoint_check: no GPU visible', file=sys.stderr)
         return 1
     rep = run(args.nocc, args.nvir, args.naux, args.directions, args.h, args.precision,
-              log=lambda s: print(s, flush=True))
+              log=lambda s: print(s, flush=True), nshards=args.nshards)
     if args.out:
         with open(args.out, 'w') as f:
             json.dump(rep, f, indent=1)

@@ -87,11 +87,15 @@ struct OrbitArgs {
 enum OrbitMode { kOrbitPlain = 0, kOrbitProjAcc = 1, kOrbitProjVjp = 2 };
 
 // V-bar reductions: the sums over one virtual index are split into kVRS parts
-// (more workgroups in flight); the scratch R holds every part.
+// (more workgroups in flight), and the last virtual index is processed in
+// column blocks of kVRC; the scratch R holds every part.
 constexpr int kVRS = 4;
+constexpr int kVRC = 512;
+__host__ __device__ inline int vr_col_blocks(int v) { return (v + kVRC - 1) / kVRC; }
 inline size_t vr_scratch_doubles(int v) {
-  const size_t vv = (size_t)v * v;
-  return 10 * vv + 2 * (size_t)v + (size_t)(kVRS - 1) * (6 * vv + 2 * (size_t)v);
+  const size_t vv = (size_t)v * v, ncb = (size_t)vr_col_blocks(v);
+  return 10 * vv + 2 * (size_t)v + (size_t)(kVRS - 1) * 6 * vv +
+         (kVRS * ncb - 1) * 2 * (size_t)v + (ncb - 1) * 4 * vv;
 }
 
 // V-bar reductions for one triple.

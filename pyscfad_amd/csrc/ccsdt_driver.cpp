@@ -127,16 +127,15 @@ static hipError_t timed(hipStream_t s, int kind, double flops, double bytes, F &
   return prof_end(s, r);
 }
 
-static int g_prec = PAMD_PREC_FP64;  // precision of the GEMMs of the call in progress
-
-static hipError_t run_gemm(const GemmDesc &d, hipStream_t s, int kind) {
+// one contraction in the precision of the call that issues it
+static hipError_t run_gemm(const GemmDesc &d, hipStream_t s, int prec, int kind) {
   if (g_record) {
     g_record->push_back(d);
     return hipSuccess;
   }
-  if (g_prec == PAMD_PREC_BF16X3)
+  if (prec == PAMD_PREC_BF16X3)
     return timed(s, kind, gemm_flops(d), 0.0, [&] { return launch_gemm_bf16x3(d, s); });
-  if (g_prec == PAMD_PREC_FP32X2)
+  if (prec == PAMD_PREC_FP32X2)
     return timed(s, kind, gemm_flops(d), 0.0, [&] { return launch_gemm_f32x2(d, s); });
   return timed(s, kind, gemm_flops(d), 0.0, [&] { return launch_gemm_f64(d, s); });
 }
@@ -275,7 +274,7 @@ static int build_w(const Ctx &c, int i, int j, int k) {
                        seg(opR(T2m(j), o, vo, V, 1, 0), op1(OOt(k, i), o, 1, 0), O, -1.0),   // H5
                        seg(opR(T2m(i), vo, o, V, 1, 0), op1(OOt(k, j), o, 1, 0), O, -1.0)},  // H6
                       out(c.W, v2, v, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, kProfWBuild));
+    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfWBuild));
   }
   // pass B: rows b, cols (a,c)
   {
@@ -285,7 +284,7 @@ static int build_w(const Ctx &c, int i, int j, int k) {
                        seg(op1(OOt(j, i), o, 1, 0), opR(T2m(k), o, vo, V, 1, 0), O, -1.0),   // H3
                        seg(op1(OOt(j, k), o, 1, 0), opR(T2m(i), vo, o, V, 1, 0), O, -1.0)},  // H4
                       out(c.W, 0, v, kBig, v2, 1, V));
-    PAMD_CHECK(run_gemm(d, c.s, kProfWBuild));
+    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfWBuild));
   }
   // pass A: rows a, cols (b,c)
   {
@@ -295,7 +294,7 @@ static int build_w(const Ctx &c, int i, int j, int k) {
                        seg(op1(OOt(i, j), o, 1, 0), opR(T2m(k), o, vo, V, 1, 0), O, -1.0),   // H1
                        seg(op1(OOt(i, k), o, 1, 0), opR(T2m(j), vo, o, V, 1, 0), O, -1.0)},  // H2
                       out(c.W, 0, v2, kBig, v, 1, V));
-    PAMD_CHECK(run_gemm(d, c.s, kProfWBuild));
+    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfWBuild));
   }
   return PAMD_OK;
 }
@@ -340,21 +339,21 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
                       {seg(opR(Wb, v2, v, V, 1, 0), op1(T2(k, j), 1, v, 1), V, 1.0),
                        seg(opR(Wb, v2, 1, V, v, 1), op1(T2(j, k), 1, v, 1), V, 1.0)},
                       out(Vpb(i), v2, v, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, kProfAdjOvvv));
+    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfAdjOvvv));
   }
   {  // Vpb[j][b][a|c][f]: P3 (K=c), P4 (K=a)
     GemmDesc d = gemm(V * V, V, kAccumulate,
                       {seg(opR(Wb, v, v2, V, 1, 0), op1(T2(k, i), 1, v, 1), V, 1.0),
                        seg(opR(Wb, v, 1, V, v2, 1), op1(T2(i, k), 1, v, 1), V, 1.0)},
                       out(Vpb(j), v2, v, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, kProfAdjOvvv));
+    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfAdjOvvv));
   }
   {  // Vpb[k][c][a|b][f]: rows m = y*v + x with x = c: P5 (K=b), P6 (K=a)
     GemmDesc d = gemm(V * V, V, kAccumulate,
                       {seg(opR(Wb, v2, 1, V, v, 1), op1(T2(j, i), 1, v, 1), V, 1.0),
                        seg(opR(Wb, v, 1, V, v2, 1), op1(T2(i, j), 1, v, 1), V, 1.0)},
                       out(Vpb(k), v, v2, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, kProfAdjOvvv));
+    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfAdjOvvv));
   }
 
   // particle t2_bar pieces: split-K over the virtual pair, partials per problem
@@ -380,7 +379,7 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
         gemm(V, V, kPartial, {seg(opK(Wb, v2, v, 1, V, 0), opK(Vp(k), 1, v, v2, V, 1), V * V, 1.0)},
              out(nullptr, 0, 0, 1, 0, 0, 1), c.S_p, c.part_p + 5 * pp),
     };
-    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(ds[q], c.s, kProfAdjT2P));
+    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(ds[q], c.s, c.prec, kProfAdjT2P));
   }
   // hole ovoo_bar pieces: split-K over the virtual pair
   const size_t ph = (size_t)c.S_h * v * o;
@@ -405,7 +404,7 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
         gemm(V, O, kPartial, {seg(opK(Wb, 1, v2, v, V, 1), opK(T2m(i), 1, vo, o, V, 1), V * V, -1.0)},
              out(nullptr, 0, 0, 1, 0, 0, 1), c.S_h, c.part_h + 5 * ph),
     };
-    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(ds[q], c.s, kProfAdjOvoo));
+    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(ds[q], c.s, c.prec, kProfAdjOvoo));
   }
   // hole-term t2 reads: T2mb[x][p][q][m] (= dE/d t2[x,m,p,q])
   {  // T2mb[i]: H4 (p=a,q=c,K=b), H6 (p=a,q=b,K=c)
@@ -413,21 +412,21 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
                       {seg(opR(Wb, v2, 1, V, v, 1), op1(OOt(j, k), 1, o, 1), V, -1.0),
                        seg(opR(Wb, v2, v, V, 1, 0), op1(OOt(k, j), 1, o, 1), V, -1.0)},
                       out(T2mb(i), vo, o, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, kProfAdjT2H));
+    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfAdjT2H));
   }
   {  // T2mb[j]: H2 (p=b,q=c,K=a), H5 (p=b,q=a,K=c)
     GemmDesc d = gemm(V * V, O, kAccumulate,
                       {seg(opR(Wb, v, 1, V, v2, 1), op1(OOt(i, k), 1, o, 1), V, -1.0),
                        seg(opR(Wb, v, v2, V, 1, 0), op1(OOt(k, i), 1, o, 1), V, -1.0)},
                       out(T2mb(j), vo, o, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, kProfAdjT2H));
+    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfAdjT2H));
   }
   {  // T2mb[k]: rows m = q*v + p with p = c: H1 (q=b,K=a), H3 (q=a,K=b)
     GemmDesc d = gemm(V * V, O, kAccumulate,
                       {seg(opR(Wb, v, 1, V, v2, 1), op1(OOt(i, j), 1, o, 1), V, -1.0),
                        seg(opR(Wb, v2, 1, V, v, 1), op1(OOt(j, i), 1, o, 1), V, -1.0)},
                       out(T2mb(k), o, vo, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, kProfAdjT2H));
+    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfAdjT2H));
   }
   // gather into the cotangents
   CombineArgs ca;
@@ -478,7 +477,6 @@ static int zero_vjp(const Ctx &c) {
 // Layout changes at entry, shared by every walk; returns the orbit count.
 static int setup(Ctx &c, long long &norb) {
   const int o = c.o, v = c.v;
-  g_prec = c.prec;
   const int fmt = split_format(c.prec);
   PAMD_CHECK(launch_prepare(c.ovvv, c.packed, c.t2, c.ovoo, o, v, c.Vp, c.T2m, c.OOt, fmt, c.s));
   PAMD_CHECK(launch_prep_sym(c.ovov, o, v, 0, c.OVS, c.s));
@@ -675,7 +673,10 @@ static int check_common(const pamd_ccsdt_config *cfg, const double *mo, const do
   if (cfg->precision != PAMD_PREC_FP64 && cfg->precision != PAMD_PREC_BF16X3 &&
       cfg->precision != PAMD_PREC_FP32X2)
     return PAMD_ERR_UNSUPPORTED;
-  if (cfg->nvir > 4096 || cfg->nocc > 1024) return PAMD_ERR_ARG;
+  if (cfg->nocc > 1024) return PAMD_ERR_ARG;
+  // the engines address an operand within one occupied slab with 32-bit
+  // element offsets, and the largest slab is v^3 (W, Wbar, Vp[x])
+  if ((long long)cfg->nvir * cfg->nvir * cfg->nvir >= (1LL << 31)) return PAMD_ERR_UNSUPPORTED;
   return PAMD_OK;
 }
 
@@ -806,13 +807,13 @@ static int vjp_impl(const pamd_ccsdt_config *cfg, const double *mo_energy, const
                     const double *fvo, const double *wt_host, double et_bar, void *workspace,
                     size_t workspace_bytes, double *e_out, double *mo_energy_bar, double *t1_bar,
                     double *t2_bar, double *ovvv_bar, double *ovoo_bar, double *ovov_bar,
-                    double *fvo_bar, double *wt_bar, void *stream) {
+                    double *fvo_bar, double *wt_bar, void *stream, int shard = 0,
+                    int nshards = 1) {
   int st = check_common(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo);
   if (st != PAMD_OK) return st;
   if (!e_out || !mo_energy_bar || !t1_bar || !t2_bar || !ovvv_bar || !ovoo_bar || !ovov_bar ||
       !fvo_bar)
     return PAMD_ERR_ARG;
-  if (cfg->nvir > 512) return PAMD_ERR_UNSUPPORTED;
   Layout L = make_layout(cfg->nocc, cfg->nvir, 1, cfg->precision);
   if (!workspace || workspace_bytes < L.total) return PAMD_ERR_WORKSPACE;
   Ctx c;
@@ -838,6 +839,8 @@ static int vjp_impl(const pamd_ccsdt_config *cfg, const double *mo_energy, const
   c.ovoo_bar = ovoo_bar;
   c.ovov_bar = ovov_bar;
   c.fvo_bar = fvo_bar;
+  c.shard = shard;
+  c.nshards = nshards;
   bind_workspace(c, L, workspace);
   st = run(c);
   if (st != PAMD_OK) return st;
@@ -864,6 +867,20 @@ int pamd_ccsdt_energy_vjp(const pamd_ccsdt_config *cfg, const double *mo_energy,
   return vjp_impl(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, nullptr, et_bar, workspace,
                   workspace_bytes, e_out, mo_energy_bar, t1_bar, t2_bar, ovvv_bar, ovoo_bar,
                   ovov_bar, fvo_bar, nullptr, stream);
+}
+
+int pamd_ccsdt_energy_vjp_shard(const pamd_ccsdt_config *cfg, const double *mo_energy,
+                                const double *t1, const double *t2, const double *ovvv,
+                                const double *ovoo, const double *ovov, const double *fvo,
+                                double et_bar, int shard, int nshards, void *workspace,
+                                size_t workspace_bytes, double *e_out, double *mo_energy_bar,
+                                double *t1_bar, double *t2_bar, double *ovvv_bar,
+                                double *ovoo_bar, double *ovov_bar, double *fvo_bar,
+                                void *stream) {
+  if (nshards < 1 || shard < 0 || shard >= nshards) return PAMD_ERR_ARG;
+  return vjp_impl(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, nullptr, et_bar, workspace,
+                  workspace_bytes, e_out, mo_energy_bar, t1_bar, t2_bar, ovvv_bar, ovoo_bar,
+                  ovov_bar, fvo_bar, nullptr, stream, shard, nshards);
 }
 
 int pamd_ccsdt_weighted_energy_vjp(const pamd_ccsdt_config *cfg, const double *mo_energy,
@@ -922,7 +939,6 @@ static int projected_impl(const pamd_ccsdt_config *cfg, const double *mo_energy,
   if (vjp && (!mo_energy_bar || !t1_bar || !t2_bar || !ovvv_bar || !ovoo_bar || !ovov_bar ||
               !fvo_bar || !ulo_bar))
     return PAMD_ERR_ARG;
-  if (vjp && cfg->nvir > 512) return PAMD_ERR_UNSUPPORTED;
   const ProjLayout P = projected_layout(cfg->nocc, cfg->nvir, nfrag, vjp, cfg->precision);
   if (!workspace || workspace_bytes < P.total) return PAMD_ERR_WORKSPACE;
   Ctx c;

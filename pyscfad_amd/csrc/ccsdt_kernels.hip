@@ -819,24 +819,28 @@ hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s, int mode) {
 //  10*v^2 + v + a : fvoi[a] = sum_bc Vb (t2[k,j,c,b] + t2[j,k,b,c])
 // Blocks 2-7 and the two vectors are sums over b (k_vr_a) or a (k_vr_b); the
 // kVRS parts of those sums are stored separately (parts 1.. after the slots
-// above: vr_scratch_doubles) and added in part order by the consumers.
+// above: vr_scratch_doubles) and added in part order by the consumers.  The
+// virtual index c is processed in column blocks of kVRC (grid z), so blocks 0,
+// 1, 8, 9 (sums over c) and the two vectors also have one part per column
+// block; every part is written by exactly one workgroup.
 // ---------------------------------------------------------------------------
-constexpr int kMaxVQ = 8;  // v <= 512 for the adjoint
+constexpr int kMaxVQ = kVRC / 64;  // per-lane columns of one column block
 
-// write the 4-wave sum of a per-lane array (element c = lane + 64 qq) to dst[c]
 constexpr int kVRW = 8;  // waves per V-bar reduction workgroup
 
-// sum over the kVRW waves of per-lane arrays indexed by c = lane + 64 qq (fixed order)
+// sum over the kVRW waves of per-lane arrays indexed by the block column
+// c = lane + 64 qq (c < vl, the width of this column block), written to
+// dst[c] in a fixed order
 #define PAMD_WAVE_COMBINE(arr, dst)                                          \
   do {                                                                       \
     _Pragma("unroll") for (int qq = 0; qq < kMaxVQ; ++qq) {                  \
       const int c_ = lane + 64 * qq;                                         \
-      if (qq < nq && c_ < v) red[wave * 512 + c_] = (arr)[qq];               \
+      if (c_ < vl) red[wave * kVRC + c_] = (arr)[qq];                        \
     }                                                                        \
     __syncthreads();                                                         \
-    for (int c_ = tid; c_ < v; c_ += 64 * kVRW) {                            \
+    for (int c_ = tid; c_ < vl; c_ += 64 * kVRW) {                           \
       double s_ = 0.0;                                                       \
-      for (int w_ = 0; w_ < kVRW; ++w_) s_ += red[w_ * 512 + c_];            \
+      for (int w_ = 0; w_ < kVRW; ++w_) s_ += red[w_ * kVRC + c_];           \
       (dst)[c_] = s_;                                                        \
     }                                                                        \
     __syncthreads();                                                         \
@@ -878,7 +882,9 @@ __device__ double block_sum_vr(double x, double *scratch) {
 
 // Part s of kVRS of the b-sums (k_vr_a) and a-sums (k_vr_b): arrays that are
 // sums over the split index are written per part; part 0 uses the slots of R
-// listed above, parts 1.. an extension after them (vr_part, vr_part_vec).
+// listed above, parts 1.. an extension after them (vr_part, vr_part_vec).  The
+// vectors have kVRS x ncb parts (index cb * kVRS + s); blocks 0, 1, 8, 9 have
+// ncb parts (vr_cpart), column block cb >= 1 in a second extension.
 __device__ __forceinline__ double *vr_part(double *R, int v, int s, int q) {
   const size_t vv = (size_t)v * v;
   if (s == 0) return R + (size_t)q * vv;
@@ -888,6 +894,13 @@ __device__ __forceinline__ double *vr_part_vec(double *R, int v, int s, int whic
   const size_t vv = (size_t)v * v;
   if (s == 0) return R + 10 * vv + (size_t)which * v;
   return R + 10 * vv + 2 * (size_t)v + (size_t)(kVRS - 1) * 6 * vv + ((size_t)(s - 1) * 2 + which) * v;
+}
+__device__ __forceinline__ double *vr_cpart(double *R, int v, int cb, int q) {
+  const size_t vv = (size_t)v * v;
+  if (cb == 0) return R + (size_t)q * vv;
+  const int slot = q < 2 ? q : q - 6;  // 0, 1, 8, 9 -> 0, 1, 2, 3
+  return R + 10 * vv + 2 * (size_t)v + (size_t)(kVRS - 1) * 6 * vv +
+         (size_t)(kVRS * vr_col_blocks(v) - 1) * 2 * v + ((size_t)(cb - 1) * 4 + slot) * vv;
 }
 
 // sums over the parts, in part order
@@ -899,28 +912,36 @@ __device__ __forceinline__ double vr_sum(const double *R, int v, int q, size_t i
 }
 __device__ __forceinline__ double vr_sum_vec(const double *R, int v, int which, int idx) {
   double x = 0.0;
-#pragma unroll
-  for (int s = 0; s < kVRS; ++s) x += vr_part_vec(const_cast<double *>(R), v, s, which)[idx];
+  const int np = kVRS * vr_col_blocks(v);
+  for (int s = 0; s < np; ++s) x += vr_part_vec(const_cast<double *>(R), v, s, which)[idx];
+  return x;
+}
+__device__ __forceinline__ double vr_csum(const double *R, int v, int q, size_t idx) {
+  double x = 0.0;
+  const int ncb = vr_col_blocks(v);
+  for (int cb = 0; cb < ncb; ++cb) x += vr_cpart(const_cast<double *>(R), v, cb, q)[idx];
   return x;
 }
 
 template <bool FVO>
 __global__ __launch_bounds__(64 * kVRW) void k_vr_a(VRArgs A) {
-  __shared__ double red[kVRW * 512];
-  __shared__ double cst[4][512];  // per-c constants: t1[k,c], fvo[c,k], S2[a,c], T2[a,c]
+  __shared__ double red[kVRW * kVRC];
+  __shared__ double cst[4][kVRC];  // per-c constants: t1[k,c], fvo[c,k], S2[a,c], T2[a,c]
   __shared__ double sred[kVRW];
-  const int a = blockIdx.x, part = blockIdx.y;
+  const int a = blockIdx.x, part = blockIdx.y, cb = blockIdx.z;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int o = A.o, v = A.v, i = A.i, j = A.j, k = A.k;
   const size_t vv = (size_t)v * v;
   const double *ovs = A.ovs, *t2s = A.t2s, *t1 = A.t1, *fvo = A.fvo;
 #define SYM(arr, x, y, p, q) arr[(((size_t)(x) * o + (y)) * v + (p)) * v + (q)]
-  const int nq = (v + 63) / 64;
-  for (int c = tid; c < v; c += 64 * kVRW) {
-    cst[0][c] = t1[k * v + c];
-    cst[2][c] = SYM(ovs, i, k, a, c);
-    cst[1][c] = FVO ? fvo[c * o + k] : 0.0;
-    cst[3][c] = SYM(t2s, i, k, a, c);
+  // this workgroup's columns: c = c0 + cl, cl < vl
+  const int c0 = cb * kVRC, vl = min(kVRC, v - c0);
+  for (int cl = tid; cl < vl; cl += 64 * kVRW) {
+    const int c = c0 + cl;
+    cst[0][cl] = t1[k * v + c];
+    cst[2][cl] = SYM(ovs, i, k, a, c);
+    cst[1][cl] = FVO ? fvo[c * o + k] : 0.0;
+    cst[3][cl] = SYM(t2s, i, k, a, c);
   }
   __syncthreads();
   double r3[kMaxVQ], r4[kMaxVQ], pk[kMaxVQ], fk[kMaxVQ];
@@ -933,14 +954,14 @@ __global__ __launch_bounds__(64 * kVRW) void k_vr_a(VRArgs A) {
     const double S1 = SYM(ovs, i, j, a, b);
     const double fvj = FVO ? fvo[b * o + j] : 0.0;
     const double T1 = SYM(t2s, i, j, a, b);
-    const double *sjk = ovs + (((size_t)j * o + k) * v + b) * v;
-    const double *tjk = t2s + (((size_t)j * o + k) * v + b) * v;
-    const double *row = A.Vb + (size_t)a * vv + (size_t)b * v;
+    const double *sjk = ovs + (((size_t)j * o + k) * v + b) * v + c0;
+    const double *tjk = t2s + (((size_t)j * o + k) * v + b) * v + c0;
+    const double *row = A.Vb + (size_t)a * vv + (size_t)b * v + c0;
     double sr1 = 0.0, sr2 = 0.0, spj = 0.0, sfj = 0.0;
 #pragma unroll
     for (int qq = 0; qq < kMaxVQ; ++qq) {
       const int c = lane + 64 * qq;
-      if (qq < nq && c < v) {
+      if (c < vl) {
         const double x = row[c];
         sr1 += x * cst[0][c];
         spj += x * cst[2][c];
@@ -961,56 +982,55 @@ __global__ __launch_bounds__(64 * kVRW) void k_vr_a(VRArgs A) {
     const double s4 = wave_sum4(sr1, sr2, spj, sfj);
     if ((lane & 15) == 0) {
       const int q = lane >> 4;
-      if (FVO || q != 1) R[(size_t)(q < 2 ? q : q + 6) * vv + (size_t)a * v + b] = s4;
+      if (FVO || q != 1) vr_cpart(R, v, cb, q < 2 ? q : q + 6)[(size_t)a * v + b] = s4;
     }
   }
   __syncthreads();
-  PAMD_WAVE_COMBINE(r3, vr_part(R, v, part, 2) + (size_t)a * v);
-  if (FVO) PAMD_WAVE_COMBINE(r4, vr_part(R, v, part, 3) + (size_t)a * v);
-  PAMD_WAVE_COMBINE(pk, vr_part(R, v, part, 6) + (size_t)a * v);
-  PAMD_WAVE_COMBINE(fk, vr_part(R, v, part, 7) + (size_t)a * v);
+  PAMD_WAVE_COMBINE(r3, vr_part(R, v, part, 2) + (size_t)a * v + c0);
+  if (FVO) PAMD_WAVE_COMBINE(r4, vr_part(R, v, part, 3) + (size_t)a * v + c0);
+  PAMD_WAVE_COMBINE(pk, vr_part(R, v, part, 6) + (size_t)a * v + c0);
+  PAMD_WAVE_COMBINE(fk, vr_part(R, v, part, 7) + (size_t)a * v + c0);
   const double t1i_s = block_sum_vr(t1i, sred);
   const double fvoi_s = block_sum_vr(fvoi, sred);
   if (tid == 0) {
-    vr_part_vec(R, v, part, 0)[a] = t1i_s;
-    vr_part_vec(R, v, part, 1)[a] = fvoi_s;
+    vr_part_vec(R, v, cb * kVRS + part, 0)[a] = t1i_s;
+    vr_part_vec(R, v, cb * kVRS + part, 1)[a] = fvoi_s;
   }
 #undef SYM
 }
 
 template <bool FVO>
 __global__ __launch_bounds__(64 * kVRW) void k_vr_b(VRArgs A) {
-  __shared__ double red[kVRW * 512];
-  const int b = blockIdx.x, part = blockIdx.y;
+  __shared__ double red[kVRW * kVRC];
+  const int b = blockIdx.x, part = blockIdx.y, cb = blockIdx.z;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int o = A.o, v = A.v, i = A.i;
   const size_t vv = (size_t)v * v;
-  const int nq = (v + 63) / 64;
+  const int c0 = cb * kVRC, vl = min(kVRC, v - c0);
   double r5[kMaxVQ], r6[kMaxVQ];
 #pragma unroll
   for (int qq = 0; qq < kMaxVQ; ++qq) r5[qq] = r6[qq] = 0.0;
   for (int a = wave + kVRW * part; a < v; a += kVRW * kVRS) {
     const double t1a = A.t1[i * v + a];
     const double fa = FVO ? A.fvo[a * o + i] : 0.0;
-    const double *row = A.Vb + (size_t)a * vv + (size_t)b * v;
+    const double *row = A.Vb + (size_t)a * vv + (size_t)b * v + c0;
 #pragma unroll
     for (int qq = 0; qq < kMaxVQ; ++qq) {
       const int c = lane + 64 * qq;
-      if (qq < nq && c < v) {
+      if (c < vl) {
         const double x = row[c];
         r5[qq] += x * t1a;
         if (FVO) r6[qq] += x * fa;
       }
     }
   }
-  PAMD_WAVE_COMBINE(r5, vr_part(A.R, v, part, 4) + (size_t)b * v);
-  if (FVO) PAMD_WAVE_COMBINE(r6, vr_part(A.R, v, part, 5) + (size_t)b * v);
+  PAMD_WAVE_COMBINE(r5, vr_part(A.R, v, part, 4) + (size_t)b * v + c0);
+  if (FVO) PAMD_WAVE_COMBINE(r6, vr_part(A.R, v, part, 5) + (size_t)b * v + c0);
 }
 #undef PAMD_WAVE_COMBINE
 
 hipError_t launch_vr(const VRArgs &a, hipStream_t s) {
-  if (a.v > 64 * kMaxVQ) return hipErrorInvalidValue;
-  const dim3 grid(a.v, kVRS), block(64 * kVRW);
+  const dim3 grid(a.v, kVRS, vr_col_blocks(a.v)), block(64 * kVRW);
   if (a.has_fvo) {
     hipLaunchKernelGGL(k_vr_a<true>, grid, block, 0, s, a);
     hipLaunchKernelGGL(k_vr_b<true>, grid, block, 0, s, a);
@@ -1038,8 +1058,8 @@ __global__ void k_combine(CombineArgs A) {
     auto ob = [&](int x, int y) -> double & {
       return A.ovov_bar[(((size_t)x * v + p) * o + y) * v + q];
     };
-    ob(i, j) += 0.5 * R[0 * vv + pq];
-    ob(j, i) += 0.5 * R[0 * vv + qp];
+    ob(i, j) += 0.5 * vr_csum(R, v, 0, pq);
+    ob(j, i) += 0.5 * vr_csum(R, v, 0, qp);
     ob(i, k) += 0.5 * vr_sum(R, v, 2, pq);
     ob(k, i) += 0.5 * vr_sum(R, v, 2, qp);
     ob(j, k) += 0.5 * vr_sum(R, v, 4, pq);
@@ -1058,8 +1078,8 @@ __global__ void k_combine(CombineArgs A) {
           case 1: s += 0.5 * vr_sum(R, v, 5, pq); break;  // t2_bar[j,k,b,c] += R6[b,c]/2
           case 2: s += 0.5 * vr_sum(R, v, 3, qp); break;  // t2_bar[k,i,c,a] += R4[a,c]/2
           case 3: s += 0.5 * vr_sum(R, v, 3, pq); break;  // t2_bar[i,k,a,c] += R4[a,c]/2
-          case 4: s += 0.5 * R[1 * vv + qp]; break;  // t2_bar[j,i,b,a] += R2[a,b]/2
-          case 5: s += 0.5 * R[1 * vv + pq]; break;  // t2_bar[i,j,a,b] += R2[a,b]/2
+          case 4: s += 0.5 * vr_csum(R, v, 1, qp); break;  // t2_bar[j,i,b,a] += R2[a,b]/2
+          case 5: s += 0.5 * vr_csum(R, v, 1, pq); break;  // t2_bar[i,j,a,b] += R2[a,b]/2
         }
       }
       tb(xs[pr], ys[pr]) += s;
@@ -1092,9 +1112,9 @@ __global__ __launch_bounds__(256) void k_vr_colsums(CombineArgs A) {
     for (int a = wave; a < v; a += 4) {
       const size_t ac = (size_t)a * v + c;
       sk += vr_sum(R, v, 6, ac);
-      sj += R[8 * vv + ac];
+      sj += vr_csum(R, v, 8, ac);
       fk += vr_sum(R, v, 7, ac);
-      fj += R[9 * vv + ac];
+      fj += vr_csum(R, v, 9, ac);
     }
   }
   red[0][wave][lane] = sk;

@@ -32,15 +32,54 @@ CASES = (
 )
 
 
+# Larger cases: too big for seeded Gaussian projection directions, so each
+# block is projected on the closed-form direction sin(0.7 t + 0.3) (t = flat
+# index) and a few entries are sampled.  nvir > 512 exercises the column blocks
+# of the V-bar reductions.
+LARGE_CASES = (
+    # name, nocc, nvir, naux, seed, fvo_scale, et_bar
+    ('o3v520_fvo', 3, 520, 80, 11, 1e-3, 1.0),
+)
+
+
+def sin_direction(n):
+    return np.sin(0.7 * np.arange(n, dtype=np.float64) + 0.3)
+
+
 def input_checksum(p):
     return np.array([float(np.sum(x * (1.0 + np.arange(x.size).reshape(x.shape) % 7)))
                      for x in (p.mo_energy, p.t1, p.t2, p.ovvv, p.ovoo, p.ovov, p.fvo)])
 
 
+def large_case(name, o, v, naux, seed, fvo_scale, et_bar):
+    p = make_problem(o, v, naux, seed=seed, fvo_scale=fvo_scale)
+    args = (p.mo_energy, p.t1, p.t2, p.ovvv, p.ovoo, p.ovov, p.fvo)
+    e = R.energy(*args)
+    g = R.energy_vjp(*args, et_bar=et_bar)
+    out = dict(nocc=o, nvir=v, naux=naux, seed=seed, fvo_scale=fvo_scale, et_bar=et_bar, e_t=e)
+    rng = np.random.default_rng(4321)
+    for k in BLOCKS:
+        x = g[k].reshape(-1)
+        out[k + '_norm'] = np.linalg.norm(x)
+        out[k + '_sinproj'] = float(sin_direction(x.size) @ x)
+        idx = rng.choice(x.size, size=min(256, x.size), replace=False)
+        out[k + '_idx'] = idx
+        out[k + '_val'] = x[idx]
+    path = os.path.join(HERE, f'ccsd_t_{name}.npz')
+    np.savez_compressed(path, **out)
+    print(name, 'E(T) =', repr(e), '->', os.path.relpath(path, ROOT))
+
+
 def main():
     if not R.available():
         raise SystemExit('oracle/_ref not built: make -C oracle')
+    only = sys.argv[1:]
+    for case in LARGE_CASES:
+        if not only or case[0] in only:
+            large_case(*case)
     for name, o, v, naux, seed, fvo_scale, et_bar, full in CASES:
+        if only and name not in only:
+            continue
         p = make_problem(o, v, naux, seed=seed, fvo_scale=fvo_scale)
         args = (p.mo_energy, p.t1, p.t2, p.ovvv, p.ovoo, p.ovov, p.fvo)
         e = R.energy(*args)

@@ -1,0 +1,184 @@
+"""The (T) path at the benchmark configurations' own sizes, on shards of the
+occupied-triple walk.
+
+A whole E(T)+VJP at nocc=40/nvir=400 takes minutes (and an FP64 E(T) at
+nocc=60/nvir=600 about half an hour), so these tests evaluate round-robin shards
+of the walk (pamd_ccsdt_energy_shard / pamd_ccsdt_energy_vjp_shard): the same
+kernels on the full-size inputs, for a fixed share of the triples.  Every
+check below holds shard by shard:
+
+* the shards' energies and cotangents add up to the whole call's;
+* split-BF16x3 and FP32 hi/lo against plain FP64 on the same shard: |dE|
+  within the shard's share of the 1e-7 Eh bound, every cotangent block
+  within 1e-6 relative (2-norm);
+* the exact identities of the adjoint (shift invariance and degree -1 in the
+  orbital energies; E = A + B with A = E(t1 = 0)), which hold per triple;
+* central differences of the FP64 shard energy along 64 seeded unit
+  directions (h = 1e-5) against the shard VJP at nocc=24/nvir=240.
+The nocc=3/nvir=520 case (nvir above 512: the V-bar reductions run in column
+blocks) is checked against golden values of the reference's own C.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from pyscfad_amd.cc import _ops
+from pyscfad_amd.gen import make_problem
+
+from .conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def _dot(a, b):
+    return float((a * b).sum())
+
+
+def _packed(p, v, dev):
+    idx, idy = torch.tril_indices(v, v, device=dev)
+    ovvv = p.ovvv[:, :, idx, idy].contiguous()
+    del p.ovvv
+    torch.cuda.empty_cache()
+    return ovvv
+
+
+def _rel(a, b):
+    return float((a - b).norm() / b.norm())
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'bf16x3'])
+def test_vjp_shards_add_up(precision, cuda):
+    o, v, nshards = 5, 24, 3
+    print(f'vjp shards add up o{o} v{v} {precision}', flush=True)
+    p = make_problem(o, v, 4 * v, seed=4, fvo_scale=1e-3, device=cuda)
+    ovvv = _packed(p, v, cuda)
+    fvo = p.fock[o:, :o].contiguous()
+    inp = _ops.Inputs(p.mo_energy, p.t1, p.t2, ovvv, p.ovoo, p.ovov, fvo, precision=precision,
+                      device=cuda)
+    e, g = _ops.energy_vjp(inp, et_bar=0.8)
+    es, gs = 0.0, {k: torch.zeros_like(x) for k, x in g.items()}
+    for s in range(nshards):
+        e_s, g_s = _ops.energy_vjp_shard(inp, s, nshards, et_bar=0.8)
+        es += float(e_s.item())
+        for k in gs:
+            gs[k] += g_s[k]
+    assert abs(es - float(e.item())) <= 1e-13 * abs(float(e.item()))
+    for k, ref in g.items():
+        assert _rel(gs[k], ref) <= 1e-12, k
+
+
+def _sin_direction(n, dev):
+    return torch.sin(0.7 * torch.arange(n, dtype=torch.float64, device=dev) + 0.3)
+
+
+def test_nvir_above_512_against_reference_golden(cuda):
+    """nocc=3/nvir=520 with fvo != 0 (full ovvv): E(T) within 1e-10 Eh and every
+    cotangent block, through its norm, its projection on a closed-form
+    direction and 256 sampled entries, within 1e-9 relative of the
+    reference's compiled C (tests/golden/make_golden.py)."""
+    ref = np.load(os.path.join(GOLDEN, 'ccsd_t_o3v520_fvo.npz'))
+    o, v, naux = int(ref['nocc']), int(ref['nvir']), int(ref['naux'])
+    print(f'golden o{o} v{v} (nvir > 512)', flush=True)
+    p = make_problem(o, v, naux, seed=int(ref['seed']), fvo_scale=float(ref['fvo_scale']),
+                     device=cuda)
+    fvo = p.fock[o:, :o].contiguous()
+    inp = _ops.Inputs(p.mo_energy, p.t1, p.t2, p.ovvv, p.ovoo, p.ovov, fvo, precision='fp64',
+                      device=cuda)
+    e, g = _ops.energy_vjp(inp, et_bar=float(ref['et_bar']))
+    assert abs(float(e.item()) - float(ref['e_t'])) <= 1e-10
+    for k in ('t1_bar', 't2_bar', 'fvo_bar', 'mo_energy_bar', 'ovoo_bar', 'ovov_bar', 'ovvv_bar'):
+        x = g[k].reshape(-1)
+        nrm = float(ref[k + '_norm'])
+        assert abs(float(x.norm()) - nrm) <= 1e-9 * nrm, k
+        proj = float(_sin_direction(x.numel(), cuda) @ x)
+        scale = nrm * (x.numel() / 2) ** 0.5
+        assert abs(proj - float(ref[k + '_sinproj'])) <= 1e-9 * scale, k
+        idx = torch.as_tensor(ref[k + '_idx'], device=cuda)
+        val = torch.as_tensor(ref[k + '_val'], device=cuda)
+        assert float((x[idx] - val).abs().max()) <= 1e-9 * float(x.abs().max()), k
+
+
+def _identities(inp, inp_t1zero, e, g, nshards, tol):
+    """Shift invariance, degree -1 homogeneity and E = A + B on one shard."""
+    A = float(_ops.energy_shard(inp_t1zero, 0, nshards).item())
+    B = e - A
+    scale = abs(e)
+    mo = inp.mo_energy
+    assert abs(float(g['mo_energy_bar'].sum())) <= tol * scale / float(mo.abs().mean())
+    assert abs(_dot(mo, g['mo_energy_bar']) + e) <= tol * scale
+    assert abs(_dot(inp.t1, g['t1_bar']) - B) <= tol * scale
+    assert abs(_dot(inp.ovov, g['ovov_bar']) - B) <= tol * scale
+    assert abs(_dot(inp.t2, g['t2_bar']) - (2 * A + B)) <= tol * scale
+    assert abs(_dot(inp.ovvv, g['ovvv_bar']) + _dot(inp.ovoo, g['ovoo_bar']) -
+               (2 * A + B)) <= tol * scale
+
+
+def test_o40v400_three_forms_on_a_shard(cuda):
+    """nocc=40/nvir=400/naux=1600, seed 0 (the benchmark problem, packed ovvv):
+    shard 0 of 64 of E(T)+VJP in FP64, split-BF16x3 and FP32 hi/lo."""
+    o, v, naux, nshards = 40, 400, 1600, 64
+    print(f'o{o} v{v} naux{naux}: shard 0/{nshards} in three forms', flush=True)
+    p = make_problem(o, v, naux, seed=0, device=cuda)
+    ovvv = _packed(p, v, cuda)
+    out = {}
+    for prec in ('fp64', 'bf16x3', 'fp32x2'):
+        inp = _ops.Inputs(p.mo_energy, p.t1, p.t2, ovvv, p.ovoo, p.ovov, None, precision=prec,
+                          device=cuda)
+        e, g = _ops.energy_vjp_shard(inp, 0, nshards, et_bar=1.0)
+        torch.cuda.synchronize()
+        out[prec] = (float(e.item()), g)
+        print(f'{prec}: shard E = {out[prec][0]:.15e}', flush=True)
+        if prec == 'fp64':
+            inp0 = _ops.Inputs(p.mo_energy, torch.zeros_like(p.t1), p.t2, ovvv, p.ovoo, p.ovov,
+                               None, precision='fp64', device=cuda)
+            _identities(inp, inp0, out[prec][0], g, nshards, 1e-11)
+            del inp0
+        del inp
+        torch.cuda.empty_cache()
+    e64, g64 = out['fp64']
+    for prec in ('bf16x3', 'fp32x2'):
+        e_p, g_p = out[prec]
+        assert abs(e_p - e64) <= 1e-7 / nshards, (prec, e_p - e64)
+        for k, ref in g64.items():
+            err = _rel(g_p[k], ref)
+            print(f'{prec} {k}: {err:.2e}', flush=True)
+            assert err < 1e-6, (prec, k, err)
+
+
+def test_o60v600_energy_on_a_shard(cuda):
+    """nocc=60/nvir=600/naux=2400, seed 0: shard 0 of 256 of E(T) in FP64 and
+    split-BF16x3 (the second size of the precision configuration)."""
+    o, v, naux, nshards = 60, 600, 2400, 256
+    print(f'o{o} v{v} naux{naux}: energy shard 0/{nshards}', flush=True)
+    p = make_problem(o, v, naux, seed=0, device=cuda)
+    ovvv = _packed(p, v, cuda)
+    e = {}
+    for prec in ('fp64', 'bf16x3'):
+        inp = _ops.Inputs(p.mo_energy, p.t1, p.t2, ovvv, p.ovoo, p.ovov, None, precision=prec,
+                          device=cuda)
+        e[prec] = float(_ops.energy_shard(inp, 0, nshards).item())
+        print(f'{prec}: shard E = {e[prec]:.15e}', flush=True)
+        del inp
+        torch.cuda.empty_cache()
+    assert abs(e['bf16x3'] - e['fp64']) <= 1e-7 / nshards
+
+
+def test_o24v240_central_differences_64_directions(cuda):
+    """The adjoint configuration: nocc=24/nvir=240/naux=960, 64 seeded unit
+    directions over (mo_energy, t1, t2, packed ovvv, ovoo, ovov, fvo), h = 1e-5,
+    central differences of the FP64 energy of shard 0 of 16 against the FP64
+    and split-BF16x3 shard VJPs."""
+    from pyscfad_amd.cc import adjoint_check
+    rep = adjoint_check.run(24, 240, 960, directions=64, h=1e-5, precisions=['fp64', 'bf16x3'],
+                            log=lambda s: print(s, flush=True), nshards=16)
+    r64 = rep['precisions']['fp64']
+    r16 = rep['precisions']['bf16x3']
+    # errors relative to the RMS directional derivative: the difference
+    # quotient's own truncation and rounding (FP64), plus the split-BF16x3
+    # gradient error (bounded by 1e-6 relative per block)
+    assert r64['err_vs_rms_max'] <= 1e-6
+    assert r16['err_vs_rms_max'] <= 1e-5
+    assert r16['vs_fp64_rms_max'] <= 1e-5
+    assert r64['peak_device_bytes_vjp'] < r64['store_everything_bytes']
