@@ -153,7 +153,9 @@ int pamd_ccsdt_weighted_energy_vjp(const pamd_ccsdt_config *cfg,
  * of _ccsd_t_energy (lno/ccsd_t.py:124-237, lno_ccsd_t_vjp.c:400-566) chained
  * through mat = ulo^T ulo:
  *   E = 1/3 sum_{l,i,j,k,abc} mat[l,i] W_ljk[abc] Z_ijk[abc] / D_ijk[abc].
- * ulo: HOST array (nfrag, nocc), 1 <= nfrag <= 8.  e_out: device scalar.
+ * ulo: HOST array (nfrag, nocc), 1 <= nfrag <= 8 per call (E is linear in
+ * mat = sum_f u_f u_f^T, so a larger fragment is the sum of calls on row groups
+ * of ulo; pyscfad_amd.cc._ops does that).  e_out: device scalar.
  * The walk takes every unordered occupied pair (j, k) and all o triples
  * (l, j, k): about three times the W builds of pamd_ccsdt_energy, and the
  * adjoint builds each W a second time.  Cotangents as pamd_ccsdt_energy_vjp,

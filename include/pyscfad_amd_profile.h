@@ -3,7 +3,9 @@
  * reference, whose only timing is the host logger: pyscfad/lib/logger.py:63-79).
  * When enabled, every launch of one occupied triple in `every` is bracketed by
  * hipEvents on the stream it runs on; per-kernel totals accumulate until reset.
- * Process-global state, not thread-safe: for benchmarks and tests.
+ * The switches and the totals are process-wide (the totals are updated and read
+ * under a lock); which triple is sampled, and its pending events, are per host
+ * thread, so concurrent calls from several threads can be profiled together.
  */
 #ifndef PYSCFAD_AMD_PROFILE_H
 #define PYSCFAD_AMD_PROFILE_H

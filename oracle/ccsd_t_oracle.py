@@ -206,7 +206,7 @@ def projected_energy_vjp(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, ulo, et_bar=1
     Xbar = r3(Q / D)
     mo_bar, g = _backprop(nocc, t1, t2, ovvv, ovoo, ovov, fvo, P + Xbar, Xbar, -Q * Z / D ** 2)
     mat_bar = et_bar * np.einsum('ljkabc,ijkabc->li', W, Y, optimize=True) / 3.0
-    return e, dict(g, mo_energy_bar=mo_bar, ulo_bar=ulo @ (mat_bar + mat_bar.T))
+    return e, dict(g, mo_energy_bar=mo_bar, mat_bar=mat_bar, ulo_bar=ulo @ (mat_bar + mat_bar.T))
 
 
 def fold_ovvv_packed(ovvv_bar):

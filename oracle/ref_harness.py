@@ -12,7 +12,9 @@ points, so the C can run on synthetic inputs without jax or pyscf:
   the canonical E(T); with ``mat = u u^T`` the occupied-weighted one);
 * VJP via ``ccsd_t_energy_vjp`` as one block ``contract(0, v, 0, v)`` with all
   four caches = ``vvop`` -- pyscfad/cc/ccsd_t.py:108-177 (a0 = 0 makes the
-  b-loop empty), ``et_bar *= 2`` (line 86), output transforms lines 179-191.
+  b-loop empty), ``et_bar *= 2`` (line 86), output transforms lines 179-191;
+* the local-orbital projected VJP via ``lnoccsdt_energy_vjp`` as one block
+  with every cache = ``vvop`` -- pyscfad/lno/ccsd_t.py:124-237.
 
 Only tests, the golden-vector script and bench.py's cpu_baseline use it.
 """
@@ -110,3 +112,51 @@ def energy_vjp(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, et_bar=1.0):
     return dict(t1_bar=t1_bar, t2_bar=t2_bar, fvo_bar=fvo_bar, mo_energy_bar=mo_bar,
                 ovoo_bar=ovoo_bar, ovov_bar=ovov_bar, ovvv_bar=ovvv_full,
                 ovvv_bar_packed=packed)
+
+
+def projected_energy_vjp(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, ulo, et_bar=1.0):
+    """Occupied-weighted (T) of mat = ulo^T ulo and its VJP from the reference C.
+
+    Forward: ``lnoccsdt_contract`` with that mat (``energy(..., mat=...)``).
+    Backward: ``lnoccsdt_energy_vjp`` as one block, b = c = (0, nvir), every
+    cache = ``vvop`` (pyscfad/lno/ccsd_t.py:205-214, first iteration of its
+    block loop with bufsize >= nvir), ``et_bar *= 2/3`` (line 132) and the
+    output transforms of lines 226-234.  ``ovvv`` is full (o,v,v,v); the packed
+    cotangent is the reference's own fold.  Returns (E, dict) with the blocks
+    of ``energy_vjp`` plus ``mat_bar`` (o,o) and ``ulo_bar = ulo (mat_bar +
+    mat_bar^T)``, the chain rule of mat = ulo^T ulo (pyscfad/lno/ccsd_t.py:42).
+    """
+    lib = load()
+    nocc, nvir = t1.shape
+    ulo = np.ascontiguousarray(np.atleast_2d(ulo), dtype=np.float64)
+    mat = np.ascontiguousarray(ulo.T @ ulo)
+    e = energy(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, mat=mat)
+    mo_energy, t1T, t2T, vooo, fvo, vvop = _layouts(mo_energy, t1, t2, ovvv, ovoo, ovov, fvo)
+    eb = et_bar * 2.0 / 3.0
+    mat_bar = np.zeros_like(mat)
+    mo_bar = np.zeros_like(mo_energy)
+    t1T_bar = np.zeros_like(t1T)
+    t2T_bar = np.zeros_like(t2T)
+    vooo_bar = np.zeros_like(vooo)
+    fvo_bar = np.zeros_like(fvo)
+    vvop_bar = np.zeros_like(vvop)
+    lib.lnoccsdt_energy_vjp(
+        _p(mat), _p(mo_energy), _p(t1T), _p(t2T), _p(vooo), _p(fvo), ctypes.c_double(eb),
+        ctypes.c_int(nocc), ctypes.c_int(nvir),
+        ctypes.c_int(0), ctypes.c_int(nvir), ctypes.c_int(0), ctypes.c_int(nvir),
+        _p(vvop), _p(vvop), _p(vvop), _p(vvop),
+        _p(mat_bar), _p(mo_bar), _p(t1T_bar), _p(t2T_bar), _p(vooo_bar), _p(fvo_bar),
+        _p(vvop_bar), _p(vvop_bar), _p(vvop_bar), _p(vvop_bar))
+    ovvv_full = np.ascontiguousarray(vvop_bar[:, :, :, nocc:].transpose(2, 0, 3, 1))
+    g = ovvv_full + ovvv_full.transpose(0, 1, 3, 2)
+    idx, idy = np.diag_indices(nvir)
+    g[:, :, idx, idy] *= .5
+    idx, idy = np.tril_indices(nvir)
+    out = dict(t1_bar=np.ascontiguousarray(t1T_bar.T),
+               t2_bar=np.ascontiguousarray(t2T_bar.transpose(3, 2, 0, 1)),
+               fvo_bar=fvo_bar, mo_energy_bar=mo_bar,
+               ovoo_bar=np.ascontiguousarray(vooo_bar.transpose(1, 0, 3, 2)),
+               ovov_bar=np.ascontiguousarray(vvop_bar[:, :, :, :nocc].transpose(2, 0, 3, 1)),
+               ovvv_bar=ovvv_full, ovvv_bar_packed=np.ascontiguousarray(g[:, :, idx, idy]),
+               mat_bar=mat_bar, ulo_bar=ulo @ (mat_bar + mat_bar.T))
+    return e, out

@@ -227,44 +227,78 @@ def _projected_ws(inp, nfrag, with_vjp):
     return torch.empty(max(n.value, 1), dtype=torch.uint8, device=inp.device), n.value
 
 
+# fragment orbitals per library call (nfrag <= 8 in pamd_ccsdt_projected_*); E is
+# linear in mat = ulo^T ulo = sum_f u_f u_f^T, so larger fragments are summed
+# over row groups of ulo
+MAX_FRAG_PER_CALL = 8
+
+
+def _row_groups(u):
+    return [u[r:r + MAX_FRAG_PER_CALL] for r in range(0, u.shape[0], MAX_FRAG_PER_CALL)]
+
+
 def projected_energy(inp: Inputs, ulo):
     """Occupied-weighted E(T) of the local-orbital projector mat = ulo^T ulo
-    (ulo: (nfrag, nocc), nfrag <= 8), as a 0-d float64 device tensor."""
+    (ulo: (nfrag, nocc), any nfrag >= 1), as a 0-d float64 device tensor."""
     lib = _lib.load()
     u = _ulo_array(inp, ulo)
-    ws, nb = _projected_ws(inp, u.shape[0], False)
     e = torch.zeros((), dtype=torch.float64, device=inp.device)
-    st = lib.pamd_ccsdt_projected_energy(ctypes.byref(inp.cfg), *inp.args(),
-                                         u.ctypes.data_as(ctypes.c_void_p), int(u.shape[0]),
-                                         _ptr(ws), nb, _ptr(e), _stream(inp.device))
-    _lib.check(st, 'pamd_ccsdt_projected_energy')
+    for ug in _row_groups(u):
+        ug = np.ascontiguousarray(ug)
+        ws, nb = _projected_ws(inp, ug.shape[0], False)
+        eg = torch.zeros((), dtype=torch.float64, device=inp.device)
+        st = lib.pamd_ccsdt_projected_energy(ctypes.byref(inp.cfg), *inp.args(),
+                                             ug.ctypes.data_as(ctypes.c_void_p), int(ug.shape[0]),
+                                             _ptr(ws), nb, _ptr(eg), _stream(inp.device))
+        _lib.check(st, 'pamd_ccsdt_projected_energy')
+        torch.cuda.current_stream(inp.device).synchronize()   # ug is read from host memory
+        e += eg
     return e
 
 
 def projected_energy_vjp(inp: Inputs, ulo, et_bar=1.0):
     """(E, cotangents) of ``projected_energy``; the cotangents include
-    ``ulo_bar`` (nfrag, nocc)."""
+    ``ulo_bar`` (nfrag, nocc).  Fragments of more than MAX_FRAG_PER_CALL orbitals
+    run as row groups of ulo: the energies and the shared cotangents add, and
+    the rows of ulo_bar = ulo (mat_bar + mat_bar^T) come from each group's own
+    rows (mat_bar does not depend on mat)."""
     lib = _lib.load()
     o, v = inp.nocc, inp.nvir
     u = _ulo_array(inp, ulo)
     dev = inp.device
-    ws, nb = _projected_ws(inp, u.shape[0], True)
     f64 = dict(dtype=torch.float64, device=dev)
     e = torch.zeros((), **f64)
-    out = dict(
-        mo_energy_bar=torch.empty(o + v, **f64),
-        t1_bar=torch.empty((o, v), **f64),
-        t2_bar=torch.empty((o, o, v, v), **f64),
-        ovvv_bar=torch.empty(tuple(inp.ovvv.shape), **f64),
-        ovoo_bar=torch.empty((o, v, o, o), **f64),
-        ovov_bar=torch.empty((o, v, o, v), **f64),
-        fvo_bar=torch.empty((v, o), **f64),
-        ulo_bar=torch.empty(tuple(u.shape), **f64),
-    )
-    st = lib.pamd_ccsdt_projected_energy_vjp(
-        ctypes.byref(inp.cfg), *inp.args(), u.ctypes.data_as(ctypes.c_void_p), int(u.shape[0]),
-        ctypes.c_double(float(et_bar)), _ptr(ws), nb, _ptr(e), _ptr(out['mo_energy_bar']),
-        _ptr(out['t1_bar']), _ptr(out['t2_bar']), _ptr(out['ovvv_bar']), _ptr(out['ovoo_bar']),
-        _ptr(out['ovov_bar']), _ptr(out['fvo_bar']), _ptr(out['ulo_bar']), _stream(dev))
-    _lib.check(st, 'pamd_ccsdt_projected_energy_vjp')
+    out = None
+    ulo_bar = []
+    for ug in _row_groups(u):
+        ug = np.ascontiguousarray(ug)
+        ws, nb = _projected_ws(inp, ug.shape[0], True)
+        eg = torch.zeros((), **f64)
+        g = dict(
+            mo_energy_bar=torch.empty(o + v, **f64),
+            t1_bar=torch.empty((o, v), **f64),
+            t2_bar=torch.empty((o, o, v, v), **f64),
+            ovvv_bar=torch.empty(tuple(inp.ovvv.shape), **f64),
+            ovoo_bar=torch.empty((o, v, o, o), **f64),
+            ovov_bar=torch.empty((o, v, o, v), **f64),
+            fvo_bar=torch.empty((v, o), **f64),
+            ulo_bar=torch.empty(tuple(ug.shape), **f64),
+        )
+        st = lib.pamd_ccsdt_projected_energy_vjp(
+            ctypes.byref(inp.cfg), *inp.args(), ug.ctypes.data_as(ctypes.c_void_p),
+            int(ug.shape[0]), ctypes.c_double(float(et_bar)), _ptr(ws), nb, _ptr(eg),
+            _ptr(g['mo_energy_bar']), _ptr(g['t1_bar']), _ptr(g['t2_bar']), _ptr(g['ovvv_bar']),
+            _ptr(g['ovoo_bar']), _ptr(g['ovov_bar']), _ptr(g['fvo_bar']), _ptr(g['ulo_bar']),
+            _stream(dev))
+        _lib.check(st, 'pamd_ccsdt_projected_energy_vjp')
+        torch.cuda.current_stream(dev).synchronize()   # ug is read from host memory
+        del ws
+        e += eg
+        ulo_bar.append(g.pop('ulo_bar'))
+        if out is None:
+            out = g
+        else:
+            for k in out:
+                out[k] += g[k]
+    out['ulo_bar'] = torch.cat(ulo_bar, 0)
     return e, out

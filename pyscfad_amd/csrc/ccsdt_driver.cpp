@@ -22,7 +22,9 @@
 #include <cstring>
 #include <algorithm>
 #include <vector>
+#include <atomic>
 #include <chrono>
+#include <mutex>
 #include "ccsdt_common.h"
 #include "gemm_desc.h"
 #include "../../include/pyscfad_amd_ccsdt.h"
@@ -33,8 +35,9 @@ namespace pamd {
 static const int kGmax = 1024;
 
 // When set, GEMM launches are recorded instead of run and the other per-triple
-// kernels are skipped (host-only descriptor inspection, no GPU work).
-static std::vector<GemmDesc> *g_record = nullptr;
+// kernels are skipped (host-only descriptor inspection, no GPU work).  Per host
+// thread, like every other piece of per-call state here.
+static thread_local std::vector<GemmDesc> *g_record = nullptr;
 
 // Sampled launch timing (pamd_profile_*): for one triple in g_prof_every, every
 // launch is bracketed by hipEvents on its own stream; the events are read after
@@ -64,11 +67,16 @@ struct ProfStat {
   long long launches;
   double ms, flops, bytes;
 };
-static int g_prof_every = 0;
-static int g_progress_every = 0;  // report (to stderr) every N triples, after a stream sync
-static bool g_prof_on = false;  // sampling the current triple
-static std::vector<ProfRec> g_prof_recs;
+// The switches and the accumulated statistics are process-wide (read and reset
+// through pamd_profile_*, the statistics under g_prof_mutex); whether the current
+// triple is sampled, and its pending event records, belong to the calling thread,
+// so concurrent calls from different host threads never see each other's state.
+static std::atomic<int> g_prof_every{0};
+static std::atomic<int> g_progress_every{0};  // report (to stderr) every N triples, after a stream sync
+static thread_local bool g_prof_on = false;  // sampling the current triple
+static thread_local std::vector<ProfRec> g_prof_recs;
 static ProfStat g_prof_stat[kProfNum];
+static std::mutex g_prof_mutex;
 
 static hipError_t prof_begin(hipStream_t s, ProfRec &r, int kind, double flops, double bytes) {
   r.kind = kind;
@@ -94,6 +102,7 @@ static hipError_t prof_collect() {
     hipError_t e = hipEventSynchronize(r.b);
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, r.a, r.b);
     if (e == hipSuccess) {
+      std::lock_guard<std::mutex> lock(g_prof_mutex);
       ProfStat &st = g_prof_stat[r.kind];
       st.launches += 1;
       st.ms += ms;
@@ -1055,6 +1064,7 @@ int pamd_profile_progress(int every) {
 }
 
 int pamd_profile_reset(void) {
+  std::lock_guard<std::mutex> lock(g_prof_mutex);
   memset(g_prof_stat, 0, sizeof(g_prof_stat));
   return PAMD_OK;
 }
@@ -1067,6 +1077,7 @@ const char *pamd_profile_kind_name(int kind) {
 
 int pamd_profile_read(int kind, long long *launches, double *ms, double *flops, double *bytes) {
   if (kind < 0 || kind >= kProfNum || !launches || !ms || !flops || !bytes) return PAMD_ERR_ARG;
+  std::lock_guard<std::mutex> lock(g_prof_mutex);
   *launches = g_prof_stat[kind].launches;
   *ms = g_prof_stat[kind].ms;
   *flops = g_prof_stat[kind].flops;

@@ -10,6 +10,9 @@ generated inputs, and the reference outputs (E(T) from ``lnoccsdt_contract``
 with mat = I, occupied-weighted energies with mat = e_i e_i^T, and every
 cotangent block of ``ccsd_t_energy_vjp``).  Small problems keep the full
 arrays; o8/v40 keeps norms, seeded random projections and sampled entries.
+The ``lno_ccsd_t_*`` fixtures hold the local-orbital projected (T) of a seeded,
+non-diagonal mat = ulo^T ulo: E from ``lnoccsdt_contract`` and every cotangent
+block plus mat_bar from ``lnoccsdt_energy_vjp``, full arrays.
 """
 import os
 import sys
@@ -40,6 +43,28 @@ LARGE_CASES = (
     # name, nocc, nvir, naux, seed, fvo_scale, et_bar
     ('o3v520_fvo', 3, 520, 80, 11, 1e-3, 1.0),
 )
+
+
+# Local-orbital projected (T): name, nocc, nvir, naux, seed, fvo_scale, et_bar,
+# nfrag, ulo seed (ulo = 0.5 N(0, 1), shape (nfrag, nocc))
+LNO_CASES = (
+    ('o4v12', 4, 12, 48, 0, 0.0, 1.0, 2, 21),
+    ('o6v20_fvo', 6, 20, 80, 3, 1e-3, 0.7, 3, 22),
+)
+
+
+def lno_case(name, o, v, naux, seed, fvo_scale, et_bar, nfrag, useed):
+    p = make_problem(o, v, naux, seed=seed, fvo_scale=fvo_scale)
+    ulo = np.random.default_rng(useed).normal(size=(nfrag, o)) * 0.5
+    args = (p.mo_energy, p.t1, p.t2, p.ovvv, p.ovoo, p.ovov, p.fvo)
+    e, g = R.projected_energy_vjp(*args, ulo, et_bar=et_bar)
+    out = dict(nocc=o, nvir=v, naux=naux, seed=seed, fvo_scale=fvo_scale, et_bar=et_bar,
+               checksum=input_checksum(p), ulo=ulo, e_t=e)
+    for k in BLOCKS + ('mat_bar', 'ulo_bar'):
+        out[k] = g[k]
+    path = os.path.join(HERE, f'lno_ccsd_t_{name}.npz')
+    np.savez_compressed(path, **out)
+    print('lno', name, 'E =', repr(e), '->', os.path.relpath(path, ROOT))
 
 
 def sin_direction(n):
@@ -74,6 +99,9 @@ def main():
     if not R.available():
         raise SystemExit('oracle/_ref not built: make -C oracle')
     only = sys.argv[1:]
+    for case in LNO_CASES:
+        if not only or 'lno_' + case[0] in only:
+            lno_case(*case)
     for case in LARGE_CASES:
         if not only or case[0] in only:
             large_case(*case)

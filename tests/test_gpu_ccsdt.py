@@ -161,6 +161,49 @@ def test_projected_energy_vjp(name, precision, cuda):
     assert _rel(g['ovvv_bar'].cpu().numpy(), O.fold_ovvv_packed(g_ref['ovvv_bar'])) <= gtol
 
 
+@pytest.mark.parametrize('name,precision', [('o4v12', 'fp64'), ('o6v20_fvo', 'fp64'),
+                                            ('o6v20_fvo', 'bf16x3')])
+def test_projected_energy_vjp_reference_lno_golden(name, precision, cuda):
+    """pamd_ccsdt_projected_energy_vjp against the reference's own LNO (T) VJP
+    (lnoccsdt_energy_vjp, golden vectors for a non-diagonal mat = ulo^T ulo):
+    E within 1e-10 Eh, every cotangent block within 1e-9 relative (FP64), and
+    ulo_bar against ulo (mat_bar + mat_bar^T) of the reference's mat_bar."""
+    z = np.load(os.path.join(GOLDEN, f'lno_ccsd_t_{name}.npz'))
+    o, v = int(z['nocc']), int(z['nvir'])
+    p = make_problem(o, v, int(z['naux']), seed=int(z['seed']), fvo_scale=float(z['fvo_scale']))
+    print(f'projected vjp vs reference lno golden {name} {precision}', flush=True)
+    fvo = p.fvo if np.any(p.fvo) else None
+    inp = _ops.Inputs(p.mo_energy, p.t1, p.t2, p.ovvv_packed(), p.ovoo, p.ovov, fvo,
+                      precision=precision, device=cuda)
+    ulo = z['ulo']
+    e, g = _ops.projected_energy_vjp(inp, ulo, et_bar=float(z['et_bar']))
+    etol, gtol = (E_TOL, G_RTOL) if precision == 'fp64' else (1e-9, 1e-5)
+    assert abs(float(e.item()) - float(z['e_t'])) <= etol
+    for k in ('t1_bar', 't2_bar', 'fvo_bar', 'mo_energy_bar', 'ovoo_bar', 'ovov_bar'):
+        assert _rel(g[k].cpu().numpy(), z[k]) <= gtol, k
+    assert _rel(g['ovvv_bar'].cpu().numpy(), z['ovvv_bar_packed']) <= gtol
+    ulo_bar_ref = ulo @ (z['mat_bar'] + z['mat_bar'].T)
+    assert _rel(g['ulo_bar'].cpu().numpy(), ulo_bar_ref) <= gtol
+
+
+def test_projected_energy_vjp_ten_fragment_orbitals(cuda):
+    """A fragment of 10 local orbitals (more than one library call takes):
+    energy, every cotangent and ulo_bar against the oracle."""
+    p = make_problem(5, 14, 56, seed=9, fvo_scale=1e-3)
+    print('projected vjp nfrag=10 o5v14', flush=True)
+    args = (p.mo_energy, p.t1, p.t2, p.ovvv, p.ovoo, p.ovov, p.fvo)
+    ulo = np.random.default_rng(10).normal(size=(10, 5)) * 0.4
+    e_ref, g_ref = O.projected_energy_vjp(*args, ulo, et_bar=1.1)
+    inp = _ops.Inputs(p.mo_energy, p.t1, p.t2, p.ovvv_packed(), p.ovoo, p.ovov, p.fvo,
+                      precision='fp64', device=cuda)
+    assert abs(float(_ops.projected_energy(inp, ulo).item()) - e_ref) <= E_TOL
+    e, g = _ops.projected_energy_vjp(inp, ulo, et_bar=1.1)
+    assert abs(float(e.item()) - e_ref) <= E_TOL
+    for k in BLOCKS + ('ulo_bar',):
+        assert _rel(g[k].cpu().numpy(), g_ref[k]) <= G_RTOL, k
+    assert _rel(g['ovvv_bar'].cpu().numpy(), O.fold_ovvv_packed(g_ref['ovvv_bar'])) <= G_RTOL
+
+
 def test_lno_kernel_drop_in_autograd(cuda):
     """pyscfad_amd.lno.ccsd_t.kernel(mycc, eris, ulo): energy and torch autograd
     gradients (ulo, t1, t2, packed ovvv, fock) against the oracle."""

@@ -126,3 +126,30 @@ def test_projected_adjoint_central_differences_and_identity():
     _e1, g1 = O.projected_energy_vjp(*_args(p), np.eye(3), et_bar=1.0)
     for k in g0:
         np.testing.assert_allclose(g1[k], g0[k], rtol=1e-12, atol=1e-18)
+
+
+def test_projected_adjoint_matches_reference_lno_vjp_golden():
+    """The oracle's projected (T) and its adjoint, including the full mat_bar,
+    against the golden vectors of the reference's own lnoccsdt_contract /
+    lnoccsdt_energy_vjp (pyscfadlib/.../vjp/cc/lno_ccsd_t_vjp.c:400-566, driven as
+    pyscfad/lno/ccsd_t.py:124-237) for a non-diagonal mat = ulo^T ulo."""
+    import os
+    from .conftest import GOLDEN
+    for name in ('o4v12', 'o6v20_fvo'):
+        z = np.load(os.path.join(GOLDEN, f'lno_ccsd_t_{name}.npz'))
+        o, v = int(z['nocc']), int(z['nvir'])
+        p = make_problem(o, v, int(z['naux']), seed=int(z['seed']),
+                         fvo_scale=float(z['fvo_scale']))
+        ulo = z['ulo']
+        assert np.count_nonzero(ulo.T @ ulo - np.diag(np.diag(ulo.T @ ulo))) > 0
+        e, g = O.projected_energy_vjp(*_args(p), ulo, et_bar=float(z['et_bar']))
+        assert abs(e - float(z['e_t'])) <= 1e-13 * abs(float(z['e_t']))
+        for k in ('t1_bar', 't2_bar', 'fvo_bar', 'mo_energy_bar', 'ovoo_bar', 'ovov_bar',
+                  'mat_bar', 'ulo_bar'):
+            ref = z[k]
+            assert np.linalg.norm(g[k] - ref) <= 1e-12 * np.linalg.norm(ref), (name, k)
+        packed = O.fold_ovvv_packed(g['ovvv_bar'])
+        assert (np.linalg.norm(packed - z['ovvv_bar_packed'])
+                <= 1e-12 * np.linalg.norm(z['ovvv_bar_packed'])), name
+        # the chain rule of mat = ulo^T ulo, on the reference's own mat_bar
+        assert np.allclose(z['ulo_bar'], ulo @ (z['mat_bar'] + z['mat_bar'].T), rtol=1e-14)
