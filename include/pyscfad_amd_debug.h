@@ -19,6 +19,12 @@ int pamd_debug_gemm_descriptors(int nocc, int nvir, int i, int j, int k, int wit
                                 const unsigned long long *bases, void *out, int max_desc,
                                 int *n_desc, int *desc_bytes);
 
+/* 1 when a recorded descriptor (one of pamd_debug_gemm_descriptors' outputs)
+ * runs on the FP64 engine's direct-to-LDS path (16-byte element pairs: fast
+ * strides 1, other strides and index divisors even, 16-byte aligned bases),
+ * 0 when it runs on the register-staged path. */
+int pamd_debug_gemm_direct_ok(const void *desc);
+
 /* Split-K factors the adjoint uses for (particle, hole) partial buffers. */
 int pamd_debug_split_factors(int nocc, int nvir, int *S_p, int *S_h);
 

@@ -1011,6 +1011,13 @@ int pamd_ccsdt_projected_energy_vjp(const pamd_ccsdt_config *cfg, const double *
                         ovvv_bar, ovoo_bar, ovov_bar, fvo_bar, ulo_bar, stream);
 }
 
+int pamd_debug_gemm_direct_ok(const void *desc) {
+  if (!desc) return -PAMD_ERR_ARG;
+  GemmDesc d;
+  memcpy(&d, desc, sizeof(d));
+  return gemm_direct_ok(d) ? 1 : 0;
+}
+
 int pamd_debug_gemm_descriptors(int nocc, int nvir, int i, int j, int k, int with_vjp,
                                 const unsigned long long *bases, void *out, int max_desc,
                                 int *n_desc, int *desc_bytes) {

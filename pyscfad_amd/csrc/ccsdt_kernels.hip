@@ -14,6 +14,7 @@
 // * k_combine    -- per-triple gather of the adjoint pieces into the cotangents.
 // * prepare/fold -- layout changes at entry/exit.
 #include <type_traits>
+#include <cstdlib>
 #include "ccsdt_common.h"
 #include "engine_staging.h"
 #include "bf16_split.h"
@@ -421,8 +422,17 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, int n_fas
       }
 }
 
+bool gemm_direct_enabled() {
+  static const bool on = [] {
+    const char *e = getenv("PYSCFAD_AMD_GEMM_DIRECT");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s) {
   if (d.M <= 0 || d.N <= 0) return hipSuccess;
+  if (gemm_direct_enabled() && gemm_direct_ok(d)) return launch_gemm_f64_direct(d, s, false);
   // 8 waves, two workgroups per CU; the wide side of the tile goes on the
   // larger output dimension, whose operand panel is then read once (n_fast).
   if (d.M >= d.N) {
@@ -439,6 +449,7 @@ hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s) {
 
 hipError_t launch_gemm_f32x2(const GemmDesc &d, hipStream_t s) {
   if (d.M <= 0 || d.N <= 0) return hipSuccess;
+  if (gemm_direct_enabled() && gemm_direct_ok(d)) return launch_gemm_f64_direct(d, s, true);
   if (d.M >= d.N) {
     const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 63) / 64) * d.splits;
     hipLaunchKernelGGL((k_gemm_f64<128, 64, 4, 2, true>), dim3((unsigned)tiles), dim3(512), 0, s,

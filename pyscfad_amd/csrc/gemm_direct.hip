@@ -1,0 +1,628 @@
+// FP64 contraction engine with direct-to-LDS operand loads (gfx950).
+//
+// Same problem and descriptors as k_gemm_f64 (ccsdt_kernels.hip): a segmented
+// strided GEMM C (=, +=, split-K partial) sum_s alpha_s A_s B_s over K chunks
+// of 16.  The difference is how operands reach LDS: every lane moves 16 bytes
+// (two consecutive elements) per `global_load_lds_dwordx4`, which writes LDS
+// at (wave base + 16 * lane) without passing through registers.  The staging
+// layout is chosen by picking WHICH global element pair each lane fetches:
+//
+//   K-contiguous operand ("kfast"), R rows x 16 k: [r][k], 128 bytes per row,
+//     the eight 16-byte chunks of row r stored at chunk c ^ f(r), with
+//     f(r) = (r ^ (r >> 3)) & 7, so the 16 rows a fragment read touches hit 16
+//     distinct 16-byte slots of every 256-byte bank window;
+//   row-contiguous operand ("rfast"): [k][r], R doubles per k, the R/2 chunks
+//     of k-row k stored at chunk p ^ (((k >> 2) & 3) << 3), so the four lane
+//     groups of a fragment read (k = 4g + s) sit 128 bytes apart.
+//
+// Three LDS stages: while chunk g is multiplied, the loads of chunks g+1 and
+// g+2 are in flight.  Each iteration waits for its own chunk's loads
+// (s_waitcnt vmcnt), then one s_barrier (without the memory fence that
+// __syncthreads adds, which would also wait for the chunks still in flight)
+// makes every wave's loads visible and retires the slot read one iteration
+// earlier, which the next loads then overwrite.
+//
+// A pair of elements must be contiguous and 16-byte aligned, so an operand is
+// eligible when its fast stride is 1, every other stride and index divisor is
+// even, its base is 16-byte aligned, the rows of a row-contiguous operand are
+// even in number, and every segment length is even (gemm_direct_ok).  Rows past
+// M or N and k past L are clamped to valid elements (finite inputs); k past L
+// is then zeroed in the A fragments, rows past M/N are never stored.
+#include "ccsdt_common.h"
+#include "engine_staging.h"
+
+namespace pamd {
+
+namespace {
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+constexpr int kDStages = 3;
+constexpr int kDK = 16;  // K chunk
+
+typedef __attribute__((address_space(3))) void *lds_ptr_t;
+
+// s_waitcnt with only the vector-memory counter constrained (gfx9 encoding:
+// vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+__device__ __forceinline__ int kfast_swz(int r) { return (r ^ (r >> 3)) & 7; }
+
+// per-thread load plan of one operand (R rows) for one segment
+template <int R>
+struct DPlan {
+  static constexpr int NI = R / 64;  // 16-byte loads per thread per chunk
+  int roff[NI];                      // row offsets (elements, clamped rows)
+  int kl[NI];                        // k within the chunk of each load
+};
+
+template <int R>
+__device__ __forceinline__ void dplan(const OpDesc &d, int Rtot, int r0, int tid, DPlan<R> &pl) {
+#pragma unroll
+  for (int q = 0; q < DPlan<R>::NI; ++q) {
+    int r, kl;
+    if (d.rfast) {
+      const int id = q * 512 + tid;
+      const int k = id / (R / 2), pc = id % (R / 2);
+      const int p = pc ^ (((k >> 2) & 3) << 3);
+      r = r0 + 2 * p;
+      if (r > Rtot - 2) r = Rtot - 2;  // Rtot is even for a row-contiguous operand
+      kl = k;
+    } else {
+      const int id = q * 512 + tid;
+      const int rl = id >> 3, pc = id & 7;
+      kl = 2 * (pc ^ kfast_swz(rl));
+      r = r0 + rl;
+      if (r > Rtot - 1) r = Rtot - 1;
+    }
+    const int qd = r / d.rdiv;
+    pl.roff[q] = (int)(qd * d.rx + (r - qd * d.rdiv) * d.ry);
+    pl.kl[q] = kl;
+  }
+}
+
+// K offset of chunk-local k = kl at chunk start kappa (kp = divmod(kappa, kdiv)),
+// clamped to the last valid element (pair) when past L
+__device__ __forceinline__ int dkoff(const OpDesc &d, int L, int kappa, KPos kp, int kl) {
+  const int lim = L - (d.rfast ? 1 : 2) - kappa;
+  if (kl > lim) kl = lim;
+  int r = kp.r + kl, q = kp.q;
+  while (r >= d.kdiv) {
+    r -= d.kdiv;
+    ++q;
+  }
+  return (int)(q * d.kx + r * d.ky);
+}
+
+template <int R>
+__device__ __forceinline__ void dissue(const OpDesc &d, int L, int kappa, KPos kp,
+                                       const DPlan<R> &pl, double *lds, int wave) {
+  const double *p = d.p;
+#pragma unroll
+  for (int q = 0; q < DPlan<R>::NI; ++q) {
+    const double *src = p + pl.roff[q] + dkoff(d, L, kappa, kp, pl.kl[q]);
+    __builtin_amdgcn_global_load_lds((const void *)src, (lds_ptr_t)(lds + (q * 512 + wave * 64) * 2),
+                                     16, 0, 0);
+  }
+}
+
+__device__ __forceinline__ void kadvance(KPos &kp, int kdiv) {
+  kp.r += kDK;
+  while (kp.r >= kdiv) {
+    kp.r -= kdiv;
+    ++kp.q;
+  }
+}
+
+}  // namespace
+
+template <int MT, int NT, int WM, int WN, bool FX2>
+__global__ __launch_bounds__(512) void k_gemm_f64d(GemmDesc d, int n_fast) {
+  static_assert(WM * WN == 8, "8 waves");
+  constexpr int WTM = MT / WM, WTN = NT / WN;
+  constexpr int TM = WTM / 16, TN = WTN / 16;
+  constexpr int NIA = MT / 64, NIB = NT / 64, NI = NIA + NIB;
+  constexpr int SA = MT * kDK, SB = NT * kDK, SS = SA + SB;  // doubles per stage
+  constexpr int kFlush2 = 8;
+  __shared__ __align__(16) double S[kDStages * SS];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+  const int mt = (d.M + MT - 1) / MT, nt = (d.N + NT - 1) / NT;
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const int split = logical / (mt * nt);
+  const int tile = logical - split * (mt * nt);
+  const int tm = n_fast ? tile / nt : tile % mt;
+  const int tn = n_fast ? tile % nt : tile / mt;
+  const int m0 = tm * MT, n0 = tn * NT;
+  const int total = d.chunk_start[d.nseg];
+  const int cps = (total + d.splits - 1) / d.splits;
+  const int g0 = split * cps;
+  const int g1 = min(total, g0 + cps);
+
+  d4v acc[TM][TN];
+  f4v facc[FX2 ? TM : 1][FX2 ? TN : 1];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = d4v{0.0, 0.0, 0.0, 0.0};
+  if constexpr (FX2) {
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int b = 0; b < TN; ++b) facc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
+  }
+  int nfl = 0;
+
+  // fetch tracker: segment, chunk start and K position of the next chunk to load
+  int fs = 0;
+  while (fs + 1 < d.nseg && g0 >= d.chunk_start[fs + 1]) ++fs;
+  int fkappa = (g0 - d.chunk_start[fs]) * kDK;
+  KPos fka{fkappa / d.seg[fs].A.kdiv, fkappa % d.seg[fs].A.kdiv};
+  KPos fkb{fkappa / d.seg[fs].B.kdiv, fkappa % d.seg[fs].B.kdiv};
+  DPlan<MT> pa;
+  DPlan<NT> pb;
+  dplan<MT>(d.seg[fs].A, d.M, m0, tid, pa);
+  dplan<NT>(d.seg[fs].B, d.N, n0, tid, pb);
+  // compute tracker
+  int cs = fs, ckappa = fkappa;
+
+  auto issue = [&](int g) {  // loads of chunk g (the fetch tracker's chunk) into its stage
+    double *st = S + ((g - g0) % kDStages) * SS;
+    const SegDesc &sg = d.seg[fs];
+    dissue<MT>(sg.A, sg.L, fkappa, fka, pa, st, wave);
+    dissue<NT>(sg.B, sg.L, fkappa, fkb, pb, st + SA, wave);
+    // advance the tracker to chunk g + 1
+    if (g + 1 < g1) {
+      if (g + 1 >= d.chunk_start[fs + 1]) {
+        while (g + 1 >= d.chunk_start[fs + 1]) ++fs;  // (skips segments of length 0)
+        fkappa = 0;
+        fka = KPos{0, 0};
+        fkb = KPos{0, 0};
+        dplan<MT>(d.seg[fs].A, d.M, m0, tid, pa);
+        dplan<NT>(d.seg[fs].B, d.N, n0, tid, pb);
+      } else {
+        fkappa += kDK;
+        kadvance(fka, d.seg[fs].A.kdiv);
+        kadvance(fkb, d.seg[fs].B.kdiv);
+      }
+    }
+  };
+
+  if (g0 < g1) issue(g0);
+  if (g0 + 1 < g1) issue(g0 + 1);
+
+  const int g4 = (lane >> 4) * 4;
+  for (int g = g0; g < g1; ++g) {
+    if (g + 1 < g1)
+      wait_vmcnt<NI>();
+    else
+      wait_vmcnt<0>();
+    __builtin_amdgcn_s_barrier();
+    if (g + 2 < g1) issue(g + 2);
+
+    // compute chunk g
+    if (g >= d.chunk_start[cs + 1]) {
+      while (g >= d.chunk_start[cs + 1]) ++cs;
+      ckappa = 0;
+    }
+    const SegDesc &sg = d.seg[cs];
+    const int la = sg.A.rfast, lb = sg.B.rfast;
+    const double alpha = sg.alpha;
+    const int kv = sg.L - ckappa;  // valid k in this chunk (>= 16: all)
+    ckappa += kDK;
+    const double *As = S + ((g - g0) % kDStages) * SS;
+    const double *Bs = As + SA;
+    double fa[4][TM], fb[4][TN];
+#pragma unroll
+    for (int t = 0; t < TM; ++t) {
+      const int r = wm * WTM + t * 16 + (lane & 15);
+      if (la) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int k = g4 + s;
+          const int c = (r >> 1) ^ (((k >> 2) & 3) << 3);
+          fa[s][t] = As[k * MT + 2 * c + (r & 1)];
+        }
+      } else {
+        const int f = kfast_swz(r);
+        const d2v x0 = *(const d2v *)(As + r * kDK + 2 * ((g4 >> 1) ^ f));
+        const d2v x1 = *(const d2v *)(As + r * kDK + 2 * (((g4 >> 1) + 1) ^ f));
+        fa[0][t] = x0[0];
+        fa[1][t] = x0[1];
+        fa[2][t] = x1[0];
+        fa[3][t] = x1[1];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < TN; ++t) {
+      const int r = wn * WTN + t * 16 + (lane & 15);
+      if (lb) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int k = g4 + s;
+          const int c = (r >> 1) ^ (((k >> 2) & 3) << 3);
+          fb[s][t] = Bs[k * NT + 2 * c + (r & 1)];
+        }
+      } else {
+        const int f = kfast_swz(r);
+        const d2v x0 = *(const d2v *)(Bs + r * kDK + 2 * ((g4 >> 1) ^ f));
+        const d2v x1 = *(const d2v *)(Bs + r * kDK + 2 * (((g4 >> 1) + 1) ^ f));
+        fb[0][t] = x0[0];
+        fb[1][t] = x0[1];
+        fb[2][t] = x1[0];
+        fb[3][t] = x1[1];
+      }
+    }
+    if (kv < kDK) {  // k past the segment: zero the A side (B holds clamped, finite values)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        if (g4 + s >= kv) {
+#pragma unroll
+          for (int t = 0; t < TM; ++t) fa[s][t] = 0.0;
+        }
+    }
+    if constexpr (FX2) {
+      if (alpha < 0.0) {  // FP32 hi/lo pairs: flip both signs
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+          for (int t = 0; t < TM; ++t)
+            fa[s][t] = __longlong_as_double(__double_as_longlong(fa[s][t]) ^ 0x8000000080000000ll);
+      }
+    } else if (alpha != 1.0) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int t = 0; t < TM; ++t) fa[s][t] *= alpha;
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TN; ++y) {
+          if constexpr (FX2) {
+            const unsigned long long ua = __double_as_longlong(fa[s][x]);
+            const unsigned long long ub = __double_as_longlong(fb[s][y]);
+            const float ah = __uint_as_float((unsigned)ua), al = __uint_as_float((unsigned)(ua >> 32));
+            const float bh = __uint_as_float((unsigned)ub), bl = __uint_as_float((unsigned)(ub >> 32));
+            facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, facc[x][y], 0, 0, 0);
+            facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, facc[x][y], 0, 0, 0);
+            facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, facc[x][y], 0, 0, 0);
+          } else {
+            acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s][x], fb[s][y], acc[x][y], 0, 0, 0);
+          }
+        }
+    if constexpr (FX2) {
+      if (++nfl == kFlush2 || g + 1 == g1) {
+        nfl = 0;
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+#pragma unroll
+          for (int y = 0; y < TN; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              acc[x][y][r] += (double)facc[x][y][r];
+              facc[x][y][r] = 0.f;
+            }
+      }
+    }
+  }
+
+  // D layout of v_mfma_f64_16x16x4f64: lane l, element r holds row 4 r + l / 16,
+  // column l % 16 (FP32 16x16x4: row 4 (l / 16) + r)
+#pragma unroll
+  for (int x = 0; x < TM; ++x)
+#pragma unroll
+    for (int y = 0; y < TN; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ml = wm * WTM + x * 16 + (FX2 ? 4 * (lane >> 4) + r : 4 * r + (lane >> 4));
+        const int nl = wn * WTN + y * 16 + (lane & 15);
+        const bool part = d.mode == kPartial;
+        const OutPos pm = out_pos<MT>(tm, ml, part ? (1 << 30) : d.C.mdiv);
+        const OutPos pn = out_pos<NT>(tn, nl, part ? (1 << 30) : d.C.ndiv);
+        const int m = pm.idx, n = pn.idx;
+        if (m >= d.M || n >= d.N) continue;
+        const double val = acc[x][y][r];
+        if (part) {
+          d.partial[((size_t)split * d.M + m) * d.N + n] = val;
+        } else {
+          double *c = d.C.p + (long long)pm.q * d.C.mx + (long long)pm.r * d.C.my +
+                      (long long)pn.q * d.C.nx + (long long)pn.r * d.C.ny;
+          if (d.mode == kAccumulate)
+            *c += val;
+          else
+            *c = val;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Split-BF16x3 engine on the same direct-to-LDS staging.
+//
+// Operands are the packed 8-byte elements (bits 0-15 hi, 16-31 mid, 32-47 lo
+// bf16) that the split-BF16x3 path keeps in the FP64 arrays' layout, so the
+// staging, the LDS layout and the descriptors are exactly those of the FP64
+// engine above.  The three bf16 planes are separated when the MFMA fragments
+// are read: v_mfma_f32_32x32x16_bf16 takes 8 consecutive k of one row per
+// lane (lane l: row l % 32, k = 8 (l / 32) + 0..7), i.e. the whole K chunk of
+// 16 in one MFMA step, and each pair of packed elements gives one dword of
+// every plane (three v_perm_b32).  The six products hh, hm, mh, mm, hl, lh are
+// summed into one FP32 tile per 32 x 32 output block over the workgroup's
+// whole K range (split-K parts span at most 8192 k), then added in FP64.
+// ---------------------------------------------------------------------------
+namespace {
+
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+typedef float f32x16v __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4v __attribute__((ext_vector_type(4)));
+typedef unsigned long long u64x2v __attribute__((ext_vector_type(2)));
+
+struct Planes {
+  bf16x8v h, m, l;
+};
+
+__device__ __forceinline__ Planes b3d_unpack(const unsigned long long (&e)[8]) {
+  u32x4v H, M, L;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const unsigned a0 = (unsigned)e[2 * q], a1 = (unsigned)e[2 * q + 1];
+    H[q] = __builtin_amdgcn_perm(a1, a0, 0x05040100u);
+    M[q] = __builtin_amdgcn_perm(a1, a0, 0x07060302u);
+    L[q] = __builtin_amdgcn_perm((unsigned)(e[2 * q + 1] >> 32), (unsigned)(e[2 * q] >> 32),
+                                 0x05040100u);
+  }
+  Planes p;
+  p.h = __builtin_bit_cast(bf16x8v, H);
+  p.m = __builtin_bit_cast(bf16x8v, M);
+  p.l = __builtin_bit_cast(bf16x8v, L);
+  return p;
+}
+
+// the 8 packed elements (k = 8 h .. 8 h + 7) of tile row r from a staged chunk
+template <int R>
+__device__ __forceinline__ void b3d_frag(const double *Sx, int rfast, int r, int h,
+                                         unsigned long long (&e)[8]) {
+  const unsigned long long *S = reinterpret_cast<const unsigned long long *>(Sx);
+  if (rfast) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = 8 * h + j;
+      const int c = (r >> 1) ^ (((k >> 2) & 3) << 3);
+      e[j] = S[k * R + 2 * c + (r & 1)];
+    }
+  } else {
+    const int f = kfast_swz(r);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const u64x2v x = *(const u64x2v *)(S + r * kDK + 2 * ((4 * h + j) ^ f));
+      e[2 * j] = x[0];
+      e[2 * j + 1] = x[1];
+    }
+  }
+}
+
+}  // namespace
+
+template <int MT, int NT, int WM, int WN>
+__global__ __launch_bounds__(512, 4) void k_gemm_b3d(GemmDesc d, int n_fast) {
+  static_assert(WM * WN == 8, "8 waves");
+  constexpr int WTM = MT / WM, WTN = NT / WN;
+  constexpr int TM = WTM / 32, TN = WTN / 32;
+  static_assert(TM * 32 == WTM && TN * 32 == WTN, "32x32 MFMA tiles");
+  constexpr int NIA = MT / 64, NIB = NT / 64, NI = NIA + NIB;
+  constexpr int SA = MT * kDK, SB = NT * kDK, SS = SA + SB;  // 8-byte elements per stage
+  __shared__ __align__(16) double S[kDStages * SS];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+  const int mt = (d.M + MT - 1) / MT, nt = (d.N + NT - 1) / NT;
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const int split = logical / (mt * nt);
+  const int tile = logical - split * (mt * nt);
+  const int tm = n_fast ? tile / nt : tile % mt;
+  const int tn = n_fast ? tile % nt : tile / mt;
+  const int m0 = tm * MT, n0 = tn * NT;
+  const int total = d.chunk_start[d.nseg];
+  const int cps = (total + d.splits - 1) / d.splits;
+  const int g0 = split * cps;
+  const int g1 = min(total, g0 + cps);
+
+  f32x16v acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
+
+  int fs = 0;
+  while (fs + 1 < d.nseg && g0 >= d.chunk_start[fs + 1]) ++fs;
+  int fkappa = (g0 - d.chunk_start[fs]) * kDK;
+  KPos fka{fkappa / d.seg[fs].A.kdiv, fkappa % d.seg[fs].A.kdiv};
+  KPos fkb{fkappa / d.seg[fs].B.kdiv, fkappa % d.seg[fs].B.kdiv};
+  DPlan<MT> pa;
+  DPlan<NT> pb;
+  dplan<MT>(d.seg[fs].A, d.M, m0, tid, pa);
+  dplan<NT>(d.seg[fs].B, d.N, n0, tid, pb);
+  int cs = fs, ckappa = fkappa;
+
+  auto issue = [&](int g) {
+    double *st = S + ((g - g0) % kDStages) * SS;
+    const SegDesc &sg = d.seg[fs];
+    dissue<MT>(sg.A, sg.L, fkappa, fka, pa, st, wave);
+    dissue<NT>(sg.B, sg.L, fkappa, fkb, pb, st + SA, wave);
+    if (g + 1 < g1) {
+      if (g + 1 >= d.chunk_start[fs + 1]) {
+        while (g + 1 >= d.chunk_start[fs + 1]) ++fs;
+        fkappa = 0;
+        fka = KPos{0, 0};
+        fkb = KPos{0, 0};
+        dplan<MT>(d.seg[fs].A, d.M, m0, tid, pa);
+        dplan<NT>(d.seg[fs].B, d.N, n0, tid, pb);
+      } else {
+        fkappa += kDK;
+        kadvance(fka, d.seg[fs].A.kdiv);
+        kadvance(fkb, d.seg[fs].B.kdiv);
+      }
+    }
+  };
+
+  if (g0 < g1) issue(g0);
+  if (g0 + 1 < g1) issue(g0 + 1);
+
+  const int h = lane >> 5;
+  for (int g = g0; g < g1; ++g) {
+    if (g + 1 < g1)
+      wait_vmcnt<NI>();
+    else
+      wait_vmcnt<0>();
+    __builtin_amdgcn_s_barrier();
+    if (g + 2 < g1) issue(g + 2);
+
+    if (g >= d.chunk_start[cs + 1]) {
+      while (g >= d.chunk_start[cs + 1]) ++cs;
+      ckappa = 0;
+    }
+    const SegDesc &sg = d.seg[cs];
+    const int la = sg.A.rfast, lb = sg.B.rfast;
+    const bool neg = sg.alpha < 0.0;
+    const int kv = sg.L - ckappa;
+    ckappa += kDK;
+    const double *As = S + ((g - g0) % kDStages) * SS;
+    const double *Bs = As + SA;
+    Planes pa_[TM];
+#pragma unroll
+    for (int t = 0; t < TM; ++t) {
+      unsigned long long e[8];
+      b3d_frag<MT>(As, la, wm * WTM + t * 32 + (lane & 31), h, e);
+      if (kv < kDK) {  // k past the segment: zero the A side
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (8 * h + j >= kv) e[j] = 0ull;
+      }
+      if (neg) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) e[j] ^= 0x0000800080008000ull;
+      }
+      pa_[t] = b3d_unpack(e);
+    }
+#pragma unroll
+    for (int t = 0; t < TN; ++t) {
+      unsigned long long e[8];
+      b3d_frag<NT>(Bs, lb, wn * WTN + t * 32 + (lane & 31), h, e);
+      const Planes pb_ = b3d_unpack(e);
+#pragma unroll
+      for (int a = 0; a < TM; ++a) {
+        f32x16v &c = acc[a][t];
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pa_[a].m, pb_.m, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pa_[a].h, pb_.l, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pa_[a].l, pb_.h, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pa_[a].h, pb_.m, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pa_[a].m, pb_.h, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pa_[a].h, pb_.h, c, 0, 0, 0);
+      }
+    }
+  }
+
+  // D layout of v_mfma_f32_32x32x16_bf16: lane l, element r -> row 8 (r/4) + 4 (l/32) + r%4,
+  // column l%32
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int ml = wm * WTM + 32 * a + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int nl = wn * WTN + 32 * b + (lane & 31);
+        const bool part = d.mode == kPartial;
+        const OutPos pm = out_pos<MT>(tm, ml, part ? (1 << 30) : d.C.mdiv);
+        const OutPos pn = out_pos<NT>(tn, nl, part ? (1 << 30) : d.C.ndiv);
+        const int m = pm.idx, n = pn.idx;
+        if (m >= d.M || n >= d.N) continue;
+        const double val = (double)acc[a][b][r];
+        if (part) {
+          d.partial[((size_t)split * d.M + m) * d.N + n] = val;
+        } else {
+          double *c = d.C.p + (long long)pm.q * d.C.mx + (long long)pm.r * d.C.my +
+                      (long long)pn.q * d.C.nx + (long long)pn.r * d.C.ny;
+          if (d.mode == kAccumulate)
+            *c += val;
+          else
+            *c = val;
+        }
+      }
+}
+
+hipError_t launch_gemm_b3_direct(const GemmDesc &d, hipStream_t s) {
+  if (d.M <= 0 || d.N <= 0) return hipSuccess;
+  if (d.M >= d.N) {
+    const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 63) / 64) * d.splits;
+    hipLaunchKernelGGL((k_gemm_b3d<128, 64, 4, 2>), dim3((unsigned)tiles), dim3(512), 0, s, d, 1);
+  } else {
+    const long long tiles = (long long)((d.M + 63) / 64) * ((d.N + 127) / 128) * d.splits;
+    hipLaunchKernelGGL((k_gemm_b3d<64, 128, 2, 4>), dim3((unsigned)tiles), dim3(512), 0, s, d, 0);
+  }
+  return hipGetLastError();
+}
+
+// An operand can be staged with 16-byte direct loads: two consecutive elements
+// along its fast index are adjacent in memory and 16-byte aligned, for every
+// row and k the kernel may address (rows clamped to [0, rows), k to [0, L)).
+static bool op_direct_ok(const OpDesc &o, int rows, int L) {
+  auto even = [](long long x) { return (x & 1) == 0; };
+  const bool rsplit = o.rdiv < (1 << 30), ksplit = o.kdiv < (1 << 30);
+  if (((uintptr_t)o.p & 15) != 0) return false;
+  if (o.rfast) {
+    if (o.ry != 1 || (rows & 1)) return false;
+    if (rsplit && ((o.rdiv & 1) || !even(o.rx))) return false;
+    if (!even(o.ky) || (ksplit && !even(o.kx))) return false;
+  } else {
+    if (o.ky != 1 || (L & 1)) return false;
+    if (ksplit && ((o.kdiv & 1) || !even(o.kx))) return false;
+    if (!even(o.ry) || (rsplit && !even(o.rx))) return false;
+  }
+  return true;
+}
+
+bool gemm_direct_ok(const GemmDesc &d) {
+  if (d.M <= 0 || d.N <= 0 || d.nseg <= 0) return false;
+  for (int q = 0; q < d.nseg; ++q) {
+    const SegDesc &s = d.seg[q];
+    if (s.L & 1) return false;
+    if (!op_direct_ok(s.A, d.M, s.L) || !op_direct_ok(s.B, d.N, s.L)) return false;
+  }
+  return true;
+}
+
+hipError_t launch_gemm_f64_direct(const GemmDesc &d, hipStream_t s, bool fx2) {
+  if (d.M <= 0 || d.N <= 0) return hipSuccess;
+  if (d.M >= d.N) {
+    const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 63) / 64) * d.splits;
+    if (fx2)
+      hipLaunchKernelGGL((k_gemm_f64d<128, 64, 4, 2, true>), dim3((unsigned)tiles), dim3(512), 0, s,
+                         d, 1);
+    else
+      hipLaunchKernelGGL((k_gemm_f64d<128, 64, 4, 2, false>), dim3((unsigned)tiles), dim3(512), 0,
+                         s, d, 1);
+  } else {
+    const long long tiles = (long long)((d.M + 63) / 64) * ((d.N + 127) / 128) * d.splits;
+    if (fx2)
+      hipLaunchKernelGGL((k_gemm_f64d<64, 128, 2, 4, true>), dim3((unsigned)tiles), dim3(512), 0, s,
+                         d, 0);
+    else
+      hipLaunchKernelGGL((k_gemm_f64d<64, 128, 2, 4, false>), dim3((unsigned)tiles), dim3(512), 0,
+                         s, d, 0);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace pamd

@@ -108,3 +108,62 @@ def test_bf16x3_split_k_bounds_fp32_sum_length(o, v):
     for S in (S_p, S_h):
         assert 1 <= S <= max(1, K // 32)
         assert -(-K // S) <= 8192 or S == max(1, K // 32)
+
+
+def test_direct_engine_eligibility():
+    """Which contractions run on the FP64 engine's direct-to-LDS path: every
+    W-build and adjoint contraction at nocc=40/nvir=400 and at the golden-vector
+    sizes (even sizes, 16-byte aligned slabs); not every one when nocc and nvir
+    are odd (element pairs are then not 16-byte aligned)."""
+    import ctypes
+    from pyscfad_amd import _lib
+    lib = _lib.load()
+    for o, v in ((40, 400), (8, 40), (4, 12), (6, 20)):
+        descs = record(o, v, 0, 1, 2, with_vjp=True)
+        ok = [lib.pamd_debug_gemm_direct_ok(ctypes.byref(d)) for d in descs]
+        assert ok == [1] * len(descs), (o, v, ok)
+    descs = record(3, 17, 0, 1, 2, with_vjp=True)
+    assert 0 in [lib.pamd_debug_gemm_direct_ok(ctypes.byref(d)) for d in descs]
+
+
+def _direct_stage_map(R, rfast):
+    """LDS slot (in doubles) of each tile element (r, k), r < R, k < 16, as the
+    direct-load plan of gemm_direct.hip places it: thread t, load q writes 16
+    bytes at byte 16 (512 q + t); kfast slot = 16 r + 2 (c ^ f(r)) + k % 2,
+    rfast slot = R k + 2 (p ^ sw(k)) + r % 2."""
+    slot = {}
+    for q in range(R // 64):
+        for t in range(512):
+            idx = q * 512 + t
+            if rfast:
+                k, pc = divmod(idx, R // 2)
+                p = pc ^ (((k >> 2) & 3) << 3)
+                for e in range(2):
+                    slot[(2 * p + e, k)] = 2 * idx + e
+            else:
+                rl, pc = idx >> 3, idx & 7
+                c = pc ^ ((rl ^ (rl >> 3)) & 7)
+                for e in range(2):
+                    slot[(rl, 2 * c + e)] = 2 * idx + e
+    return slot
+
+
+@pytest.mark.parametrize('R', [64, 128])
+@pytest.mark.parametrize('rfast', [0, 1])
+def test_direct_engine_lds_layout_round_trip(R, rfast):
+    """The direct-load plan stores every (row, k) of a chunk exactly once, and
+    the fragment reads of the kernel (lane l: row 16 t + l % 16, k = 4 (l / 16)
+    + s) find each element where it was stored."""
+    slot = _direct_stage_map(R, rfast)
+    assert len(slot) == R * 16 and sorted(slot.values()) == list(range(R * 16))
+    for r in range(R):
+        for k in range(16):
+            g4, s = (k // 4) * 4, k % 4
+            if rfast:
+                c = (r >> 1) ^ (((k >> 2) & 3) << 3)
+                read = k * R + 2 * c + (r & 1)
+            else:
+                f = (r ^ (r >> 3)) & 7
+                chunk = ((g4 >> 1) + (s >> 1)) ^ f
+                read = r * 16 + 2 * chunk + (s & 1)
+            assert read == slot[(r, k)], (r, k)
