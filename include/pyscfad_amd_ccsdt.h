@@ -116,6 +116,28 @@ int pamd_ccsdt_energy_vjp_shard(const pamd_ccsdt_config *cfg,
                                 double *t2_bar, double *ovvv_bar, double *ovoo_bar,
                                 double *ovov_bar, double *fvo_bar, void *stream);
 
+/* pamd_ccsdt_energy_vjp without a workspace argument and without any device
+ * allocation.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The workspace is a library-owned device
+ * array of pamd_ccsdt_noalloc_scratch_bytes() bytes (128 MiB), one per GPU,
+ * held from the first call on that GPU until the process ends; problems whose
+ * pamd_ccsdt_workspace_size(cfg, 1, ...) exceeds it are rejected with
+ * PAMD_ERR_WORKSPACE (e.g. nocc=10/nvir=64 fits, nocc=16/nvir=64 does not).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (several streams or host threads) race and must be serialised by the caller.
+ * The call overwrites every part of the array it reads, so nothing carries
+ * over from an earlier call.  Arguments, outputs and results are those of
+ * pamd_ccsdt_energy_vjp; sampled profiling is not recorded for these calls. */
+int pamd_ccsdt_energy_vjp_noalloc(const pamd_ccsdt_config *cfg,
+                                  const double *mo_energy, const double *t1, const double *t2,
+                                  const double *ovvv, const double *ovoo, const double *ovov,
+                                  const double *fvo, double et_bar,
+                                  double *e_out, double *mo_energy_bar, double *t1_bar,
+                                  double *t2_bar, double *ovvv_bar, double *ovoo_bar,
+                                  double *ovov_bar, double *fvo_bar, void *stream);
+size_t pamd_ccsdt_noalloc_scratch_bytes(void);
+
 /* Occupied-pair resolved E(T): e_pair[i*o + j] = 1/3 sum_{k,abc} W Z / D,
  * sum(e_pair) = E(T).  Row sums are the occupied-weighted (T) of
  * lnoccsdt_contract with mat = e_i e_i^T (lno_ccsd_t.c:16-26); the labels of the

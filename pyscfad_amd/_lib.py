@@ -25,6 +25,7 @@ PRECISIONS = {'fp64': 0, 'bf16x3': 1, 'fp32x2': 2}
 EXPORTS = ('pamd_version', 'pamd_ccsdt_workspace_size', 'pamd_ccsdt_energy',
            'pamd_ccsdt_energy_vjp', 'pamd_ccsdt_pair_energy', 'pamd_ccsdt_num_triples',
            'pamd_ccsdt_energy_shard', 'pamd_ccsdt_energy_vjp_shard',
+           'pamd_ccsdt_energy_vjp_noalloc', 'pamd_ccsdt_noalloc_scratch_bytes',
            'pamd_ccsdt_weighted_energy_vjp',
            'pamd_ccsdt_projected_workspace_size', 'pamd_ccsdt_projected_energy',
            'pamd_ccsdt_projected_energy_vjp', 'pamd_profile_enable',
@@ -73,6 +74,10 @@ def load():
     lib.pamd_ccsdt_energy_vjp_shard.restype = i
     lib.pamd_ccsdt_energy_vjp_shard.argtypes = ([cfgp] + [vp] * 7 + [d, i, i, vp, sz] +
                                                 [vp] * 8 + [vp])
+    lib.pamd_ccsdt_energy_vjp_noalloc.restype = i
+    lib.pamd_ccsdt_energy_vjp_noalloc.argtypes = [cfgp] + [vp] * 7 + [d] + [vp] * 8 + [vp]
+    lib.pamd_ccsdt_noalloc_scratch_bytes.restype = sz
+    lib.pamd_ccsdt_noalloc_scratch_bytes.argtypes = []
     lib.pamd_ccsdt_projected_workspace_size.restype = i
     lib.pamd_ccsdt_projected_workspace_size.argtypes = [cfgp, i, i, ctypes.POINTER(sz)]
     lib.pamd_ccsdt_projected_energy_vjp.restype = i

@@ -74,6 +74,8 @@ struct ProfStat {
 static std::atomic<int> g_prof_every{0};
 static std::atomic<int> g_progress_every{0};  // report (to stderr) every N triples, after a stream sync
 static thread_local bool g_prof_on = false;  // sampling the current triple
+// set for the duration of a call that must not allocate: no event records then
+static thread_local bool g_prof_blocked = false;
 static thread_local std::vector<ProfRec> g_prof_recs;
 static ProfStat g_prof_stat[kProfNum];
 static std::mutex g_prof_mutex;
@@ -127,7 +129,7 @@ static double gemm_flops(const GemmDesc &d) {
 // timed launch wrapper for any kernel
 template <class F>
 static hipError_t timed(hipStream_t s, int kind, double flops, double bytes, F &&launch) {
-  if (!g_prof_on) return launch();
+  if (!g_prof_on || g_prof_blocked) return launch();
   ProfRec r;
   hipError_t e = prof_begin(s, r, kind, flops, bytes);
   if (e != hipSuccess) return e;
@@ -719,6 +721,46 @@ static void bind_workspace(Ctx &c, const Layout &L, void *ws) {
   c.S_h = L.S_h;
 }
 
+// ---------------------------------------------------------------------------
+// Library-owned scratch for pamd_ccsdt_energy_vjp_noalloc: the workspace of
+// pamd_ccsdt_energy_vjp in a file-scope device array, so that call allocates
+// nothing.  Each GPU holds its own copy for the life of the process; its
+// address on a device is looked up once (hipGetSymbolAddress, with that device
+// current) and kept per device.  The host-side name `scratch` is never passed
+// to a kernel or a copy: only the looked-up address is.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static void *g_scratch_ptr[kMaxDevices];
+
+// Address of this device's scratch array (the device current on the calling
+// thread), or a PAMD_* error.
+static int scratch_on_current_device(void **ptr) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    fprintf(stderr, "pyscfad_amd: hipGetDevice failed: %s\n", hipGetErrorString(e));
+    return PAMD_ERR_HIP;
+  }
+  if (dev < 0 || dev >= kMaxDevices) return PAMD_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "pyscfad_amd: hipGetSymbolAddress(scratch) failed on device %d: %s\n", dev,
+              hipGetErrorString(e));
+      return PAMD_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return PAMD_OK;
+}
+
 }  // namespace pamd
 
 using namespace pamd;
@@ -890,6 +932,36 @@ int pamd_ccsdt_energy_vjp_shard(const pamd_ccsdt_config *cfg, const double *mo_e
   return vjp_impl(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, nullptr, et_bar, workspace,
                   workspace_bytes, e_out, mo_energy_bar, t1_bar, t2_bar, ovvv_bar, ovoo_bar,
                   ovov_bar, fvo_bar, nullptr, stream, shard, nshards);
+}
+
+size_t pamd_ccsdt_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
+
+int pamd_ccsdt_energy_vjp_noalloc(const pamd_ccsdt_config *cfg, const double *mo_energy,
+                                  const double *t1, const double *t2, const double *ovvv,
+                                  const double *ovoo, const double *ovov, const double *fvo,
+                                  double et_bar, double *e_out, double *mo_energy_bar,
+                                  double *t1_bar, double *t2_bar, double *ovvv_bar,
+                                  double *ovoo_bar, double *ovov_bar, double *fvo_bar,
+                                  void *stream) {
+  int st = check_common(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo);
+  if (st != PAMD_OK) return st;
+  const Layout L = make_layout(cfg->nocc, cfg->nvir, 1, cfg->precision);
+  if (L.total > kNoAllocScratchBytes) {
+    fprintf(stderr,
+            "pyscfad_amd: pamd_ccsdt_energy_vjp_noalloc: nocc=%d nvir=%d needs %zu bytes of "
+            "scratch, more than the library's %zu; use pamd_ccsdt_energy_vjp with a workspace\n",
+            cfg->nocc, cfg->nvir, L.total, kNoAllocScratchBytes);
+    return PAMD_ERR_WORKSPACE;
+  }
+  void *ws = nullptr;
+  st = scratch_on_current_device(&ws);
+  if (st != PAMD_OK) return st;
+  g_prof_blocked = true;
+  st = vjp_impl(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, nullptr, et_bar, ws,
+                kNoAllocScratchBytes, e_out, mo_energy_bar, t1_bar, t2_bar, ovvv_bar, ovoo_bar,
+                ovov_bar, fvo_bar, nullptr, stream);
+  g_prof_blocked = false;
+  return st;
 }
 
 int pamd_ccsdt_weighted_energy_vjp(const pamd_ccsdt_config *cfg, const double *mo_energy,
