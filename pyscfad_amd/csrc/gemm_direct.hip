@@ -28,6 +28,7 @@
 // even in number, and every segment length is even (gemm_direct_ok).  Rows past
 // M or N and k past L are clamped to valid elements (finite inputs); k past L
 // is then zeroed in the A fragments, rows past M/N are never stored.
+#include <cstdlib>
 #include "ccsdt_common.h"
 #include "engine_staging.h"
 
@@ -55,36 +56,49 @@ __device__ __forceinline__ void wait_vmcnt() {
 __device__ __forceinline__ int kfast_swz(int r) { return (r ^ (r >> 3)) & 7; }
 
 // per-thread load plan of one operand (R rows) for one segment
+// When no K chunk of the segment straddles a divisor boundary of the K index
+// (kdiv a multiple of 16, or K not split), every load of a full chunk sits at
+// a per-thread pointer (row offset + its chunk-local k times ky, fixed for the
+// segment) plus one wave-uniform offset of the chunk start: ptr[q].
 template <int R>
 struct DPlan {
   static constexpr int NI = R / 64;  // 16-byte loads per thread per chunk
-  int roff[NI];                      // row offsets (elements, clamped rows)
-  int kl[NI];                        // k within the chunk of each load
+  const double *ptr[NI];             // p + row offset + kl * ky (used when `fast`)
+  bool fast;
 };
+
+// row offset (elements, clamped row) and chunk-local k of this thread's load q
+template <int R>
+__device__ __forceinline__ void dslot(const OpDesc &d, int Rtot, int r0, int tid, int q, int &roff,
+                                      int &kl) {
+  int r;
+  if (d.rfast) {
+    const int id = q * 512 + tid;
+    const int k = id / (R / 2), pc = id % (R / 2);
+    const int p = pc ^ (((k >> 2) & 3) << 3);
+    r = r0 + 2 * p;
+    if (r > Rtot - 2) r = Rtot - 2;  // Rtot is even for a row-contiguous operand
+    kl = k;
+  } else {
+    const int id = q * 512 + tid;
+    const int rl = id >> 3, pc = id & 7;
+    kl = 2 * (pc ^ kfast_swz(rl));
+    r = r0 + rl;
+    if (r > Rtot - 1) r = Rtot - 1;
+  }
+  const int qd = r / d.rdiv;
+  roff = (int)(qd * d.rx + (r - qd * d.rdiv) * d.ry);
+}
 
 template <int R>
 __device__ __forceinline__ void dplan(const OpDesc &d, int Rtot, int r0, int tid, DPlan<R> &pl) {
 #pragma unroll
   for (int q = 0; q < DPlan<R>::NI; ++q) {
-    int r, kl;
-    if (d.rfast) {
-      const int id = q * 512 + tid;
-      const int k = id / (R / 2), pc = id % (R / 2);
-      const int p = pc ^ (((k >> 2) & 3) << 3);
-      r = r0 + 2 * p;
-      if (r > Rtot - 2) r = Rtot - 2;  // Rtot is even for a row-contiguous operand
-      kl = k;
-    } else {
-      const int id = q * 512 + tid;
-      const int rl = id >> 3, pc = id & 7;
-      kl = 2 * (pc ^ kfast_swz(rl));
-      r = r0 + rl;
-      if (r > Rtot - 1) r = Rtot - 1;
-    }
-    const int qd = r / d.rdiv;
-    pl.roff[q] = (int)(qd * d.rx + (r - qd * d.rdiv) * d.ry);
-    pl.kl[q] = kl;
+    int roff, kl;
+    dslot<R>(d, Rtot, r0, tid, q, roff, kl);
+    pl.ptr[q] = d.p + roff + (long long)kl * d.ky;
   }
+  pl.fast = d.kdiv >= (1 << 30) || (d.kdiv % kDK) == 0;
 }
 
 // K offset of chunk-local k = kl at chunk start kappa (kp = divmod(kappa, kdiv)),
@@ -100,13 +114,27 @@ __device__ __forceinline__ int dkoff(const OpDesc &d, int L, int kappa, KPos kp,
   return (int)(q * d.kx + r * d.ky);
 }
 
+// The loads of one chunk.  Full chunks of a `fast` operand use the stored
+// pointers; otherwise (divisor boundary inside a chunk, or the last chunk of a
+// segment whose length is not a multiple of 16) each load's row and k are
+// recomputed and k is clamped to the segment.
 template <int R>
-__device__ __forceinline__ void dissue(const OpDesc &d, int L, int kappa, KPos kp,
-                                       const DPlan<R> &pl, double *lds, int wave) {
-  const double *p = d.p;
+__device__ __forceinline__ void dissue(const OpDesc &d, int Rtot, int r0, int tid, int L,
+                                       int kappa, KPos kp, const DPlan<R> &pl, double *lds,
+                                       int wave) {
+  if (pl.fast && kappa + kDK <= L) {
+    const long long kb = (long long)kp.q * d.kx + (long long)kp.r * d.ky;  // wave-uniform
+#pragma unroll
+    for (int q = 0; q < DPlan<R>::NI; ++q)
+      __builtin_amdgcn_global_load_lds((const void *)(pl.ptr[q] + kb),
+                                       (lds_ptr_t)(lds + (q * 512 + wave * 64) * 2), 16, 0, 0);
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < DPlan<R>::NI; ++q) {
-    const double *src = p + pl.roff[q] + dkoff(d, L, kappa, kp, pl.kl[q]);
+    int roff, kl;
+    dslot<R>(d, Rtot, r0, tid, q, roff, kl);
+    const double *src = d.p + roff + dkoff(d, L, kappa, kp, kl);
     __builtin_amdgcn_global_load_lds((const void *)src, (lds_ptr_t)(lds + (q * 512 + wave * 64) * 2),
                                      16, 0, 0);
   }
@@ -122,15 +150,15 @@ __device__ __forceinline__ void kadvance(KPos &kp, int kdiv) {
 
 }  // namespace
 
-template <int MT, int NT, int WM, int WN, bool FX2>
-__global__ __launch_bounds__(512) void k_gemm_f64d(GemmDesc d, int n_fast) {
+template <int MT, int NT, int WM, int WN, bool FX2, int NST>
+__global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, int n_fast) {
   static_assert(WM * WN == 8, "8 waves");
   constexpr int WTM = MT / WM, WTN = NT / WN;
   constexpr int TM = WTM / 16, TN = WTN / 16;
   constexpr int NIA = MT / 64, NIB = NT / 64, NI = NIA + NIB;
   constexpr int SA = MT * kDK, SB = NT * kDK, SS = SA + SB;  // doubles per stage
   constexpr int kFlush2 = 8;
-  __shared__ __align__(16) double S[kDStages * SS];
+  __shared__ __align__(16) double S[NST * SS];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;
@@ -170,54 +198,65 @@ __global__ __launch_bounds__(512) void k_gemm_f64d(GemmDesc d, int n_fast) {
   DPlan<NT> pb;
   dplan<MT>(d.seg[fs].A, d.M, m0, tid, pa);
   dplan<NT>(d.seg[fs].B, d.N, n0, tid, pb);
-  // compute tracker
-  int cs = fs, ckappa = fkappa;
+  // compute tracker (segment fields cached while the segment lasts)
+  int cs = fs, ckappa = fkappa, cend = d.chunk_start[fs + 1];
+  int la = d.seg[fs].A.rfast, lb = d.seg[fs].B.rfast, cL = d.seg[fs].L;
+  double alpha = d.seg[fs].alpha;
 
+  // fetch-side segment fields, cached while the segment lasts
+  OpDesc FA = d.seg[fs].A, FB = d.seg[fs].B;
+  int FL = d.seg[fs].L, fend = d.chunk_start[fs + 1];
   auto issue = [&](int g) {  // loads of chunk g (the fetch tracker's chunk) into its stage
-    double *st = S + ((g - g0) % kDStages) * SS;
-    const SegDesc &sg = d.seg[fs];
-    dissue<MT>(sg.A, sg.L, fkappa, fka, pa, st, wave);
-    dissue<NT>(sg.B, sg.L, fkappa, fkb, pb, st + SA, wave);
+    double *st = S + ((g - g0) % NST) * SS;
+    dissue<MT>(FA, d.M, m0, tid, FL, fkappa, fka, pa, st, wave);
+    dissue<NT>(FB, d.N, n0, tid, FL, fkappa, fkb, pb, st + SA, wave);
     // advance the tracker to chunk g + 1
     if (g + 1 < g1) {
-      if (g + 1 >= d.chunk_start[fs + 1]) {
+      if (g + 1 >= fend) {
         while (g + 1 >= d.chunk_start[fs + 1]) ++fs;  // (skips segments of length 0)
+        FA = d.seg[fs].A;
+        FB = d.seg[fs].B;
+        FL = d.seg[fs].L;
+        fend = d.chunk_start[fs + 1];
         fkappa = 0;
         fka = KPos{0, 0};
         fkb = KPos{0, 0};
-        dplan<MT>(d.seg[fs].A, d.M, m0, tid, pa);
-        dplan<NT>(d.seg[fs].B, d.N, n0, tid, pb);
+        dplan<MT>(FA, d.M, m0, tid, pa);
+        dplan<NT>(FB, d.N, n0, tid, pb);
       } else {
         fkappa += kDK;
-        kadvance(fka, d.seg[fs].A.kdiv);
-        kadvance(fkb, d.seg[fs].B.kdiv);
+        kadvance(fka, FA.kdiv);
+        kadvance(fkb, FB.kdiv);
       }
     }
   };
 
+  static_assert(NST == 2 || NST == 3, "two or three LDS stages");
   if (g0 < g1) issue(g0);
-  if (g0 + 1 < g1) issue(g0 + 1);
+  if (NST == 3 && g0 + 1 < g1) issue(g0 + 1);
 
   const int g4 = (lane >> 4) * 4;
   for (int g = g0; g < g1; ++g) {
-    if (g + 1 < g1)
+    if (NST == 3 && g + 1 < g1)
       wait_vmcnt<NI>();
     else
       wait_vmcnt<0>();
     __builtin_amdgcn_s_barrier();
-    if (g + 2 < g1) issue(g + 2);
+    if (g + NST - 1 < g1) issue(g + NST - 1);
 
     // compute chunk g
-    if (g >= d.chunk_start[cs + 1]) {
+    if (g >= cend) {
       while (g >= d.chunk_start[cs + 1]) ++cs;
       ckappa = 0;
+      cend = d.chunk_start[cs + 1];
+      la = d.seg[cs].A.rfast;
+      lb = d.seg[cs].B.rfast;
+      alpha = d.seg[cs].alpha;
+      cL = d.seg[cs].L;
     }
-    const SegDesc &sg = d.seg[cs];
-    const int la = sg.A.rfast, lb = sg.B.rfast;
-    const double alpha = sg.alpha;
-    const int kv = sg.L - ckappa;  // valid k in this chunk (>= 16: all)
+    const int kv = cL - ckappa;  // valid k in this chunk (>= 16: all)
     ckappa += kDK;
-    const double *As = S + ((g - g0) % kDStages) * SS;
+    const double *As = S + ((g - g0) % NST) * SS;
     const double *Bs = As + SA;
     double fa[4][TM], fb[4][TN];
 #pragma unroll
@@ -458,8 +497,8 @@ __global__ __launch_bounds__(512, 4) void k_gemm_b3d(GemmDesc d, int n_fast) {
   auto issue = [&](int g) {
     double *st = S + ((g - g0) % kDStages) * SS;
     const SegDesc &sg = d.seg[fs];
-    dissue<MT>(sg.A, sg.L, fkappa, fka, pa, st, wave);
-    dissue<NT>(sg.B, sg.L, fkappa, fkb, pb, st + SA, wave);
+    dissue<MT>(sg.A, d.M, m0, tid, sg.L, fkappa, fka, pa, st, wave);
+    dissue<NT>(sg.B, d.N, n0, tid, sg.L, fkappa, fkb, pb, st + SA, wave);
     if (g + 1 < g1) {
       if (g + 1 >= d.chunk_start[fs + 1]) {
         while (g + 1 >= d.chunk_start[fs + 1]) ++fs;
@@ -603,25 +642,26 @@ bool gemm_direct_ok(const GemmDesc &d) {
   return true;
 }
 
-hipError_t launch_gemm_f64_direct(const GemmDesc &d, hipStream_t s, bool fx2) {
-  if (d.M <= 0 || d.N <= 0) return hipSuccess;
+template <bool FX2, int NST>
+static void launch_f64d(const GemmDesc &d, hipStream_t s) {
   if (d.M >= d.N) {
     const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 63) / 64) * d.splits;
-    if (fx2)
-      hipLaunchKernelGGL((k_gemm_f64d<128, 64, 4, 2, true>), dim3((unsigned)tiles), dim3(512), 0, s,
-                         d, 1);
-    else
-      hipLaunchKernelGGL((k_gemm_f64d<128, 64, 4, 2, false>), dim3((unsigned)tiles), dim3(512), 0,
-                         s, d, 1);
+    hipLaunchKernelGGL((k_gemm_f64d<128, 64, 4, 2, FX2, NST>), dim3((unsigned)tiles), dim3(512), 0,
+                       s, d, 1);
   } else {
     const long long tiles = (long long)((d.M + 63) / 64) * ((d.N + 127) / 128) * d.splits;
-    if (fx2)
-      hipLaunchKernelGGL((k_gemm_f64d<64, 128, 2, 4, true>), dim3((unsigned)tiles), dim3(512), 0, s,
-                         d, 0);
-    else
-      hipLaunchKernelGGL((k_gemm_f64d<64, 128, 2, 4, false>), dim3((unsigned)tiles), dim3(512), 0,
-                         s, d, 0);
+    hipLaunchKernelGGL((k_gemm_f64d<64, 128, 2, 4, FX2, NST>), dim3((unsigned)tiles), dim3(512), 0,
+                       s, d, 0);
   }
+}
+
+hipError_t launch_gemm_f64_direct(const GemmDesc &d, hipStream_t s, bool fx2) {
+  if (d.M <= 0 || d.N <= 0) return hipSuccess;
+  // three LDS stages (two measured the same at o10/v400: 34.1 vs 33.6 TFLOP/s)
+  if (fx2)
+    launch_f64d<true, kDStages>(d, s);
+  else
+    launch_f64d<false, kDStages>(d, s);
   return hipGetLastError();
 }
 
