@@ -131,8 +131,6 @@ hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s);
 // usable when gemm_direct_ok(d); launch_gemm_f64 / launch_gemm_f32x2 pick it then
 bool gemm_direct_ok(const GemmDesc &d);
 hipError_t launch_gemm_f64_direct(const GemmDesc &d, hipStream_t s, bool fx2);
-// split-BF16x3 engine on the same direct staging (packed elements in LDS)
-hipError_t launch_gemm_b3_direct(const GemmDesc &d, hipStream_t s);
 bool gemm_direct_enabled();  // env PYSCFAD_AMD_GEMM_DIRECT=0 turns the direct engine off
 int gemm_tiles(int M, int N);     // FP64 engine: workgroup tiles of one split
 int gemm_tiles_b3(int M, int N);  // split-BF16x3 engine: workgroup tiles of one split

@@ -384,235 +384,6 @@ __global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, int n_fast) {
       }
 }
 
-// ---------------------------------------------------------------------------
-// Split-BF16x3 engine on the same direct-to-LDS staging.
-//
-// Operands are the packed 8-byte elements (bits 0-15 hi, 16-31 mid, 32-47 lo
-// bf16) that the split-BF16x3 path keeps in the FP64 arrays' layout, so the
-// staging, the LDS layout and the descriptors are exactly those of the FP64
-// engine above.  The three bf16 planes are separated when the MFMA fragments
-// are read: v_mfma_f32_32x32x16_bf16 takes 8 consecutive k of one row per
-// lane (lane l: row l % 32, k = 8 (l / 32) + 0..7), i.e. the whole K chunk of
-// 16 in one MFMA step, and each pair of packed elements gives one dword of
-// every plane (three v_perm_b32).  The six products hh, hm, mh, mm, hl, lh are
-// summed into one FP32 tile per 32 x 32 output block over the workgroup's
-// whole K range (split-K parts span at most 8192 k), then added in FP64.
-// ---------------------------------------------------------------------------
-namespace {
-
-typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
-typedef float f32x16v __attribute__((ext_vector_type(16)));
-typedef unsigned u32x4v __attribute__((ext_vector_type(4)));
-typedef unsigned long long u64x2v __attribute__((ext_vector_type(2)));
-
-struct Planes {
-  bf16x8v h, m, l;
-};
-
-__device__ __forceinline__ Planes b3d_unpack(const unsigned long long (&e)[8]) {
-  u32x4v H, M, L;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const unsigned a0 = (unsigned)e[2 * q], a1 = (unsigned)e[2 * q + 1];
-    H[q] = __builtin_amdgcn_perm(a1, a0, 0x05040100u);
-    M[q] = __builtin_amdgcn_perm(a1, a0, 0x07060302u);
-    L[q] = __builtin_amdgcn_perm((unsigned)(e[2 * q + 1] >> 32), (unsigned)(e[2 * q] >> 32),
-                                 0x05040100u);
-  }
-  Planes p;
-  p.h = __builtin_bit_cast(bf16x8v, H);
-  p.m = __builtin_bit_cast(bf16x8v, M);
-  p.l = __builtin_bit_cast(bf16x8v, L);
-  return p;
-}
-
-// the 8 packed elements (k = 8 h .. 8 h + 7) of tile row r from a staged chunk
-template <int R>
-__device__ __forceinline__ void b3d_frag(const double *Sx, int rfast, int r, int h,
-                                         unsigned long long (&e)[8]) {
-  const unsigned long long *S = reinterpret_cast<const unsigned long long *>(Sx);
-  if (rfast) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int k = 8 * h + j;
-      const int c = (r >> 1) ^ (((k >> 2) & 3) << 3);
-      e[j] = S[k * R + 2 * c + (r & 1)];
-    }
-  } else {
-    const int f = kfast_swz(r);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const u64x2v x = *(const u64x2v *)(S + r * kDK + 2 * ((4 * h + j) ^ f));
-      e[2 * j] = x[0];
-      e[2 * j + 1] = x[1];
-    }
-  }
-}
-
-}  // namespace
-
-template <int MT, int NT, int WM, int WN>
-__global__ __launch_bounds__(512, 4) void k_gemm_b3d(GemmDesc d, int n_fast) {
-  static_assert(WM * WN == 8, "8 waves");
-  constexpr int WTM = MT / WM, WTN = NT / WN;
-  constexpr int TM = WTM / 32, TN = WTN / 32;
-  static_assert(TM * 32 == WTM && TN * 32 == WTN, "32x32 MFMA tiles");
-  constexpr int NIA = MT / 64, NIB = NT / 64, NI = NIA + NIB;
-  constexpr int SA = MT * kDK, SB = NT * kDK, SS = SA + SB;  // 8-byte elements per stage
-  __shared__ __align__(16) double S[kDStages * SS];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave / WN, wn = wave % WN;
-  const int mt = (d.M + MT - 1) / MT, nt = (d.N + NT - 1) / NT;
-  const int logical = xcd_remap(blockIdx.x, gridDim.x);
-  const int split = logical / (mt * nt);
-  const int tile = logical - split * (mt * nt);
-  const int tm = n_fast ? tile / nt : tile % mt;
-  const int tn = n_fast ? tile % nt : tile / mt;
-  const int m0 = tm * MT, n0 = tn * NT;
-  const int total = d.chunk_start[d.nseg];
-  const int cps = (total + d.splits - 1) / d.splits;
-  const int g0 = split * cps;
-  const int g1 = min(total, g0 + cps);
-
-  f32x16v acc[TM][TN];
-#pragma unroll
-  for (int a = 0; a < TM; ++a)
-#pragma unroll
-    for (int b = 0; b < TN; ++b)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
-
-  int fs = 0;
-  while (fs + 1 < d.nseg && g0 >= d.chunk_start[fs + 1]) ++fs;
-  int fkappa = (g0 - d.chunk_start[fs]) * kDK;
-  KPos fka{fkappa / d.seg[fs].A.kdiv, fkappa % d.seg[fs].A.kdiv};
-  KPos fkb{fkappa / d.seg[fs].B.kdiv, fkappa % d.seg[fs].B.kdiv};
-  DPlan<MT> pa;
-  DPlan<NT> pb;
-  dplan<MT>(d.seg[fs].A, d.M, m0, tid, pa);
-  dplan<NT>(d.seg[fs].B, d.N, n0, tid, pb);
-  int cs = fs, ckappa = fkappa;
-
-  auto issue = [&](int g) {
-    double *st = S + ((g - g0) % kDStages) * SS;
-    const SegDesc &sg = d.seg[fs];
-    dissue<MT>(sg.A, d.M, m0, tid, sg.L, fkappa, fka, pa, st, wave);
-    dissue<NT>(sg.B, d.N, n0, tid, sg.L, fkappa, fkb, pb, st + SA, wave);
-    if (g + 1 < g1) {
-      if (g + 1 >= d.chunk_start[fs + 1]) {
-        while (g + 1 >= d.chunk_start[fs + 1]) ++fs;
-        fkappa = 0;
-        fka = KPos{0, 0};
-        fkb = KPos{0, 0};
-        dplan<MT>(d.seg[fs].A, d.M, m0, tid, pa);
-        dplan<NT>(d.seg[fs].B, d.N, n0, tid, pb);
-      } else {
-        fkappa += kDK;
-        kadvance(fka, d.seg[fs].A.kdiv);
-        kadvance(fkb, d.seg[fs].B.kdiv);
-      }
-    }
-  };
-
-  if (g0 < g1) issue(g0);
-  if (g0 + 1 < g1) issue(g0 + 1);
-
-  const int h = lane >> 5;
-  for (int g = g0; g < g1; ++g) {
-    if (g + 1 < g1)
-      wait_vmcnt<NI>();
-    else
-      wait_vmcnt<0>();
-    __builtin_amdgcn_s_barrier();
-    if (g + 2 < g1) issue(g + 2);
-
-    if (g >= d.chunk_start[cs + 1]) {
-      while (g >= d.chunk_start[cs + 1]) ++cs;
-      ckappa = 0;
-    }
-    const SegDesc &sg = d.seg[cs];
-    const int la = sg.A.rfast, lb = sg.B.rfast;
-    const bool neg = sg.alpha < 0.0;
-    const int kv = sg.L - ckappa;
-    ckappa += kDK;
-    const double *As = S + ((g - g0) % kDStages) * SS;
-    const double *Bs = As + SA;
-    Planes pa_[TM];
-#pragma unroll
-    for (int t = 0; t < TM; ++t) {
-      unsigned long long e[8];
-      b3d_frag<MT>(As, la, wm * WTM + t * 32 + (lane & 31), h, e);
-      if (kv < kDK) {  // k past the segment: zero the A side
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-          if (8 * h + j >= kv) e[j] = 0ull;
-      }
-      if (neg) {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) e[j] ^= 0x0000800080008000ull;
-      }
-      pa_[t] = b3d_unpack(e);
-    }
-#pragma unroll
-    for (int t = 0; t < TN; ++t) {
-      unsigned long long e[8];
-      b3d_frag<NT>(Bs, lb, wn * WTN + t * 32 + (lane & 31), h, e);
-      const Planes pb_ = b3d_unpack(e);
-#pragma unroll
-      for (int a = 0; a < TM; ++a) {
-        f32x16v &c = acc[a][t];
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pa_[a].m, pb_.m, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pa_[a].h, pb_.l, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pa_[a].l, pb_.h, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pa_[a].h, pb_.m, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pa_[a].m, pb_.h, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pa_[a].h, pb_.h, c, 0, 0, 0);
-      }
-    }
-  }
-
-  // D layout of v_mfma_f32_32x32x16_bf16: lane l, element r -> row 8 (r/4) + 4 (l/32) + r%4,
-  // column l%32
-#pragma unroll
-  for (int a = 0; a < TM; ++a)
-#pragma unroll
-    for (int b = 0; b < TN; ++b)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int ml = wm * WTM + 32 * a + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int nl = wn * WTN + 32 * b + (lane & 31);
-        const bool part = d.mode == kPartial;
-        const OutPos pm = out_pos<MT>(tm, ml, part ? (1 << 30) : d.C.mdiv);
-        const OutPos pn = out_pos<NT>(tn, nl, part ? (1 << 30) : d.C.ndiv);
-        const int m = pm.idx, n = pn.idx;
-        if (m >= d.M || n >= d.N) continue;
-        const double val = (double)acc[a][b][r];
-        if (part) {
-          d.partial[((size_t)split * d.M + m) * d.N + n] = val;
-        } else {
-          double *c = d.C.p + (long long)pm.q * d.C.mx + (long long)pm.r * d.C.my +
-                      (long long)pn.q * d.C.nx + (long long)pn.r * d.C.ny;
-          if (d.mode == kAccumulate)
-            *c += val;
-          else
-            *c = val;
-        }
-      }
-}
-
-hipError_t launch_gemm_b3_direct(const GemmDesc &d, hipStream_t s) {
-  if (d.M <= 0 || d.N <= 0) return hipSuccess;
-  if (d.M >= d.N) {
-    const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 63) / 64) * d.splits;
-    hipLaunchKernelGGL((k_gemm_b3d<128, 64, 4, 2>), dim3((unsigned)tiles), dim3(512), 0, s, d, 1);
-  } else {
-    const long long tiles = (long long)((d.M + 63) / 64) * ((d.N + 127) / 128) * d.splits;
-    hipLaunchKernelGGL((k_gemm_b3d<64, 128, 2, 4>), dim3((unsigned)tiles), dim3(512), 0, s, d, 0);
-  }
-  return hipGetLastError();
-}
-
 // An operand can be staged with 16-byte direct loads: two consecutive elements
 // along its fast index are adjacent in memory and 16-byte aligned, for every
 // row and k the kernel may address (rows clamped to [0, rows), k to [0, L)).
