@@ -254,6 +254,9 @@ def main():
     ap.add_argument('--cpu-sample', default='12,120', help='nocc,nvir of the CPU baseline sample')
     ap.add_argument('--cpu-threads', type=int, default=16)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--cpu-scaling', default=None, metavar='O,V;O,V;...',
+                    help='only time the reference C E(T)+VJP at these sizes on --cpu-threads '
+                         'threads and print their GFLOP/s (no GPU work)')
     ap.add_argument('--no-df', action='store_true',
                     help='skip the timing of the DF integral producer and its adjoint')
     ap.add_argument('--energy-only', action='store_true', help='time E(T) without the VJP')
@@ -269,6 +272,26 @@ def main():
               f'--gpus {args.gpus} is not supported.', file=sys.stderr)
         sys.exit(2)
     deadline = _T_START + args.budget
+    if args.cpu_scaling:
+        rows = []
+        for item in args.cpu_scaling.split(';'):
+            so, sv = (int(x) for x in item.split(','))
+            r = cpu_baseline(so, sv, (so, sv), args.cpu_threads)
+            if r is None:
+                print('bench.py: the reference library (oracle/_ref) is not built', file=sys.stderr)
+                sys.exit(1)
+            rows.append({'nocc': so, 'nvir': sv, 'naux': 4 * sv, 'sec': r['sample_sec'],
+                         'gflops': r['gflops']})
+            _log(f'reference C E(T)+VJP o{so}/v{sv}: {r["sample_sec"]:.2f} s, '
+                 f'{r["gflops"]:.1f} GFLOP/s')
+        model, ncpu = cpu_model()
+        line = {'cpu_scaling': rows, 'threads': args.cpu_threads, 'host_cpu': model,
+                'host_logical_cpus': ncpu, 'flops': 'E+VJP algorithmic 6 o^3 v^3 (o + v)'}
+        print(json.dumps(line), flush=True)
+        if args.out:
+            with open(args.out, 'w') as f:
+                f.write(json.dumps(line) + '\n')
+        return
 
     import torch
     from pyscfad_amd import _lib

@@ -549,7 +549,8 @@ hipError_t launch_orbit_table(long long norb, unsigned short *tab, hipStream_t s
 // positions (read from LDS), the energy term, Wbar, Vbar and the energy
 // denominators' adjoint.  Thread t handles position t of every tile, so wave w
 // holds the slab a = 8 P1 + w: the dE/d(e_virtual) sums over two of the three
-// tile axes reduce within the wave, with a few LDS atomics per wave.
+// tile axes reduce within the wave; the per-wave sums are then added over the
+// waves in a fixed order (no atomics), so every output is bitwise reproducible.
 // Modes: kOrbitPlain -- energy, and for the adjoint the Wbar/Vbar/dE/de seeds;
 // kOrbitProjAcc -- accA[f] += u_f W, accB[f] += u_f Z/D (projected walk);
 // kOrbitProjVjp -- seeds of the projected energy from the finished accA, accB
@@ -679,6 +680,12 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
     }
     __syncthreads();
     // phase 2
+    // dE/de_virtual of this orbit, per lane and per tile range r of the orbit
+    // (val[r]): lane l < 8 holds the c-axis sum for offset l, 8 <= l < 16 the
+    // b-axis sum for offset l - 8, lane 16 the a-axis sum for offset = wave;
+    // added over the orbit's tiles in program order, then summed over the waves
+    // in a fixed order below (no atomics: every output is reproducible)
+    double vacc[3] = {0.0, 0.0, 0.0};
     for (int p = 0; p < np; ++p) {
       const int A0 = val[pt.sel[p][0]] * OT, B0 = val[pt.sel[p][1]] * OT, C0 = val[pt.sel[p][2]] * OT;
       const int a = A0 + al, b = B0 + be, c = C0 + ga;
@@ -760,12 +767,45 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
         sa += __shfl_xor(sa, 8, 64);
         sa += __shfl_xor(sa, 16, 64);
         sa += __shfl_xor(sa, 32, 64);
-        if (lane < 8 && C0 + lane < v) atomicAdd(&vir[C0 + lane], -sc);
-        if ((lane & 7) == 0 && b < v) atomicAdd(&vir[b], -sb);
-        if (lane == 0 && a < v) atomicAdd(&vir[a], -sa);
+        // lane l in 8..15 takes the b-axis sum of offset l - 8 (held by lane 8 (l - 8))
+        const double sbl = __shfl(sb, 8 * ((lane - 8) & 7), 64);
+        const double t = lane < 8 ? sc : (lane < 16 ? sbl : sa);
+        const int axis = lane < 8 ? 2 : (lane < 16 ? 1 : 0);
+        const int rr = pt.sel[p][axis];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) vacc[r] += rr == r ? -t : 0.0;
       }
     }
     __syncthreads();
+    if (kVjp && (MODE == kOrbitProjVjp || A.vjp)) {
+      // stage the per-wave sums in the (now idle) W tiles, then one thread per
+      // (range, offset) of the orbit adds them over the waves, and over the
+      // ranges that name the same tile, in a fixed order
+      double *stg = &Ws[0][0];  // [wave][range][17]
+      if (lane < 17) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) stg[(wave * 3 + r) * 17 + lane] = vacc[r];
+      }
+      __syncthreads();
+      if (tid < 3 * OT) {
+        const int r = tid >> 3, off = tid & 7;
+        bool owner = true;
+        for (int r2 = 0; r2 < r; ++r2) owner = owner && val[r2] != val[r];
+        const int x = val[r] * OT + off;
+        if (owner && x < v) {
+          double sum = 0.0;
+          for (int r2 = r; r2 < 3; ++r2) {
+            if (val[r2] != val[r]) continue;
+            for (int w = 0; w < 8; ++w) {
+              const double *q = stg + (w * 3 + r2) * 17;
+              sum += q[off] + q[8 + off] + (off == w ? q[16] : 0.0);
+            }
+          }
+          vir[x] += sum;
+        }
+      }
+      __syncthreads();
+    }
   }
   // block sums (512 threads, deterministic order)
   eacc = wave_sum(eacc);
