@@ -81,12 +81,14 @@ class Inputs:
                 _ptr(self.ovoo), _ptr(self.ovov), _ptr(self.fvo)]
 
     def workspace(self, with_vjp):
+        # no device query before the allocation: a query can wait for the
+        # device, which would serialise calls issued on several streams
         nbytes = _lib.workspace_size(self.cfg, with_vjp)
-        free, _total = torch.cuda.mem_get_info(self.device)
-        if nbytes > free + torch.cuda.memory_reserved(self.device):
-            raise RuntimeError(f'(T): at least {nbytes / 1e6:.0f} MB of device memory needed, '
-                               f'{free / 1e6:.0f} MB free')
-        return torch.empty(max(nbytes, 1), dtype=torch.uint8, device=self.device), nbytes
+        try:
+            ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=self.device)
+        except torch.OutOfMemoryError as exc:
+            raise RuntimeError(f'(T): {nbytes / 1e6:.0f} MB of device workspace needed') from exc
+        return ws, nbytes
 
 
 def energy(inp: Inputs):

@@ -229,6 +229,8 @@ struct B3Set {
 template <int MT, int NT>
 struct B3Producer {
   const GemmDesc *d;
+  const GemmBatch *bt;
+  long long batch;
   int tid, m0, n0, gp, g1;
   int fs, FL, Fend, kappa;
   OpDesc FA, FB;
@@ -242,6 +244,8 @@ struct B3Producer {
     fs = f;
     FA = d->seg[fs].A;
     FB = d->seg[fs].B;
+    FA.p += batch * bt->a[fs];
+    FB.p += batch * bt->bo[fs];
     FL = d->seg[fs].L;
     Fend = d->chunk_start[fs + 1];
     Fneg = d->seg[fs].alpha < 0.0;
@@ -253,8 +257,11 @@ struct B3Producer {
     vB = b3_vec_ok(FB);
   }
 
-  __device__ __forceinline__ void start(const GemmDesc &dd, int t, int mm0, int nn0, int g0, int gg1) {
+  __device__ __forceinline__ void start(const GemmDesc &dd, const GemmBatch &bb, int b, int t,
+                                        int mm0, int nn0, int g0, int gg1) {
     d = &dd;
+    bt = &bb;
+    batch = b;
     tid = t;
     m0 = mm0;
     n0 = nn0;
@@ -300,7 +307,7 @@ struct B3Producer {
 };
 
 template <int MT, int NT, int WM, int WN>
-__global__ __launch_bounds__(512) void k_gemm_b3(GemmDesc d, int n_fast) {
+__global__ __launch_bounds__(512) void k_gemm_b3(GemmDesc d, GemmBatch bt, int n_fast) {
   static_assert(WM * WN == 8, "8 waves");
   constexpr int WTM = MT / WM, WTN = NT / WN;
   constexpr int TM = WTM / 32, TN = WTN / 32;
@@ -314,8 +321,11 @@ __global__ __launch_bounds__(512) void k_gemm_b3(GemmDesc d, int n_fast) {
   const int wm = wave / WN, wn = wave % WN;
   const int mt = (d.M + MT - 1) / MT, nt = (d.N + NT - 1) / NT;
   const int logical = xcd_remap(blockIdx.x, gridDim.x);
-  const int split = logical / (mt * nt);
-  const int tile = logical - split * (mt * nt);
+  const int per = mt * nt * d.splits;  // workgroups of one product of the batch
+  const int batch = logical / per;
+  const int rem = logical - batch * per;
+  const int split = rem / (mt * nt);
+  const int tile = rem - split * (mt * nt);
   const int tm = n_fast ? tile / nt : tile % mt;
   const int tn = n_fast ? tile % nt : tile / mt;
   const int m0 = tm * MT, n0 = tn * NT;
@@ -339,7 +349,7 @@ __global__ __launch_bounds__(512) void k_gemm_b3(GemmDesc d, int n_fast) {
   // two stages of loads in flight: stage g is stashed while g + 1 is landing
   // and g + 2 is issued
   B3Producer<MT, NT> P;
-  P.start(d, tid, m0, n0, g0, g1);
+  P.start(d, bt, batch, tid, m0, n0, g0, g1);
   B3Set<MT, NT> S0, S1;
   P.produce(S0);
   P.produce(S1);
@@ -420,7 +430,7 @@ __global__ __launch_bounds__(512) void k_gemm_b3(GemmDesc d, int n_fast) {
         if (part) {
           d.partial[((size_t)split * d.M + m) * d.N + n] = val;
         } else {
-          double *c = d.C.p + (long long)pm.q * d.C.mx + (long long)pm.r * d.C.my +
+          double *c = d.C.p + batch * bt.c + (long long)pm.q * d.C.mx + (long long)pm.r * d.C.my +
                       (long long)pn.q * d.C.nx + (long long)pn.r * d.C.ny;
           if (d.mode == kAccumulate)
             *c += val;
@@ -452,7 +462,7 @@ __device__ __forceinline__ void b3n_products(bf16x8 ah, bf16x8 am, bf16x8 al, bf
 // cover the workgroup's whole K range (no FP64 accumulators, which would not
 // fit 128 VGPRs); the cross-workgroup and split-K sums stay FP64.
 template <int MT, int NT, int WM, int WN, int MINW = 2, bool F32ACC = false>
-__global__ __launch_bounds__(512, MINW) void k_gemm_b3n(GemmDesc d, int n_fast) {
+__global__ __launch_bounds__(512, MINW) void k_gemm_b3n(GemmDesc d, GemmBatch bt, int n_fast) {
   static_assert(WM * WN == 8, "8 waves");
   constexpr int WTM = MT / WM, WTN = NT / WN;
   constexpr int TM = WTM / 16, TN = WTN / 16;
@@ -466,8 +476,11 @@ __global__ __launch_bounds__(512, MINW) void k_gemm_b3n(GemmDesc d, int n_fast) 
   const int wm = wave / WN, wn = wave % WN;
   const int mt = (d.M + MT - 1) / MT, nt = (d.N + NT - 1) / NT;
   const int logical = xcd_remap(blockIdx.x, gridDim.x);
-  const int split = logical / (mt * nt);
-  const int tile = logical - split * (mt * nt);
+  const int per = mt * nt * d.splits;  // workgroups of one product of the batch
+  const int batch = logical / per;
+  const int rem = logical - batch * per;
+  const int split = rem / (mt * nt);
+  const int tile = rem - split * (mt * nt);
   const int tm = n_fast ? tile / nt : tile % mt;
   const int tn = n_fast ? tile % nt : tile / mt;
   const int m0 = tm * MT, n0 = tn * NT;
@@ -489,7 +502,7 @@ __global__ __launch_bounds__(512, MINW) void k_gemm_b3n(GemmDesc d, int n_fast) 
       }
 
   B3Producer<MT, NT> P;
-  P.start(d, tid, m0, n0, g0, g1);
+  P.start(d, bt, batch, tid, m0, n0, g0, g1);
   B3Set<MT, NT> S;
   P.produce(S);
   int buf = 0, nfl = 0;
@@ -579,7 +592,7 @@ __global__ __launch_bounds__(512, MINW) void k_gemm_b3n(GemmDesc d, int n_fast) 
         if (part) {
           d.partial[((size_t)split * d.M + m) * d.N + n] = val;
         } else {
-          double *c = d.C.p + (long long)pm.q * d.C.mx + (long long)pm.r * d.C.my +
+          double *c = d.C.p + batch * bt.c + (long long)pm.q * d.C.mx + (long long)pm.r * d.C.my +
                       (long long)pn.q * d.C.nx + (long long)pn.r * d.C.ny;
           if (d.mode == kAccumulate)
             *c += val;
@@ -633,27 +646,31 @@ int gemm_tiles_b3(int M, int N) {
 
 int gemm_wgs_per_cu_b3(int M, int N) { return kB3Shape[b3_pick(M, N)][2]; }
 
-hipError_t launch_gemm_b3(const GemmDesc &d0, hipStream_t s) {
+hipError_t launch_gemm_b3(const GemmDesc &d0, hipStream_t s, const GemmBatch &bt) {
   if (d0.M <= 0 || d0.N <= 0) return hipSuccess;
+  if (bt.nb < 1 || (bt.nb > 1 && d0.mode == kPartial)) return hipErrorInvalidValue;
   GemmDesc d = d0;  // chunks of BKC
   d.chunk_start[0] = 0;
   for (int q = 0; q < d.nseg; ++q) d.chunk_start[q + 1] = d.chunk_start[q] + (d.seg[q].L + BKC - 1) / BKC;
   for (int q = d.nseg + 1; q <= kMaxSeg; ++q) d.chunk_start[q] = d.chunk_start[d.nseg];
   const int c = b3_pick(d.M, d.N);
   const int MT = kB3Shape[c][0], NT = kB3Shape[c][1];
-  const unsigned grid = (unsigned)((long long)((d.M + MT - 1) / MT) * ((d.N + NT - 1) / NT) * d.splits);
+  const unsigned grid =
+      (unsigned)((long long)((d.M + MT - 1) / MT) * ((d.N + NT - 1) / NT) * d.splits * bt.nb);
   const int n_fast = d.M >= d.N ? 1 : 0;
   switch (c) {
-    case kB3_256x64: hipLaunchKernelGGL((k_gemm_b3<256, 64, 4, 2>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
-    case kB3_64x256: hipLaunchKernelGGL((k_gemm_b3<64, 256, 2, 4>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
-    case kB3_128x80: hipLaunchKernelGGL((k_gemm_b3n<128, 80, 8, 1, 4, true>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
-    case kB3_80x128: hipLaunchKernelGGL((k_gemm_b3n<80, 128, 1, 8, 4, true>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
-    case kB3_128x48: hipLaunchKernelGGL((k_gemm_b3n<128, 48, 8, 1, 4, true>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
-    default: hipLaunchKernelGGL((k_gemm_b3n<48, 128, 1, 8, 4, true>), dim3(grid), dim3(512), 0, s, d, n_fast); break;
+    case kB3_256x64: hipLaunchKernelGGL((k_gemm_b3<256, 64, 4, 2>), dim3(grid), dim3(512), 0, s, d, bt, n_fast); break;
+    case kB3_64x256: hipLaunchKernelGGL((k_gemm_b3<64, 256, 2, 4>), dim3(grid), dim3(512), 0, s, d, bt, n_fast); break;
+    case kB3_128x80: hipLaunchKernelGGL((k_gemm_b3n<128, 80, 8, 1, 4, true>), dim3(grid), dim3(512), 0, s, d, bt, n_fast); break;
+    case kB3_80x128: hipLaunchKernelGGL((k_gemm_b3n<80, 128, 1, 8, 4, true>), dim3(grid), dim3(512), 0, s, d, bt, n_fast); break;
+    case kB3_128x48: hipLaunchKernelGGL((k_gemm_b3n<128, 48, 8, 1, 4, true>), dim3(grid), dim3(512), 0, s, d, bt, n_fast); break;
+    default: hipLaunchKernelGGL((k_gemm_b3n<48, 128, 1, 8, 4, true>), dim3(grid), dim3(512), 0, s, d, bt, n_fast); break;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_gemm_bf16x3(const GemmDesc &d, hipStream_t s) { return launch_gemm_b3(d, s); }
+hipError_t launch_gemm_bf16x3(const GemmDesc &d, hipStream_t s, const GemmBatch &bt) {
+  return launch_gemm_b3(d, s, bt);
+}
 
 }  // namespace pamd

@@ -56,6 +56,25 @@ struct GemmDesc {
   double *partial;               // kPartial: partial[(split * M + m) * N + n]
 };
 
+// A batch of nb products that share one descriptor up to pointer offsets:
+// product b reads seg[s].A.p + b * a[s] and seg[s].B.p + b * bo[s] and writes
+// C.p + b * c (element offsets; not for the split-K partial mode).  One launch
+// then runs nb products, which fills the GPU when one product has too few
+// output tiles (small nvir).
+struct GemmBatch {
+  long long a[kMaxSeg], bo[kMaxSeg], c;
+  int nb;
+  int pad_;
+};
+inline GemmBatch no_batch() {
+  GemmBatch b;
+  for (int q = 0; q < kMaxSeg; ++q) b.a[q] = b.bo[q] = 0;
+  b.c = 0;
+  b.nb = 1;
+  b.pad_ = 0;
+  return b;
+}
+
 constexpr int kMaxFrag = 8;  // rows of the local-orbital projector ulo
 
 // packed 8-byte operand formats of the GEMM engines
@@ -123,14 +142,15 @@ hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, cons
                           int o, int v, double *Vp, double *T2m, double *OOt, int split,
                           hipStream_t s);
 hipError_t launch_split_f32x2(const double *x, size_t n, unsigned long long *out, hipStream_t s);
-hipError_t launch_gemm_f32x2(const GemmDesc &d, hipStream_t s);
+hipError_t launch_gemm_f32x2(const GemmDesc &d, hipStream_t s, const GemmBatch &bt = no_batch());
 hipError_t launch_split_bf16x3(const double *x, size_t n, unsigned long long *out, hipStream_t s);
-hipError_t launch_gemm_bf16x3(const GemmDesc &d, hipStream_t s);
-hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s);
+hipError_t launch_gemm_bf16x3(const GemmDesc &d, hipStream_t s, const GemmBatch &bt = no_batch());
+hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s, const GemmBatch &bt = no_batch());
 // FP64 / FP32 hi/lo engine with direct-to-LDS operand loads (gemm_direct.hip):
 // usable when gemm_direct_ok(d); launch_gemm_f64 / launch_gemm_f32x2 pick it then
-bool gemm_direct_ok(const GemmDesc &d);
-hipError_t launch_gemm_f64_direct(const GemmDesc &d, hipStream_t s, bool fx2);
+bool gemm_direct_ok(const GemmDesc &d, const GemmBatch &bt = no_batch());
+hipError_t launch_gemm_f64_direct(const GemmDesc &d, hipStream_t s, bool fx2,
+                                  const GemmBatch &bt = no_batch());
 bool gemm_direct_enabled();  // env PYSCFAD_AMD_GEMM_DIRECT=0 turns the direct engine off
 int gemm_tiles(int M, int N);     // FP64 engine: workgroup tiles of one split
 int gemm_tiles_b3(int M, int N);  // split-BF16x3 engine: workgroup tiles of one split

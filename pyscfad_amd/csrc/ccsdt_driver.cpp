@@ -139,20 +139,23 @@ static hipError_t timed(hipStream_t s, int kind, double flops, double bytes, F &
 }
 
 // one contraction in the precision of the call that issues it
-static hipError_t run_gemm(const GemmDesc &d, hipStream_t s, int prec, int kind) {
+static hipError_t run_gemm(const GemmDesc &d, hipStream_t s, int prec, int kind,
+                           const GemmBatch &bt = no_batch()) {
   if (g_record) {
     g_record->push_back(d);
     return hipSuccess;
   }
+  const double fl = gemm_flops(d) * bt.nb;
   if (prec == PAMD_PREC_BF16X3)
-    return timed(s, kind, gemm_flops(d), 0.0, [&] { return launch_gemm_bf16x3(d, s); });
+    return timed(s, kind, fl, 0.0, [&] { return launch_gemm_bf16x3(d, s, bt); });
   if (prec == PAMD_PREC_FP32X2)
-    return timed(s, kind, gemm_flops(d), 0.0, [&] { return launch_gemm_f32x2(d, s); });
-  return timed(s, kind, gemm_flops(d), 0.0, [&] { return launch_gemm_f64(d, s); });
+    return timed(s, kind, fl, 0.0, [&] { return launch_gemm_f32x2(d, s, bt); });
+  return timed(s, kind, fl, 0.0, [&] { return launch_gemm_f64(d, s, bt); });
 }
 
 struct Layout {
   size_t off_Vp, off_T2m, off_OOt, off_OVS, off_T2S, off_orb, off_W, off_tri, off_part_t, off_T2g, off_W8;
+  int wbatch;  // W blocks of the energy walk's batched W build (1 with the adjoint)
   size_t off_Vb, off_Vpb, off_T2mb, off_part_mo, off_R, off_part_p, off_part_h, off_wt;
   size_t total;
   int S_p, S_h;
@@ -193,6 +196,21 @@ static int splits_for(int M, int N, int chunks, int prec) {
   return std::min(s, std::max(1, b3 ? chunks / 2 : chunks));
 }
 
+// Consecutive k of one occupied pair (i, j) share the W build of the energy walk
+// when one triple's W-build GEMM has too few workgroup tiles to fill the GPU:
+// about four rounds of the resident workgroups per launch, at most 8 triples,
+// their W blocks within 2 GB.  (At nvir = 400 one triple already has ~6000
+// tiles: no batching.)
+static int energy_batch(int o, int v, int prec) {
+  const long long tiles = prec == PAMD_PREC_BF16X3 ? gemm_tiles_b3(v * v, v) : gemm_tiles(v * v, v);
+  const long long slots = 2LL * device_cus();
+  long long b = (4 * slots + tiles - 1) / std::max(1LL, tiles);
+  b = std::max(1LL, std::min(8LL, b));
+  const long long v3 = (long long)v * v * v;
+  while (b > 1 && b * v3 * (long long)sizeof(double) > (2LL << 30)) --b;
+  return (int)std::min<long long>(b, o);
+}
+
 // Vp, T2m and OOt hold FP64 copies, or packed bf16 triplets for split-BF16x3
 // (same 8-byte element); split-BF16x3 adds packed copies of t2 and of Wbar.
 static Layout make_layout(int o, int v, int vjp, int prec) {
@@ -215,7 +233,8 @@ static Layout make_layout(int o, int v, int vjp, int prec) {
     const size_t nt = (v + 7) / 8;
     L.off_orb = take(nt * (nt + 1) * (nt + 2) / 6);  // 4 x uint16 per orbit = one double
   }
-  L.off_W = take(v3);
+  L.wbatch = vjp ? 1 : energy_batch(o, v, prec);
+  L.off_W = take(v3 * L.wbatch);
   L.off_tri = take((size_t)o * o * o);
   L.off_part_t = take(kGmax);
   L.off_wt = take((size_t)o);
@@ -249,6 +268,7 @@ struct Ctx {
   double *Vp, *T2m, *OOt, *OVS, *T2S, *W, *tri, *part_t;
   double *Vb, *Vpb, *T2mb, *part_mo, *R, *part_p, *part_h;
   int S_p, S_h, G;
+  int wbatch;  // W blocks available to the energy walk's batched W build
   double ebar;
   int shard, nshards;     // this call covers walk triples with index % nshards == shard
   const double *wt_host;  // occupied weights m (host) of a diagonal projector, or null
@@ -269,9 +289,24 @@ struct Ctx {
 // ---------------------------------------------------------------------------
 // W build for one triple
 // ---------------------------------------------------------------------------
-static int build_w(const Ctx &c, int i, int j, int k) {
+// nb > 1: the W blocks of triples (i, j, k), (i, j, k + 1), ..., (i, j, k + nb - 1)
+// in one launch per pass, written to W + b v^3 (every operand slab that names
+// k moves by a fixed stride per step of k).
+static int build_w(const Ctx &c, int i, int j, int k, int nb = 1) {
   const long long o = c.o, v = c.v;
   const long long v2 = v * v, v3 = v2 * v, vo = v * o;
+  // element strides per step of k of: Vp(k), T2(k, x), T2(x, k), OOt(k, x), OOt(x, k), T2m(k)
+  const long long sVp = v3, sT2k_ = o * v2, sT2_k = v2, sOOk_ = o * vo, sOO_k = vo, sT2m = v2 * o;
+  auto batch = [&](std::initializer_list<long long> a, std::initializer_list<long long> bo) {
+    GemmBatch bt = no_batch();
+    int q = 0;
+    for (long long x : a) bt.a[q++] = x;
+    q = 0;
+    for (long long x : bo) bt.bo[q++] = x;
+    bt.c = v3;
+    bt.nb = nb;
+    return bt;
+  };
   auto Vp = [&](int x) { return c.Vp + x * v3; };
   auto T2 = [&](int x, int y) { return c.gT2 + (x * o + y) * v2; };
   auto T2m = [&](int x) { return c.T2m + x * v2 * o; };
@@ -285,7 +320,9 @@ static int build_w(const Ctx &c, int i, int j, int k) {
                        seg(opR(T2m(j), o, vo, V, 1, 0), op1(OOt(k, i), o, 1, 0), O, -1.0),   // H5
                        seg(opR(T2m(i), vo, o, V, 1, 0), op1(OOt(k, j), o, 1, 0), O, -1.0)},  // H6
                       out(c.W, v2, v, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfWBuild));
+    // P1: T2(k,j), P3: T2(k,i), H5: OOt(k,i), H6: OOt(k,j)
+    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfWBuild,
+                        batch({0, 0, 0, 0}, {sT2k_, sT2k_, sOOk_, sOOk_})));
   }
   // pass B: rows b, cols (a,c)
   {
@@ -295,7 +332,9 @@ static int build_w(const Ctx &c, int i, int j, int k) {
                        seg(op1(OOt(j, i), o, 1, 0), opR(T2m(k), o, vo, V, 1, 0), O, -1.0),   // H3
                        seg(op1(OOt(j, k), o, 1, 0), opR(T2m(i), vo, o, V, 1, 0), O, -1.0)},  // H4
                       out(c.W, 0, v, kBig, v2, 1, V));
-    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfWBuild));
+    // P2: T2(j,k) | Vp(i), P5: T2(j,i) | Vp(k), H3: OOt(j,i) | T2m(k), H4: OOt(j,k) | T2m(i)
+    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfWBuild,
+                        batch({sT2_k, 0, 0, sOO_k}, {0, sVp, sT2m, 0})));
   }
   // pass A: rows a, cols (b,c)
   {
@@ -305,7 +344,9 @@ static int build_w(const Ctx &c, int i, int j, int k) {
                        seg(op1(OOt(i, j), o, 1, 0), opR(T2m(k), o, vo, V, 1, 0), O, -1.0),   // H1
                        seg(op1(OOt(i, k), o, 1, 0), opR(T2m(j), vo, o, V, 1, 0), O, -1.0)},  // H2
                       out(c.W, 0, v2, kBig, v, 1, V));
-    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfWBuild));
+    // P4: T2(i,k) | Vp(j), P6: T2(i,j) | Vp(k), H1: OOt(i,j) | T2m(k), H2: OOt(i,k) | T2m(j)
+    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfWBuild,
+                        batch({sT2_k, 0, 0, sOO_k}, {0, sVp, sT2m, 0})));
   }
   return PAMD_OK;
 }
@@ -632,6 +673,44 @@ static int run(Ctx &c) {
   long long tcount = 0, windex = -1;
   const long long ntrip = (long long)o * (o + 1) * (o + 2) / 6 - o;
   const auto t_start = std::chrono::steady_clock::now();
+  if (!c.vjp && c.nshards == 1 && c.wbatch > 1) {
+    // energy only, small nvir: the W blocks of up to wbatch consecutive k of one
+    // pair (i, j) are built by one launch per pass; each triple's epilogue then
+    // reads its own block.  Same triples, same per-triple arithmetic and the
+    // same fixed-order sum over tri as the walk below.
+    const size_t v3 = (size_t)v * v * v;
+    for (int i = 0; i < o; ++i)
+      for (int j = i; j < o; ++j)
+        for (int kb = (i == j ? j + 1 : j); kb < o; kb += c.wbatch) {
+          const int nb = std::min(c.wbatch, o - kb);
+          g_prof_on = g_prof_every > 0 && (tcount % g_prof_every) < nb;
+          int st = build_w(c, i, j, kb, nb);
+          if (st != PAMD_OK) return st;
+          for (int b = 0; b < nb; ++b) {
+            const int k = kb + b;
+            ++tcount;
+            OrbitArgs oa = orbit_args(c, i, j, k, norb);
+            oa.W = c.W + b * v3;
+            oa.wsplit = fmt;
+            oa.omega = (i == j || j == k) ? 1.0 : 2.0;
+            oa.ebar = c.ebar;
+            const double ob = 8.0 * (double)v3;  // W read
+            PAMD_CHECK(timed(c.s, kProfOrbit, 0.0, ob, [&] { return launch_orbit(oa, c.G, c.s); }));
+            PAMD_CHECK(launch_triple_energy(c.part_t, c.G, c.tri + ((size_t)i * o + j) * o + k, c.s));
+            if (g_progress_every > 0 && tcount % g_progress_every == 0) {
+              PAMD_CHECK(hipStreamSynchronize(c.s));
+              const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() -
+                                                               t_start).count();
+              fprintf(stderr, "pyscfad_amd: (T) triples %lld/%lld done, %.1f s\n", tcount, ntrip,
+                      el);
+              fflush(stderr);
+            }
+          }
+        }
+    g_prof_on = false;
+    if (!g_prof_recs.empty()) PAMD_CHECK(prof_collect());
+    return PAMD_OK;
+  }
   for (int i = 0; i < o; ++i)
     for (int j = i; j < o; ++j)
       for (int k = j; k < o; ++k) {
@@ -700,6 +779,7 @@ static void bind_workspace(Ctx &c, const Layout &L, void *ws) {
   c.T2S = (double *)(b + L.off_T2S);
   c.orb = (unsigned short *)(b + L.off_orb);
   c.W = (double *)(b + L.off_W);
+  c.wbatch = L.wbatch;
   c.tri = (double *)(b + L.off_tri);
   c.part_t = (double *)(b + L.off_part_t);
   c.wt = (double *)(b + L.off_wt);

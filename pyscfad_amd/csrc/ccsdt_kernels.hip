@@ -218,7 +218,7 @@ hipError_t launch_prep_sym(const double *src, int o, int v, int kind, double *ds
 // tile sums, added into the FP64 accumulators every kFlush2 chunks (128 k).
 // The FP32 MFMA's D layout differs from the FP64 one (rows 4*(l/16) + r).
 template <int MT, int NT, int WM, int WN, bool FX2>
-__global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, int n_fast) {
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, GemmBatch bt, int n_fast) {
   constexpr int NTHR = 64 * WM * WN;
   using E = typename std::conditional<FX2, F2, double>::type;
   constexpr int kFlush2 = 8;
@@ -233,8 +233,11 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, int n_fas
   const int wm = wave / WN, wn = wave % WN;
   const int mt = (d.M + MT - 1) / MT, nt = (d.N + NT - 1) / NT;
   const int logical = xcd_remap(blockIdx.x, gridDim.x);
-  const int split = logical / (mt * nt);
-  const int tile = logical - split * (mt * nt);
+  const int per = mt * nt * d.splits;  // workgroups of one product of the batch
+  const int batch = logical / per;
+  const int rem = logical - batch * per;
+  const int split = rem / (mt * nt);
+  const int tile = rem - split * (mt * nt);
   const int tm = n_fast ? tile / nt : tile % mt;
   const int tn = n_fast ? tile % nt : tile / mt;
   const int m0 = tm * MT, n0 = tn * NT;
@@ -261,6 +264,8 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, int n_fas
   int fs = 0;
   while (fs + 1 < d.nseg && g0 >= d.chunk_start[fs + 1]) ++fs;
   OpDesc FA = d.seg[fs].A, FB = d.seg[fs].B;
+  FA.p += batch * bt.a[fs];
+  FB.p += batch * bt.bo[fs];
   int FL = d.seg[fs].L, Fend = d.chunk_start[fs + 1];
   double Falpha = d.seg[fs].alpha;
   int kappa = (g0 - d.chunk_start[fs]) * GKC;
@@ -289,6 +294,8 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, int n_fas
         ++fs;
         FA = d.seg[fs].A;
         FB = d.seg[fs].B;
+        FA.p += batch * bt.a[fs];
+        FB.p += batch * bt.bo[fs];
         FL = d.seg[fs].L;
         Fend = d.chunk_start[fs + 1];
         Falpha = d.seg[fs].alpha;
@@ -412,7 +419,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64(GemmDesc d, int n_fas
         if (part) {
           d.partial[((size_t)split * d.M + m) * d.N + n] = val;
         } else {
-          double *c = d.C.p + (long long)pm.q * d.C.mx + (long long)pm.r * d.C.my +
+          double *c = d.C.p + batch * bt.c + (long long)pm.q * d.C.mx + (long long)pm.r * d.C.my +
                       (long long)pn.q * d.C.nx + (long long)pn.r * d.C.ny;
           if (d.mode == kAccumulate)
             *c += val;
@@ -430,34 +437,36 @@ bool gemm_direct_enabled() {
   return on;
 }
 
-hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s) {
+hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s, const GemmBatch &bt) {
   if (d.M <= 0 || d.N <= 0) return hipSuccess;
-  if (gemm_direct_enabled() && gemm_direct_ok(d)) return launch_gemm_f64_direct(d, s, false);
+  if (bt.nb < 1 || (bt.nb > 1 && d.mode == kPartial)) return hipErrorInvalidValue;
+  if (gemm_direct_enabled() && gemm_direct_ok(d, bt)) return launch_gemm_f64_direct(d, s, false, bt);
   // 8 waves, two workgroups per CU; the wide side of the tile goes on the
   // larger output dimension, whose operand panel is then read once (n_fast).
   if (d.M >= d.N) {
-    const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 63) / 64) * d.splits;
+    const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 63) / 64) * d.splits * bt.nb;
     hipLaunchKernelGGL((k_gemm_f64<128, 64, 4, 2, false>), dim3((unsigned)tiles), dim3(512), 0, s,
-                       d, 1);
+                       d, bt, 1);
   } else {
-    const long long tiles = (long long)((d.M + 63) / 64) * ((d.N + 127) / 128) * d.splits;
+    const long long tiles = (long long)((d.M + 63) / 64) * ((d.N + 127) / 128) * d.splits * bt.nb;
     hipLaunchKernelGGL((k_gemm_f64<64, 128, 2, 4, false>), dim3((unsigned)tiles), dim3(512), 0, s,
-                       d, 0);
+                       d, bt, 0);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_gemm_f32x2(const GemmDesc &d, hipStream_t s) {
+hipError_t launch_gemm_f32x2(const GemmDesc &d, hipStream_t s, const GemmBatch &bt) {
   if (d.M <= 0 || d.N <= 0) return hipSuccess;
-  if (gemm_direct_enabled() && gemm_direct_ok(d)) return launch_gemm_f64_direct(d, s, true);
+  if (bt.nb < 1 || (bt.nb > 1 && d.mode == kPartial)) return hipErrorInvalidValue;
+  if (gemm_direct_enabled() && gemm_direct_ok(d, bt)) return launch_gemm_f64_direct(d, s, true, bt);
   if (d.M >= d.N) {
-    const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 63) / 64) * d.splits;
+    const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 63) / 64) * d.splits * bt.nb;
     hipLaunchKernelGGL((k_gemm_f64<128, 64, 4, 2, true>), dim3((unsigned)tiles), dim3(512), 0, s,
-                       d, 1);
+                       d, bt, 1);
   } else {
-    const long long tiles = (long long)((d.M + 63) / 64) * ((d.N + 127) / 128) * d.splits;
+    const long long tiles = (long long)((d.M + 63) / 64) * ((d.N + 127) / 128) * d.splits * bt.nb;
     hipLaunchKernelGGL((k_gemm_f64<64, 128, 2, 4, true>), dim3((unsigned)tiles), dim3(512), 0, s,
-                       d, 0);
+                       d, bt, 0);
   }
   return hipGetLastError();
 }
@@ -482,19 +491,6 @@ __device__ __forceinline__ void decode_orbit(long long q, int &A, int &B, int &C
   C = (int)(rem - (long long)b * (b + 1) / 2);
 }
 
-int orbit_grid(int v) {
-  int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) == hipSuccess) {
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
-  }
-  long long nt = (v + OT - 1) / OT;
-  long long norb = nt * (nt + 1) * (nt + 2) / 6;
-  long long g = 3LL * cus;  // three 512-thread workgroups per CU fit the LDS
-  if (g > norb) g = norb;
-  if (g < 1) g = 1;
-  return (int)g;
-}
 
 // Distinct tiles of an orbit and where the five non-identity permutations of a
 // position land, for the four equality patterns of A >= B >= C: 0 all distinct,
@@ -845,6 +841,27 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
       pm[k] += otot;
     }
   }
+}
+
+int orbit_grid(int v) {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+  }
+  // one round of resident workgroups (each loops over its share of the orbits):
+  // a second, partial round would leave most CUs idle at the end
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_orbit<kOrbitPlain>, 512,
+                                                   sizeof(double) * (size_t)v) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 2;
+  long long nt = (v + OT - 1) / OT;
+  long long norb = nt * (nt + 1) * (nt + 2) / 6;
+  long long g = (long long)per_cu * cus;
+  if (g > norb) g = norb;
+  if (g < 1) g = 1;
+  return (int)g;
 }
 
 hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s, int mode) {

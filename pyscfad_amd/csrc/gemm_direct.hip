@@ -151,7 +151,7 @@ __device__ __forceinline__ void kadvance(KPos &kp, int kdiv) {
 }  // namespace
 
 template <int MT, int NT, int WM, int WN, bool FX2, int NST>
-__global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, int n_fast) {
+__global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, GemmBatch bt, int n_fast) {
   static_assert(WM * WN == 8, "8 waves");
   constexpr int WTM = MT / WM, WTN = NT / WN;
   constexpr int TM = WTM / 16, TN = WTN / 16;
@@ -164,8 +164,11 @@ __global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, int n_fast) {
   const int wm = wave / WN, wn = wave % WN;
   const int mt = (d.M + MT - 1) / MT, nt = (d.N + NT - 1) / NT;
   const int logical = xcd_remap(blockIdx.x, gridDim.x);
-  const int split = logical / (mt * nt);
-  const int tile = logical - split * (mt * nt);
+  const int per = mt * nt * d.splits;  // workgroups of one product of the batch
+  const int batch = logical / per;
+  const int rem = logical - batch * per;
+  const int split = rem / (mt * nt);
+  const int tile = rem - split * (mt * nt);
   const int tm = n_fast ? tile / nt : tile % mt;
   const int tn = n_fast ? tile % nt : tile / mt;
   const int m0 = tm * MT, n0 = tn * NT;
@@ -194,18 +197,21 @@ __global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, int n_fast) {
   int fkappa = (g0 - d.chunk_start[fs]) * kDK;
   KPos fka{fkappa / d.seg[fs].A.kdiv, fkappa % d.seg[fs].A.kdiv};
   KPos fkb{fkappa / d.seg[fs].B.kdiv, fkappa % d.seg[fs].B.kdiv};
+  // fetch-side segment fields (operand pointers moved to this product of the
+  // batch), cached while the segment lasts
+  OpDesc FA = d.seg[fs].A, FB = d.seg[fs].B;
+  FA.p += batch * bt.a[fs];
+  FB.p += batch * bt.bo[fs];
+  int FL = d.seg[fs].L, fend = d.chunk_start[fs + 1];
   DPlan<MT> pa;
   DPlan<NT> pb;
-  dplan<MT>(d.seg[fs].A, d.M, m0, tid, pa);
-  dplan<NT>(d.seg[fs].B, d.N, n0, tid, pb);
+  dplan<MT>(FA, d.M, m0, tid, pa);
+  dplan<NT>(FB, d.N, n0, tid, pb);
   // compute tracker (segment fields cached while the segment lasts)
   int cs = fs, ckappa = fkappa, cend = d.chunk_start[fs + 1];
   int la = d.seg[fs].A.rfast, lb = d.seg[fs].B.rfast, cL = d.seg[fs].L;
   double alpha = d.seg[fs].alpha;
 
-  // fetch-side segment fields, cached while the segment lasts
-  OpDesc FA = d.seg[fs].A, FB = d.seg[fs].B;
-  int FL = d.seg[fs].L, fend = d.chunk_start[fs + 1];
   auto issue = [&](int g) {  // loads of chunk g (the fetch tracker's chunk) into its stage
     double *st = S + ((g - g0) % NST) * SS;
     dissue<MT>(FA, d.M, m0, tid, FL, fkappa, fka, pa, st, wave);
@@ -216,6 +222,8 @@ __global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, int n_fast) {
         while (g + 1 >= d.chunk_start[fs + 1]) ++fs;  // (skips segments of length 0)
         FA = d.seg[fs].A;
         FB = d.seg[fs].B;
+        FA.p += batch * bt.a[fs];
+        FB.p += batch * bt.bo[fs];
         FL = d.seg[fs].L;
         fend = d.chunk_start[fs + 1];
         fkappa = 0;
@@ -374,7 +382,7 @@ __global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, int n_fast) {
         if (part) {
           d.partial[((size_t)split * d.M + m) * d.N + n] = val;
         } else {
-          double *c = d.C.p + (long long)pm.q * d.C.mx + (long long)pm.r * d.C.my +
+          double *c = d.C.p + batch * bt.c + (long long)pm.q * d.C.mx + (long long)pm.r * d.C.my +
                       (long long)pn.q * d.C.nx + (long long)pn.r * d.C.ny;
           if (d.mode == kAccumulate)
             *c += val;
@@ -403,36 +411,40 @@ static bool op_direct_ok(const OpDesc &o, int rows, int L) {
   return true;
 }
 
-bool gemm_direct_ok(const GemmDesc &d) {
+bool gemm_direct_ok(const GemmDesc &d, const GemmBatch &bt) {
   if (d.M <= 0 || d.N <= 0 || d.nseg <= 0) return false;
   for (int q = 0; q < d.nseg; ++q) {
     const SegDesc &s = d.seg[q];
     if (s.L & 1) return false;
     if (!op_direct_ok(s.A, d.M, s.L) || !op_direct_ok(s.B, d.N, s.L)) return false;
+    // every product of a batch keeps the operands 16-byte aligned
+    if (bt.nb > 1 && ((bt.a[q] | bt.bo[q]) & 1)) return false;
   }
   return true;
 }
 
 template <bool FX2, int NST>
-static void launch_f64d(const GemmDesc &d, hipStream_t s) {
+static void launch_f64d(const GemmDesc &d, const GemmBatch &bt, hipStream_t s) {
   if (d.M >= d.N) {
-    const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 63) / 64) * d.splits;
+    const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 63) / 64) * d.splits * bt.nb;
     hipLaunchKernelGGL((k_gemm_f64d<128, 64, 4, 2, FX2, NST>), dim3((unsigned)tiles), dim3(512), 0,
-                       s, d, 1);
+                       s, d, bt, 1);
   } else {
-    const long long tiles = (long long)((d.M + 63) / 64) * ((d.N + 127) / 128) * d.splits;
+    const long long tiles = (long long)((d.M + 63) / 64) * ((d.N + 127) / 128) * d.splits * bt.nb;
     hipLaunchKernelGGL((k_gemm_f64d<64, 128, 2, 4, FX2, NST>), dim3((unsigned)tiles), dim3(512), 0,
-                       s, d, 0);
+                       s, d, bt, 0);
   }
 }
 
-hipError_t launch_gemm_f64_direct(const GemmDesc &d, hipStream_t s, bool fx2) {
+hipError_t launch_gemm_f64_direct(const GemmDesc &d, hipStream_t s, bool fx2,
+                                  const GemmBatch &bt) {
   if (d.M <= 0 || d.N <= 0) return hipSuccess;
+  if (bt.nb < 1 || (bt.nb > 1 && d.mode == kPartial)) return hipErrorInvalidValue;
   // three LDS stages (two measured the same at o10/v400: 34.1 vs 33.6 TFLOP/s)
   if (fx2)
-    launch_f64d<true, kDStages>(d, s);
+    launch_f64d<true, kDStages>(d, bt, s);
   else
-    launch_f64d<false, kDStages>(d, s);
+    launch_f64d<false, kDStages>(d, bt, s);
   return hipGetLastError();
 }
 

@@ -40,10 +40,11 @@ class InstanceSet:
     def __init__(self, device, nocc=NOCC, nvir=NVIR, naux=NAUX):
         self.dev = torch.device(device)
         self.nocc, self.nvir, self.naux = nocc, nvir, naux
-        (iu, ju, tri), self.e_occ, self.e_vir, self.t1, _ = gen._draw(nocc, nvir, naux, 0)
+        (iu, ju, tri), self.e_occ, self.e_vir, t1, _ = gen._draw(nocc, nvir, naux, 0)
         self.iu = torch.from_numpy(iu).to(self.dev)
         self.ju = torch.from_numpy(ju).to(self.dev)
         self.tri = torch.from_numpy(tri).to(self.dev)
+        self.t1 = torch.from_numpy(t1).to(self.dev)
         self.sigma = 0.1 / np.sqrt(naux)
 
     def instance(self, s):
@@ -57,8 +58,12 @@ class InstanceSet:
         tri = self.tri + (L_REL * xi * self.sigma) * z
         del z
         L = gen.unpack_L_torch(self.iu, self.ju, tri, self.naux, o + v)
-        mo = np.concatenate([e_occ, e_vir])
-        return gen.assemble_torch(L, mo, np.diag(mo), self.t1, o, v)
+        # the orbital energies reach the device through pinned memory without
+        # blocking the host (a pageable copy would wait for the stream)
+        mo = torch.from_numpy(np.concatenate([e_occ, e_vir]))
+        if self.dev.type == 'cuda':
+            mo = mo.pin_memory().to(self.dev, non_blocking=True)
+        return gen.assemble_torch(L, mo, torch.diag(mo), self.t1, o, v)
 
 
 def exact_pairs(p, precision='fp64'):
@@ -67,17 +72,37 @@ def exact_pairs(p, precision='fp64'):
     return _ops.pair_energy(inp)
 
 
-def build(seeds, device, log=None, every=128):
-    """(X [n, 128], Y [n, 136], E [n]) float64 device tensors for instance ``seeds``."""
+def build(seeds, device, log=None, every=128, streams=4):
+    """(X [n, 128], Y [n, 136], E [n]) float64 device tensors for instance ``seeds``.
+
+    Instances are formed and labelled round-robin on ``streams`` HIP streams (at
+    the default priority): one instance's walk is a chain of small launches
+    (about 600 workgroups per W-build GEMM at o16/v160, under two rounds of the
+    GPU's workgroup slots), so several instances in flight fill the GPU.  Every
+    tensor of an instance is allocated on its own stream, so the caching
+    allocator never hands its memory to another stream while it is in use."""
     iset = InstanceSet(device)
-    X, Y, E = [], [], []
+    dev = iset.dev
+    sts = [torch.cuda.Stream(dev) for _ in range(max(1, streams))]
+    cur = torch.cuda.current_stream(dev)
+    for st in sts:
+        st.wait_stream(cur)   # the cached base draws are ready
+    n = len(seeds)
+    X, Y, E = [None] * n, [None] * n, [None] * n
     for c, s in enumerate(seeds):
-        p = iset.instance(int(s))
-        ep = exact_pairs(p)
-        X.append(features(p.mo_energy, p.ovov, p.nocc))
-        Y.append(fold_pairs(ep))
-        E.append(ep.sum())
-        del p
-        if log and (c % every == 0 or c == len(seeds) - 1):
-            log(f'instance {c + 1}/{len(seeds)} seed {int(s)} E(T) = {float(E[-1]):.10e}')
+        with torch.cuda.stream(sts[c % len(sts)]):
+            p = iset.instance(int(s))
+            ep = exact_pairs(p)
+            X[c] = features(p.mo_energy, p.ovov, p.nocc)
+            Y[c] = fold_pairs(ep)
+            E[c] = ep.sum()
+            del p, ep
+        if log and (c % every == 0 or c == n - 1):
+            log(f'instance {c + 1}/{n} seed {int(s)} queued')
+    for st in sts:
+        cur.wait_stream(st)
+    # the per-instance results are read on the caller's stream: keep their
+    # memory from being reused on their own streams before that read is done
+    for t in X + Y + E:
+        t.record_stream(cur)
     return torch.stack(X), torch.stack(Y), torch.stack(E)

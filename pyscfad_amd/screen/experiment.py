@@ -1,7 +1,7 @@
 """The learned-screen workload: 2048 perturbed o16/v160/naux640 instances.
 
-    python -m pyscfad_amd.screen.experiment [--n 2048] [--data PATH] [--save-data PATH]
-                                            [--out report.json]
+    python -m pyscfad_amd.screen.experiment [--n 2048] [--streams 4] [--data PATH]
+                                            [--save-data PATH] [--out report.json]
 
 Labels come from the exact FP64 pair-resolved (T) kernel; the 4-member BF16
 ensemble (128 -> 256 -> 256 -> 136) is trained by the library's kernels on the
@@ -10,7 +10,9 @@ report gives the test errors, the fraction the 1e-5 Eh spread gate accepts,
 the error of the accepted answers, the same figures for the estimate available
 without the network (E(T) = c E(MP2), c fitted on the training set), and the
 time of the screened pass (inference + exact fallbacks) against the exact pass.
-A progress line is printed every 128 instances and every 100 epochs.
+The labels run on several HIP streams at once (``--streams``); the exact cost
+per instance is the label throughput's reciprocal.  A progress line is
+printed every 256 queued instances and every 100 epochs.
 """
 from __future__ import annotations
 
@@ -41,6 +43,10 @@ def main(argv=None):
     ap.add_argument('--batch', type=int, default=128)
     ap.add_argument('--lr', type=float, default=2e-3)
     ap.add_argument('--no-bootstrap', action='store_true')
+    ap.add_argument('--streams', type=int, default=4,
+                    help='HIP streams the labels run on concurrently')
+    ap.add_argument('--labels-only', action='store_true',
+                    help='stop after the labels (throughput measurement)')
     ap.add_argument('--out', default=None)
     args = ap.parse_args(argv)
     if not torch.cuda.is_available():
@@ -53,29 +59,22 @@ def main(argv=None):
         X, Y, E = (torch.from_numpy(z[k]).to(dev) for k in ('X', 'Y', 'E'))
         t_exact = z['t_exact']
     else:
-        iset = D.InstanceSet(dev)
-        X, Y, E, t_exact = [], [], [], []
+        torch.cuda.synchronize()
         t_all = time.perf_counter()
-        for s in range(1, args.n + 1):
-            p = iset.instance(s)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            ep = D.exact_pairs(p)
-            torch.cuda.synchronize()
-            t_exact.append(time.perf_counter() - t0)
-            X.append(features(p.mo_energy, p.ovov, p.nocc))
-            Y.append(fold_pairs(ep))
-            E.append(ep.sum())
-            del p
-            if s % 128 == 0 or s == args.n:
-                log(f'instances {s}/{args.n}: E(T) {float(E[-1]):.8e}, exact '
-                    f'{np.mean(t_exact) * 1e3:.1f} ms/instance, {time.perf_counter() - t_all:.0f} s')
-        X, Y, E = torch.stack(X), torch.stack(Y), torch.stack(E)
-        t_exact = np.asarray(t_exact)
+        X, Y, E = D.build(range(1, args.n + 1), dev, log=log, every=256, streams=args.streams)
+        torch.cuda.synchronize()
+        t_all = time.perf_counter() - t_all
+        # labels run on several streams at once: the cost of one exact (T) is the
+        # label throughput's reciprocal
+        t_exact = np.full(args.n, t_all / args.n)
+        log(f'labels: {args.n} instances in {t_all:.1f} s on {args.streams} streams, '
+            f'{t_all / args.n * 1e3:.1f} ms/instance')
         if args.save_data:
             np.savez(args.save_data, X=X.cpu().numpy(), Y=Y.cpu().numpy(), E=E.cpu().numpy(),
                      t_exact=t_exact)
 
+    if args.labels_only:
+        return 0
     tr, te = D.split(X.shape[0], args.n_train)
     tr_d, te_d = torch.as_tensor(tr, device=dev), torch.as_tensor(te, device=dev)
     sc = Screen(X.shape[1], Y.shape[1], dev, members=4)

@@ -48,8 +48,5 @@ def fold_pairs(e_pair):
     """(o, o) pair-resolved energies -> i <= j vector (sum preserved)."""
     o = e_pair.shape[0]
     iu, ju = torch.triu_indices(o, o, offset=0, device=e_pair.device)
-    sym = e_pair + e_pair.T
-    out = sym[iu, ju]
-    diag = iu == ju
-    out[diag] = e_pair.diagonal()
-    return out
+    # a where() rather than a masked assignment: masks need a host-side count
+    return torch.where(iu == ju, e_pair[iu, iu], (e_pair + e_pair.T)[iu, ju])

@@ -181,9 +181,14 @@ def test_screen_gain_small_instances(cuda):
           f'{float(err_mp2.mean()):.3e}); accepted {frac:.2f} with max err '
           f'{float(err[acc].max()) if acc.any() else float("nan"):.3e}; exact '
           f'{t_exact / n * 1e3:.2f} ms/instance, screen {t_screen / len(te) * 1e3:.3f} ms/instance')
+    n_over = int(((err > 1e-5) & acc).sum())
+    print(f'accepted above the 1e-5 Eh gate: {n_over} of {int(acc.sum())}')
     assert float(err.mean()) < 0.5 * float(err_mp2.mean())
     assert frac >= 0.5
-    assert float(err[acc].max()) <= 5e-5
+    # the spread is an estimate, not a bound: a small measured fraction of the
+    # accepted answers may exceed the gate, none by more than a factor of two
+    assert n_over <= 0.05 * int(acc.sum())
+    assert float(err[acc].max()) <= 2e-5
     exact_cost = t_exact / n * len(te)
     screened_cost = t_screen + (1 - frac) * exact_cost
     assert screened_cost < 0.75 * exact_cost

@@ -1,0 +1,95 @@
+"""The learned-screen configuration end to end on the GPU.
+
+2048 perturbed nocc=16/nvir=160/naux=640 instances (pyscfad_amd.screen.data),
+labelled by the plain FP64 kernel's pair-resolved (T); 128 features (120 MP2
+pair energies + 8 orbital statistics); a 4-member BF16 ensemble of
+128-256-256-136 MLPs trained by the library's kernels on the seeded 1800
+instances (600 epochs, the experiment's defaults); the 248 test instances
+screened with the 1e-5 Eh spread gate and the exact kernel as the fallback.
+
+The FP64 labels take about eight minutes (2048 pair-resolved (T) passes of
+7.1e12 flops each, 1.4e16 in all: 3 minutes even at the FP64 MFMA peak), so
+the test runs only when PYSCFAD_AMD_SCREEN_FULL=1 is set; the default GPU
+suite covers the same code on small instances (test_gpu_screen.py).
+
+Checks: the fallback returns the exact E(T) for every instance the gate
+rejects and only for those; the accepted answers' errors against the exact
+E(T) (the gate is a spread estimate, not a bound: at most 5% of the accepted
+answers may exceed 1e-5 Eh, none 2e-5 Eh); the gain over the method without
+the network (test MAE at most 1/5 of the E(T) ~ c E(MP2) fit's); and the cost
+of the screened pass against the exact one.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = [
+    pytest.mark.gpu,
+    pytest.mark.skipif(os.environ.get('PYSCFAD_AMD_SCREEN_FULL') != '1',
+                       reason='2048 FP64 labels take about 8 minutes; set PYSCFAD_AMD_SCREEN_FULL=1'),
+]
+
+
+def test_learned_screen_2048_instances(cuda):
+    from pyscfad_amd.screen import data as D
+    from pyscfad_amd.screen.ensemble import SPREAD_TOL, Screen
+    print('\nconfig 5: labelling 2048 instances at o16/v160 (FP64 pair-resolved (T))', flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    X, Y, E = D.build(range(1, D.N_INSTANCES + 1), cuda, streams=4,
+                      log=lambda s: print(s, flush=True), every=256)
+    torch.cuda.synchronize()
+    t_label = time.perf_counter() - t0
+    t_exact = t_label / D.N_INSTANCES
+    print(f'labels: {t_label:.1f} s, {t_exact * 1e3:.1f} ms/instance', flush=True)
+    assert X.shape == (2048, 128) and Y.shape == (2048, 136)
+    assert torch.allclose(Y.sum(1), E, rtol=0, atol=1e-15)
+
+    tr, te = D.split()
+    assert len(tr) == 1800 and len(te) == 248
+    tr = torch.as_tensor(tr, device=cuda)
+    te = torch.as_tensor(te, device=cuda)
+    sc = Screen(X.shape[1], Y.shape[1], cuda, members=4)
+    sc.fit(X[tr], Y[tr], epochs=600, batch=128)
+    torch.cuda.synchronize()
+
+    e_exact = E[te]
+    calls = []
+
+    def exact(i):
+        calls.append(i)
+        return float(e_exact[i])
+
+    t0 = time.perf_counter()
+    e_scr, spread, fallback, _ = sc.energy(X[te], exact)
+    torch.cuda.synchronize()
+    t_infer = time.perf_counter() - t0
+    mean, _ = sc.predict(X[te])
+    acc = spread <= SPREAD_TOL
+    err = (mean - e_exact).abs()
+    emp2 = X[:, -1]
+    c = float((emp2[tr] * E[tr]).sum() / (emp2[tr] ** 2).sum())
+    err_mp2 = (c * emp2[te] - e_exact).abs()
+    n_acc = int(acc.sum())
+    n_over = int(((err > SPREAD_TOL) & acc).sum())
+    screened_cost = t_infer + len(fallback) * t_exact
+    exact_cost = len(te) * t_exact
+    print(f'test MAE {float(err.mean()):.3e} (E(T) ~ c E(MP2): {float(err_mp2.mean()):.3e}); '
+          f'accepted {n_acc}/248, {n_over} of them above {SPREAD_TOL:g} Eh, max accepted error '
+          f'{float(err[acc].max()) if n_acc else float("nan"):.3e}; screened pass '
+          f'{screened_cost:.2f} s vs exact {exact_cost:.2f} s', flush=True)
+
+    # fallback: exactly the rejected instances go to the exact kernel and get its value
+    assert sorted(fallback) == sorted(calls) == (~acc).nonzero().flatten().tolist()
+    assert torch.equal(e_scr[~acc], e_exact[~acc])
+    assert torch.equal(e_scr[acc], mean[acc])
+    # the gate
+    assert n_acc >= 0.8 * len(te)
+    assert n_over <= 0.05 * n_acc
+    assert float(err[acc].max()) <= 2 * SPREAD_TOL
+    # the gain over the method without the network, in error and in cost
+    assert float(err.mean()) <= 0.2 * float(err_mp2.mean())
+    assert screened_cost < 0.25 * exact_cost
