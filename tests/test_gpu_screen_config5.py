@@ -39,9 +39,14 @@ def test_learned_screen_2048_instances(cuda):
     print('\nconfig 5: labelling 2048 instances at o16/v160 (FP64 pair-resolved (T))', flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    X, Y, E = D.build(range(1, D.N_INSTANCES + 1), cuda, streams=4,
-                      log=lambda s: print(s, flush=True), every=256)
-    torch.cuda.synchronize()
+    parts = []
+    for c0 in range(1, D.N_INSTANCES + 1, 256):
+        parts.append(D.build(range(c0, c0 + 256), cuda, streams=4))
+        torch.cuda.synchronize()
+        print(f'instances {c0 + 255}/{D.N_INSTANCES} labelled, '
+              f'{time.perf_counter() - t0:.0f} s', flush=True)
+    X, Y, E = (torch.cat(z) for z in zip(*parts))
+    del parts
     t_label = time.perf_counter() - t0
     t_exact = t_label / D.N_INSTANCES
     print(f'labels: {t_label:.1f} s, {t_exact * 1e3:.1f} ms/instance', flush=True)
