@@ -1,15 +1,15 @@
-// Split-BF16x3 contraction engine for gfx950.
+// Split-BF16x3 contraction engine for gfx950 (register-staged form).
 //
 // Every FP64 operand is split once into three bf16 pieces x ~ hi + mid + lo
-// (hi = bf16(x), mid = bf16(x - hi), lo = bf16(x - hi - mid)), packed in one
-// 8-byte element (bits 0-15 hi, 16-31 mid, 32-47 lo).  The packed arrays keep
-// the FP64 arrays' element layout, so the contraction descriptors are the same
-// as for the FP64 engine.  A product is evaluated as the six bf16 MFMA terms
+// (hi = bf16(x), mid = bf16(x - hi), lo = bf16(x - hi - mid)), stored as three
+// bf16 planes with the FP64 array's element layout (OpDesc::ps apart), so the
+// contraction descriptors are the same as for the FP64 engine.  A product is
+// evaluated as the six bf16 MFMA terms
 //   hi*hi + (hi*mid + mid*hi) + (hi*lo + lo*hi + mid*mid)
-// (dropped: mid*lo, lo*mid, lo*lo, all below 2^-24 relative) with
-// v_mfma_f32_32x32x16_bf16 into two FP32 tile accumulators (the large and the
-// small terms), which are added into FP64 accumulators after every K chunk of
-// 16: FP32 accumulation per tile, FP64 across tiles and chunks.
+// (dropped: mid*lo, lo*mid, lo*lo, all below 2^-24 relative) with FP32
+// accumulation per tile.  Descriptors whose operands allow 16-byte direct
+// loads run on the direct-to-LDS form (gemm_b3p.hip); this one takes the rest
+// (odd sizes and strides).
 #include "ccsdt_common.h"
 #include "engine_staging.h"
 #include "bf16_split.h"
@@ -20,14 +20,15 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+// x (n doubles) -> three bf16 planes of n at out, out + n, out + 2 n
 __global__ void k_split_bf16x3(const double *__restrict__ x, size_t n,
-                               unsigned long long *__restrict__ out) {
+                               unsigned short *__restrict__ out) {
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (size_t)gridDim.x * blockDim.x)
-    out[t] = split3(x[t]);
+    put3(out, (long long)n, t, x[t]);
 }
 
-hipError_t launch_split_bf16x3(const double *x, size_t n, unsigned long long *out, hipStream_t s) {
+hipError_t launch_split_bf16x3(const double *x, size_t n, unsigned short *out, hipStream_t s) {
   if (n == 0) return hipSuccess;
   size_t b = (n + 255) / 256;
   if (b > 262144) b = 262144;
@@ -38,17 +39,17 @@ hipError_t launch_split_bf16x3(const double *x, size_t n, unsigned long long *ou
 // ---------------------------------------------------------------------------
 // Split-BF16x3 engine, K stages of 32 with planar LDS.
 //
-// Each stage stages R rows x 32 k of an operand and writes them to LDS split
-// into three planes (hi, mid, lo) of bf16, 64 bytes per row and plane, so that
+// Each stage stages R rows x 32 k of an operand and writes them to LDS as
+// three planes (hi, mid, lo) of bf16, 64 bytes per row and plane, so that
 // an MFMA operand (8 consecutive k of one row) is one 16-byte LDS read per
 // plane with no unpacking.  The four 16-byte chunks of row r are stored
 // XOR-swizzled by ((r / 4) ^ (r / 16)) % 4, which keeps the 16-lane groups of
 // those reads (16 consecutive rows, one chunk) conflict-free.
-// A thread stages "quads" (4 consecutive k of one row, four 8-byte packed
-// elements), R / 64 of them per stage, and writes each as one 8-byte LDS store
+// A thread stages "quads" (4 consecutive k of one row, in each of the three
+// planes), R / 64 of them per stage, and writes each as one 8-byte LDS store
 // per plane:
 //   K-contiguous operand: row = tid / 8 + 64 j, quad = tid % 8 (8 lanes cover
-//     a row's 32 k; two 16-byte loads per quad);
+//     a row's 32 k; one 8-byte load per plane and quad);
 //   row-contiguous operand: row = tid % 64 + 64 j, quad = tid / 64 (64 lanes
 //     read 64 consecutive rows at one k per load).
 // Accumulation: FP32 MFMA tile sums over four stages (128 k), then added into
@@ -59,7 +60,6 @@ constexpr int BKC = 32;
 template <int R>
 struct B3Stage {
   static constexpr int NQ = (R + 63) / 64;  // quads per thread per stage
-  static constexpr int NE = 4 * NQ;         // elements per thread per stage
   static constexpr bool FULL = R % 64 == 0; // otherwise rows row0 + 64 j >= R are skipped
   static constexpr int PLANE = R * 64;
   static constexpr int BYTES = 3 * PLANE;
@@ -128,15 +128,17 @@ __device__ __forceinline__ B3K b3_kpos(const OpDesc &d, int L, int kappa0, KPos 
   return k;
 }
 
-typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 
+// the quads of one stage: v[j] = (hi, mid, lo) planes, two dwords (4 bf16) each
 template <int R>
 __device__ __forceinline__ void b3_fetch(const OpDesc &d, const B3Rows &m,
                                          const int (&roff)[B3Stage<R>::NQ],
                                          const bool (&rok)[B3Stage<R>::NQ], const B3K &k,
-                                         KPos kp, bool vec, unsigned long long (&v)[B3Stage<R>::NE],
+                                         KPos kp, bool vec, unsigned (&v)[B3Stage<R>::NQ][6],
                                          unsigned &rowmask, unsigned &kmask) {
-  const unsigned long long *p = reinterpret_cast<const unsigned long long *>(d.p);
+  const unsigned short *p = reinterpret_cast<const unsigned short *>(d.p);
+  const long long ps = d.ps;
   unsigned rm = 0;
 #pragma unroll
   for (int j = 0; j < B3Stage<R>::NQ; ++j) rm |= rok[j] ? (1u << j) : 0u;
@@ -147,60 +149,51 @@ __device__ __forceinline__ void b3_fetch(const OpDesc &d, const B3Rows &m,
     const unsigned msk = rok[j] ? k.kmask : 0u;
     const int base = roff[j] + k.off;
     if (msk == 15u && vec) {
-      // four consecutive elements, 16-byte aligned (checked per segment)
-      const u64x2 *q2 = reinterpret_cast<const u64x2 *>(p + base);
-      const u64x2 x0 = q2[0], x1 = q2[1];
-      v[4 * j] = x0[0];
-      v[4 * j + 1] = x0[1];
-      v[4 * j + 2] = x1[0];
-      v[4 * j + 3] = x1[1];
+      // four consecutive elements per plane, 8-byte aligned (checked per segment)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const u32x2 x = *reinterpret_cast<const u32x2 *>(p + base + q * ps);
+        v[j][2 * q] = x[0];
+        v[j][2 * q + 1] = x[1];
+      }
     } else {
+      unsigned e[3][4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         int o = 0;
         if ((msk >> u) & 1u) o = k.wrap ? roff[j] + b3_koff(d, kp, 4 * m.slot + u) : base + u * k.step;
-        v[4 * j + u] = p[o];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) e[q][u] = p[o + q * ps];
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        v[j][2 * q] = e[q][0] | (e[q][1] << 16);
+        v[j][2 * q + 1] = e[q][2] | (e[q][3] << 16);
       }
     }
   }
 }
 
-// a segment's operand qualifies for 16-byte loads of whole quads
+// a segment's operand qualifies for 8-byte loads of whole quads
 __device__ __forceinline__ bool b3_vec_ok(const OpDesc &d) {
-  return !d.rfast && d.ky == 1 && ((d.rx | d.ry | d.kx) & 1) == 0 && (d.kdiv & 3) == 0 &&
-         ((unsigned long long)d.p & 15ull) == 0;
-}
-
-// hi / mid / lo planes of 2 packed elements (one dword each)
-__device__ __forceinline__ void b3_planes2(unsigned long long w0, unsigned long long w1,
-                                           unsigned &hi, unsigned &mid, unsigned &lo) {
-  const unsigned a0 = (unsigned)w0, a1 = (unsigned)w1;
-  hi = __builtin_amdgcn_perm(a1, a0, 0x05040100u);
-  mid = __builtin_amdgcn_perm(a1, a0, 0x07060302u);
-  lo = __builtin_amdgcn_perm((unsigned)(w1 >> 32), (unsigned)(w0 >> 32), 0x05040100u);
+  return !d.rfast && d.ky == 1 && ((d.rx | d.ry | d.kx | d.ps) & 3) == 0 && (d.kdiv & 3) == 0 &&
+         ((unsigned long long)d.p & 7ull) == 0;
 }
 
 template <int R>
 __device__ __forceinline__ void b3_stash(const B3Rows &m, bool neg,
-                                         const unsigned long long (&v)[B3Stage<R>::NE],
+                                         const unsigned (&v)[B3Stage<R>::NQ][6],
                                          unsigned rowmask, unsigned kmask, char *S) {
-  typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-  const unsigned long long flip = neg ? 0x0000800080008000ull : 0ull;
+  const unsigned flip = neg ? 0x80008000u : 0u;
 #pragma unroll
   for (int j = 0; j < B3Stage<R>::NQ; ++j) {
     if (!B3Stage<R>::FULL && m.row0 + 64 * j >= R) continue;  // beyond the panel
-    unsigned long long w[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) w[u] = v[4 * j + u] ^ flip;
     const unsigned okj = ((rowmask >> j) & 1u) ? kmask : 0u;
-    if (okj != 15u) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) w[u] = ((okj >> u) & 1u) ? w[u] : 0ull;
-    }
-    unsigned h0, m0, l0, h1, m1, l1;
-    b3_planes2(w[0], w[1], h0, m0, l0);
-    b3_planes2(w[2], w[3], h1, m1, l1);
-    const u32x2 hi = {h0, h1}, mid = {m0, m1}, lo = {l0, l1};
+    const unsigned keep0 = ((okj & 1u) ? 0xffffu : 0u) | ((okj & 2u) ? 0xffff0000u : 0u);
+    const unsigned keep1 = ((okj & 4u) ? 0xffffu : 0u) | ((okj & 8u) ? 0xffff0000u : 0u);
+    const u32x2 hi = {(v[j][0] ^ flip) & keep0, (v[j][1] ^ flip) & keep1};
+    const u32x2 mid = {(v[j][2] ^ flip) & keep0, (v[j][3] ^ flip) & keep1};
+    const u32x2 lo = {(v[j][4] ^ flip) & keep0, (v[j][5] ^ flip) & keep1};
     const int r = m.row0 + 64 * j;
     const int off = b3_lds_off(r, m.slot >> 1) + 8 * (m.slot & 1);
     *(u32x2 *)(S + off) = hi;
@@ -218,7 +211,7 @@ __device__ __forceinline__ bf16x8 b3_read(const char *S, int plane_bytes, int pl
 // its LDS stash, with the staging map and sign it was fetched under.
 template <int MT, int NT>
 struct B3Set {
-  unsigned long long va[B3Stage<MT>::NE], vb[B3Stage<NT>::NE];
+  unsigned va[B3Stage<MT>::NQ][6], vb[B3Stage<NT>::NQ][6];
   unsigned rma, kma, rmb, kmb;  // valid rows (bit per group) and valid k of the group
   B3Rows mA, mB;
   bool neg;
@@ -244,8 +237,10 @@ struct B3Producer {
     fs = f;
     FA = d->seg[fs].A;
     FB = d->seg[fs].B;
-    FA.p += batch * bt->a[fs];
-    FB.p += batch * bt->bo[fs];
+    FA.p = reinterpret_cast<const double *>(reinterpret_cast<const unsigned short *>(FA.p) +
+                                            batch * bt->a[fs]);
+    FB.p = reinterpret_cast<const double *>(reinterpret_cast<const unsigned short *>(FB.p) +
+                                            batch * bt->bo[fs]);
     FL = d->seg[fs].L;
     Fend = d->chunk_start[fs + 1];
     Fneg = d->seg[fs].alpha < 0.0;
@@ -655,6 +650,8 @@ hipError_t launch_gemm_b3(const GemmDesc &d0, hipStream_t s, const GemmBatch &bt
   for (int q = d.nseg + 1; q <= kMaxSeg; ++q) d.chunk_start[q] = d.chunk_start[d.nseg];
   const int c = b3_pick(d.M, d.N);
   const int MT = kB3Shape[c][0], NT = kB3Shape[c][1];
+  if (c < kB3Wide && gemm_b3p_enabled() && gemm_b3p_ok(d, bt))
+    return launch_gemm_b3p(d, MT, NT, s, bt);
   const unsigned grid =
       (unsigned)((long long)((d.M + MT - 1) / MT) * ((d.N + NT - 1) / NT) * d.splits * bt.nb);
   const int n_fast = d.M >= d.N ? 1 : 0;

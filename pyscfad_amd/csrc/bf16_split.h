@@ -1,5 +1,6 @@
-// FP64 -> three bf16 pieces, packed in one 8-byte element:
-// bits 0-15 hi = bf16(x), 16-31 mid = bf16(x - hi), 32-47 lo = bf16(x - hi - mid).
+// FP64 -> three bf16 pieces hi = bf16(x), mid = bf16(x - hi), lo = bf16(x - hi - mid)
+// (split3 returns them in bits 0-15, 16-31 and 32-47 of one word), stored by the
+// split-BF16x3 engine as three planes; FP64 -> FP32 hi/lo pairs (split2).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -39,9 +40,22 @@ __device__ __forceinline__ F2 split2(double x) {
   return r;
 }
 
-// store policy of the layout kernels: FP64 copy or packed split
+// a planar split-BF16x3 array: element t of plane q at p[t + q * ps]
+struct Planar3 {
+  unsigned short *p;
+  long long ps;
+};
+
+__device__ __forceinline__ void put3(unsigned short *p, long long ps, size_t t, double x) {
+  const unsigned long long w = split3(x);
+  p[t] = (unsigned short)w;
+  p[t + ps] = (unsigned short)(w >> 16);
+  p[t + 2 * ps] = (unsigned short)(w >> 32);
+}
+
+// store policy of the layout kernels: FP64 copy, planar split-BF16x3 or packed FP32 pair
 __device__ __forceinline__ void put(double *p, size_t t, double x) { p[t] = x; }
-__device__ __forceinline__ void put(unsigned long long *p, size_t t, double x) { p[t] = split3(x); }
+__device__ __forceinline__ void put(Planar3 p, size_t t, double x) { put3(p.p, p.ps, t, x); }
 __device__ __forceinline__ void put(F2 *p, size_t t, double x) { p[t] = split2(x); }
 
 }  // namespace pamd

@@ -18,7 +18,7 @@ PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
 LIB_NAME = 'libpyscfad_amd_cc.so'
 LIB_PATH = os.path.join(PKG, LIB_NAME)
-SOURCES = ('ccsdt_kernels.hip', 'gemm_direct.hip', 'bf16_kernels.hip', 'mlp_kernels.hip', 'eris_kernels.hip',
+SOURCES = ('ccsdt_kernels.hip', 'gemm_direct.hip', 'gemm_b3p.hip', 'bf16_kernels.hip', 'mlp_kernels.hip', 'eris_kernels.hip',
            'ccsdt_driver.cpp', 'mlp_capi.cpp', 'eris_driver.cpp')
 ARCH = os.environ.get('PYSCFAD_AMD_ARCH', 'gfx950')
 

@@ -29,6 +29,10 @@ struct OpDesc {
   int rdiv, kdiv;
   int rfast;                 // 1: contiguous along the row index -> stage with threads along rows
   int pad_;
+  // split-BF16x3 operands are planar: p addresses bf16 elements, and element e
+  // of plane q (0 hi, 1 mid, 2 lo) is ((const uint16_t *)p)[e + q * ps].  The
+  // FP64 and FP32 hi/lo engines (8-byte elements) ignore ps.
+  long long ps;
 };
 
 struct SegDesc {
@@ -77,13 +81,15 @@ inline GemmBatch no_batch() {
 
 constexpr int kMaxFrag = 8;  // rows of the local-orbital projector ulo
 
-// packed 8-byte operand formats of the GEMM engines
+// operand formats of the GEMM engines: FP64; split-BF16x3 as three planes of
+// bf16 (hi, mid, lo; OpDesc::ps); FP32 hi/lo pairs packed in 8-byte elements
 enum SplitFormat { kSplitNone = 0, kSplitBf16x3 = 1, kSplitF32x2 = 2 };
 
 // Epilogue over orbits of virtual tiles for one occupied triple.
 struct OrbitArgs {
-  double *W;            // v^3; W in (and Wbar out when Wb8 is null, vjp)
-  unsigned long long *Wb8;  // v^3 packed bf16 triplets of Wbar (split-BF16x3 vjp) or null
+  double *W;            // v^3; W in (and Wbar out when Wb8 and Wb3 are null, vjp)
+  unsigned long long *Wb8;  // v^3 packed FP32 hi/lo pairs of Wbar (FP32 hi/lo vjp) or null
+  unsigned short *Wb3;      // 3 planes of v^3 bf16 (hi, mid, lo) of Wbar (split-BF16x3 vjp) or null
   double *Vb;           // v^3; Vbar out (vjp)
   // OVS[x][y][p][q] = ovov[x,p,y,q] + ovov[y,q,x,p], T2S[x][y][p][q] = t2[x,y,p,q] + t2[y,x,q,p]
   const double *t1, *ovs, *t2s, *fvo, *mo;
@@ -100,7 +106,6 @@ struct OrbitArgs {
   int nfrag;
   double u[kMaxFrag];
   double *part_u;       // [G][kMaxFrag] per-workgroup dE/du_f sums (projected adjoint)
-  int wsplit;           // format of Wb8: kSplitBf16x3 or kSplitF32x2
 };
 
 enum OrbitMode { kOrbitPlain = 0, kOrbitProjAcc = 1, kOrbitProjVjp = 2 };
@@ -143,8 +148,15 @@ hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, cons
                           hipStream_t s);
 hipError_t launch_split_f32x2(const double *x, size_t n, unsigned long long *out, hipStream_t s);
 hipError_t launch_gemm_f32x2(const GemmDesc &d, hipStream_t s, const GemmBatch &bt = no_batch());
-hipError_t launch_split_bf16x3(const double *x, size_t n, unsigned long long *out, hipStream_t s);
+hipError_t launch_split_bf16x3(const double *x, size_t n, unsigned short *out, hipStream_t s);
 hipError_t launch_gemm_bf16x3(const GemmDesc &d, hipStream_t s, const GemmBatch &bt = no_batch());
+// split-BF16x3 engine with direct-to-LDS operand loads (gemm_b3p.hip), on
+// descriptors whose chunk_start counts stages of 32; launch_gemm_bf16x3 picks
+// it when gemm_b3p_ok(d) and the tile shape is one of its four
+bool gemm_b3p_ok(const GemmDesc &d, const GemmBatch &bt = no_batch());
+bool gemm_b3p_enabled();  // env PYSCFAD_AMD_B3_DIRECT=0 turns it off
+hipError_t launch_gemm_b3p(const GemmDesc &d, int MT, int NT, hipStream_t s,
+                           const GemmBatch &bt = no_batch());
 hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s, const GemmBatch &bt = no_batch());
 // FP64 / FP32 hi/lo engine with direct-to-LDS operand loads (gemm_direct.hip):
 // usable when gemm_direct_ok(d); launch_gemm_f64 / launch_gemm_f32x2 pick it then

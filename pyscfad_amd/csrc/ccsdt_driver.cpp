@@ -138,21 +138,6 @@ static hipError_t timed(hipStream_t s, int kind, double flops, double bytes, F &
   return prof_end(s, r);
 }
 
-// one contraction in the precision of the call that issues it
-static hipError_t run_gemm(const GemmDesc &d, hipStream_t s, int prec, int kind,
-                           const GemmBatch &bt = no_batch()) {
-  if (g_record) {
-    g_record->push_back(d);
-    return hipSuccess;
-  }
-  const double fl = gemm_flops(d) * bt.nb;
-  if (prec == PAMD_PREC_BF16X3)
-    return timed(s, kind, fl, 0.0, [&] { return launch_gemm_bf16x3(d, s, bt); });
-  if (prec == PAMD_PREC_FP32X2)
-    return timed(s, kind, fl, 0.0, [&] { return launch_gemm_f32x2(d, s, bt); });
-  return timed(s, kind, fl, 0.0, [&] { return launch_gemm_f64(d, s, bt); });
-}
-
 struct Layout {
   size_t off_Vp, off_T2m, off_OOt, off_OVS, off_T2S, off_orb, off_W, off_tri, off_part_t, off_T2g, off_W8;
   int wbatch;  // W blocks of the energy walk's batched W build (1 with the adjoint)
@@ -211,8 +196,9 @@ static int energy_batch(int o, int v, int prec) {
   return (int)std::min<long long>(b, o);
 }
 
-// Vp, T2m and OOt hold FP64 copies, or packed bf16 triplets for split-BF16x3
-// (same 8-byte element); split-BF16x3 adds packed copies of t2 and of Wbar.
+// Vp, T2m and OOt hold FP64 copies, or their split forms: three bf16 planes
+// (split-BF16x3, 6 of the 8 bytes per element) or packed FP32 hi/lo pairs; the
+// split precisions add split copies of t2 and of Wbar.
 static Layout make_layout(int o, int v, int vjp, int prec) {
   Layout L;
   memset(&L, 0, sizeof(L));
@@ -258,11 +244,23 @@ static Layout make_layout(int o, int v, int vjp, int prec) {
   return L;
 }
 
+// Planar split-BF16x3 operands: the arrays the walk's descriptors point into
+// (Vp, T2m, OOt, t2 copy, Wbar copy), allocated as FP64 (8 bytes per element)
+// and holding three bf16 planes of their element count in the first 6 bytes.
+// A descriptor built with FP64 element arithmetic is moved onto them: element
+// e of an array becomes bf16 element e of its hi plane, with the array's
+// element count as the plane stride.
+struct PlanarMap {
+  const char *base[5];
+  long long n[5];
+  int count;
+};
+
 struct Ctx {
   int o, v, packed, has_fvo, vjp, prec;
-  const double *gT2;  // t2 as a GEMM operand (FP64, or packed bf16 triplets)
+  const double *gT2;  // t2 as a GEMM operand (FP64, or its split copy)
   unsigned short *orb;  // orbit table
-  double *gW;         // Wbar as a GEMM operand (= W, or its packed copy)
+  double *gW;         // Wbar as a GEMM operand (= W, or its split copy)
   hipStream_t s;
   const double *mo, *t1, *t2, *ovvv, *ovoo, *ovov, *fvo;
   double *Vp, *T2m, *OOt, *OVS, *T2S, *W, *tri, *part_t;
@@ -273,9 +271,43 @@ struct Ctx {
   int shard, nshards;     // this call covers walk triples with index % nshards == shard
   const double *wt_host;  // occupied weights m (host) of a diagonal projector, or null
   double *wt;             // device copy of m
+  PlanarMap planes;       // split-BF16x3 operand arrays
   // outputs
   double *t1_bar, *t2_bar, *ovoo_bar, *ovov_bar, *fvo_bar;
 };
+
+static bool planarize(const PlanarMap &pm, OpDesc &o) {
+  const char *p = (const char *)o.p;
+  for (int a = 0; a < pm.count; ++a) {
+    if (p < pm.base[a] || p >= pm.base[a] + pm.n[a] * (long long)sizeof(double)) continue;
+    const long long e = (p - pm.base[a]) / (long long)sizeof(double);
+    o.p = (const double *)(pm.base[a] + 2 * e);
+    o.ps = pm.n[a];
+    return true;
+  }
+  return false;
+}
+
+// one contraction in the precision of the call that issues it
+static hipError_t run_gemm(const Ctx &c, const GemmDesc &d0, int kind,
+                           const GemmBatch &bt = no_batch()) {
+  if (g_record) {
+    g_record->push_back(d0);
+    return hipSuccess;
+  }
+  const double fl = gemm_flops(d0) * bt.nb;
+  const hipStream_t s = c.s;
+  if (c.prec == PAMD_PREC_BF16X3) {
+    GemmDesc d = d0;
+    for (int q = 0; q < d.nseg; ++q)
+      if (!planarize(c.planes, d.seg[q].A) || !planarize(c.planes, d.seg[q].B))
+        return hipErrorInvalidValue;
+    return timed(s, kind, fl, 0.0, [&] { return launch_gemm_bf16x3(d, s, bt); });
+  }
+  if (c.prec == PAMD_PREC_FP32X2)
+    return timed(s, kind, fl, 0.0, [&] { return launch_gemm_f32x2(d0, s, bt); });
+  return timed(s, kind, fl, 0.0, [&] { return launch_gemm_f64(d0, s, bt); });
+}
 
 #define PAMD_CHECK(expr)                                                      \
   do {                                                                        \
@@ -321,7 +353,7 @@ static int build_w(const Ctx &c, int i, int j, int k, int nb = 1) {
                        seg(opR(T2m(i), vo, o, V, 1, 0), op1(OOt(k, j), o, 1, 0), O, -1.0)},  // H6
                       out(c.W, v2, v, V, 0, 1, kBig));
     // P1: T2(k,j), P3: T2(k,i), H5: OOt(k,i), H6: OOt(k,j)
-    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfWBuild,
+    PAMD_CHECK(run_gemm(c, d, kProfWBuild,
                         batch({0, 0, 0, 0}, {sT2k_, sT2k_, sOOk_, sOOk_})));
   }
   // pass B: rows b, cols (a,c)
@@ -333,7 +365,7 @@ static int build_w(const Ctx &c, int i, int j, int k, int nb = 1) {
                        seg(op1(OOt(j, k), o, 1, 0), opR(T2m(i), vo, o, V, 1, 0), O, -1.0)},  // H4
                       out(c.W, 0, v, kBig, v2, 1, V));
     // P2: T2(j,k) | Vp(i), P5: T2(j,i) | Vp(k), H3: OOt(j,i) | T2m(k), H4: OOt(j,k) | T2m(i)
-    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfWBuild,
+    PAMD_CHECK(run_gemm(c, d, kProfWBuild,
                         batch({sT2_k, 0, 0, sOO_k}, {0, sVp, sT2m, 0})));
   }
   // pass A: rows a, cols (b,c)
@@ -345,7 +377,7 @@ static int build_w(const Ctx &c, int i, int j, int k, int nb = 1) {
                        seg(op1(OOt(i, k), o, 1, 0), opR(T2m(j), vo, o, V, 1, 0), O, -1.0)},  // H2
                       out(c.W, 0, v2, kBig, v, 1, V));
     // P4: T2(i,k) | Vp(j), P6: T2(i,j) | Vp(k), H1: OOt(i,j) | T2m(k), H2: OOt(i,k) | T2m(j)
-    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfWBuild,
+    PAMD_CHECK(run_gemm(c, d, kProfWBuild,
                         batch({sT2_k, 0, 0, sOO_k}, {0, sVp, sT2m, 0})));
   }
   return PAMD_OK;
@@ -391,21 +423,21 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
                       {seg(opR(Wb, v2, v, V, 1, 0), op1(T2(k, j), 1, v, 1), V, 1.0),
                        seg(opR(Wb, v2, 1, V, v, 1), op1(T2(j, k), 1, v, 1), V, 1.0)},
                       out(Vpb(i), v2, v, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfAdjOvvv));
+    PAMD_CHECK(run_gemm(c, d, kProfAdjOvvv));
   }
   {  // Vpb[j][b][a|c][f]: P3 (K=c), P4 (K=a)
     GemmDesc d = gemm(V * V, V, kAccumulate,
                       {seg(opR(Wb, v, v2, V, 1, 0), op1(T2(k, i), 1, v, 1), V, 1.0),
                        seg(opR(Wb, v, 1, V, v2, 1), op1(T2(i, k), 1, v, 1), V, 1.0)},
                       out(Vpb(j), v2, v, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfAdjOvvv));
+    PAMD_CHECK(run_gemm(c, d, kProfAdjOvvv));
   }
   {  // Vpb[k][c][a|b][f]: rows m = y*v + x with x = c: P5 (K=b), P6 (K=a)
     GemmDesc d = gemm(V * V, V, kAccumulate,
                       {seg(opR(Wb, v2, 1, V, v, 1), op1(T2(j, i), 1, v, 1), V, 1.0),
                        seg(opR(Wb, v, 1, V, v2, 1), op1(T2(i, j), 1, v, 1), V, 1.0)},
                       out(Vpb(k), v, v2, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfAdjOvvv));
+    PAMD_CHECK(run_gemm(c, d, kProfAdjOvvv));
   }
 
   // particle t2_bar pieces: split-K over the virtual pair, partials per problem
@@ -431,7 +463,7 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
         gemm(V, V, kPartial, {seg(opK(Wb, v2, v, 1, V, 0), opK(Vp(k), 1, v, v2, V, 1), V * V, 1.0)},
              out(nullptr, 0, 0, 1, 0, 0, 1), c.S_p, c.part_p + 5 * pp),
     };
-    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(ds[q], c.s, c.prec, kProfAdjT2P));
+    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(c, ds[q], kProfAdjT2P));
   }
   // hole ovoo_bar pieces: split-K over the virtual pair
   const size_t ph = (size_t)c.S_h * v * o;
@@ -456,7 +488,7 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
         gemm(V, O, kPartial, {seg(opK(Wb, 1, v2, v, V, 1), opK(T2m(i), 1, vo, o, V, 1), V * V, -1.0)},
              out(nullptr, 0, 0, 1, 0, 0, 1), c.S_h, c.part_h + 5 * ph),
     };
-    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(ds[q], c.s, c.prec, kProfAdjOvoo));
+    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(c, ds[q], kProfAdjOvoo));
   }
   // hole-term t2 reads: T2mb[x][p][q][m] (= dE/d t2[x,m,p,q])
   {  // T2mb[i]: H4 (p=a,q=c,K=b), H6 (p=a,q=b,K=c)
@@ -464,21 +496,21 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
                       {seg(opR(Wb, v2, 1, V, v, 1), op1(OOt(j, k), 1, o, 1), V, -1.0),
                        seg(opR(Wb, v2, v, V, 1, 0), op1(OOt(k, j), 1, o, 1), V, -1.0)},
                       out(T2mb(i), vo, o, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfAdjT2H));
+    PAMD_CHECK(run_gemm(c, d, kProfAdjT2H));
   }
   {  // T2mb[j]: H2 (p=b,q=c,K=a), H5 (p=b,q=a,K=c)
     GemmDesc d = gemm(V * V, O, kAccumulate,
                       {seg(opR(Wb, v, 1, V, v2, 1), op1(OOt(i, k), 1, o, 1), V, -1.0),
                        seg(opR(Wb, v, v2, V, 1, 0), op1(OOt(k, i), 1, o, 1), V, -1.0)},
                       out(T2mb(j), vo, o, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfAdjT2H));
+    PAMD_CHECK(run_gemm(c, d, kProfAdjT2H));
   }
   {  // T2mb[k]: rows m = q*v + p with p = c: H1 (q=b,K=a), H3 (q=a,K=b)
     GemmDesc d = gemm(V * V, O, kAccumulate,
                       {seg(opR(Wb, v, 1, V, v2, 1), op1(OOt(i, j), 1, o, 1), V, -1.0),
                        seg(opR(Wb, v2, 1, V, v, 1), op1(OOt(j, i), 1, o, 1), V, -1.0)},
                       out(T2mb(k), o, vo, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(d, c.s, c.prec, kProfAdjT2H));
+    PAMD_CHECK(run_gemm(c, d, kProfAdjT2H));
   }
   // gather into the cotangents
   CombineArgs ca;
@@ -505,10 +537,17 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
   return PAMD_OK;
 }
 
-// packed operand format of a precision (FP64 operands are not packed)
+// operand format of a precision
 static int split_format(int prec) {
   return prec == PAMD_PREC_BF16X3 ? kSplitBf16x3
                                   : (prec == PAMD_PREC_FP32X2 ? kSplitF32x2 : kSplitNone);
+}
+
+// where the orbit epilogue writes Wbar: the GEMM operand copy in the engine's
+// format (split forms), or over W (FP64)
+static void wbar_out(const Ctx &c, int fmt, OrbitArgs &oa) {
+  oa.Wb3 = fmt == kSplitBf16x3 ? (unsigned short *)c.gW : nullptr;
+  oa.Wb8 = fmt == kSplitF32x2 ? (unsigned long long *)c.gW : nullptr;
 }
 
 // Zero the adjoint accumulators and the cotangent outputs of one call.
@@ -534,7 +573,7 @@ static int setup(Ctx &c, long long &norb) {
   PAMD_CHECK(launch_prep_sym(c.ovov, o, v, 0, c.OVS, c.s));
   PAMD_CHECK(launch_prep_sym(c.t2, o, v, 1, c.T2S, c.s));
   if (fmt == kSplitBf16x3)
-    PAMD_CHECK(launch_split_bf16x3(c.t2, (size_t)o * o * v * v, (unsigned long long *)c.gT2, c.s));
+    PAMD_CHECK(launch_split_bf16x3(c.t2, (size_t)o * o * v * v, (unsigned short *)c.gT2, c.s));
   if (fmt == kSplitF32x2)
     PAMD_CHECK(launch_split_f32x2(c.t2, (size_t)o * o * v * v, (unsigned long long *)c.gT2, c.s));
   PAMD_CHECK(launch_fill(c.tri, (size_t)o * o * o, 0.0, c.s));
@@ -634,8 +673,7 @@ static int run_projected(Ctx &c, const double *ulo, int nfrag, double *accA, dou
         st = build_w(c, l, j, k);
         if (st != PAMD_OK) return st;
         OrbitArgs oa = orbit_args(c, l, j, k, norb);
-        oa.Wb8 = fmt != kSplitNone ? (unsigned long long *)c.gW : nullptr;
-        oa.wsplit = fmt;
+        wbar_out(c, fmt, oa);
         oa.vjp = 1;
         oa.ebar = g;
         oa.accA = accA;
@@ -691,7 +729,6 @@ static int run(Ctx &c) {
             ++tcount;
             OrbitArgs oa = orbit_args(c, i, j, k, norb);
             oa.W = c.W + b * v3;
-            oa.wsplit = fmt;
             oa.omega = (i == j || j == k) ? 1.0 : 2.0;
             oa.ebar = c.ebar;
             const double ob = 8.0 * (double)v3;  // W read
@@ -723,8 +760,7 @@ static int run(Ctx &c) {
         int st = build_w(c, i, j, k);
         if (st != PAMD_OK) return st;
         OrbitArgs oa = orbit_args(c, i, j, k, norb);
-        oa.Wb8 = (fmt != kSplitNone && c.vjp) ? (unsigned long long *)c.gW : nullptr;
-        oa.wsplit = fmt;
+        if (c.vjp) wbar_out(c, fmt, oa);
         oa.vjp = c.vjp;
         oa.omega = (i == j || j == k) ? 1.0 : 2.0;
         // a diagonal occupied projector diag(m) weighs the whole triple by
@@ -786,6 +822,21 @@ static void bind_workspace(Ctx &c, const Layout &L, void *ws) {
   const bool packed_ops = c.prec == PAMD_PREC_BF16X3 || c.prec == PAMD_PREC_FP32X2;
   c.gT2 = packed_ops ? (const double *)(b + L.off_T2g) : c.t2;
   c.gW = (packed_ops && c.vjp) ? (double *)(b + L.off_W8) : c.W;
+  {
+    const long long o = c.o, v = c.v, v3 = v * v * v;
+    PlanarMap &pm = c.planes;
+    pm.count = 0;
+    auto add = [&](const void *p, long long n) {
+      pm.base[pm.count] = (const char *)p;
+      pm.n[pm.count] = n;
+      ++pm.count;
+    };
+    add(c.Vp, o * v3);
+    add(c.T2m, o * o * v * v);
+    add(c.OOt, o * o * o * v);
+    add(c.gT2, o * o * v * v);
+    if (c.vjp) add(c.gW, v3);
+  }
   if (c.vjp) {
     c.Vb = (double *)(b + L.off_Vb);
     c.Vpb = (double *)(b + L.off_Vpb);

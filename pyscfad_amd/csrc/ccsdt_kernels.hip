@@ -67,8 +67,7 @@ __device__ double block_sum256(double x, double *scratch) {
 //   OOt[x][y][p][m] = ovoo[x,p,m,y]
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ void k_prep_vp(const double *__restrict__ ovvv, int packed, int o, int v,
-                          T *__restrict__ Vp) {
+__global__ void k_prep_vp(const double *__restrict__ ovvv, int packed, int o, int v, T Vp) {
   const size_t total = (size_t)o * v * v * v;
   const size_t vv = (size_t)v * v;
   const size_t npair = (size_t)v * (v + 1) / 2;
@@ -89,7 +88,7 @@ __global__ void k_prep_vp(const double *__restrict__ ovvv, int packed, int o, in
 }
 
 template <class T>
-__global__ void k_prep_t2m(const double *__restrict__ t2, int o, int v, T *__restrict__ T2m) {
+__global__ void k_prep_t2m(const double *__restrict__ t2, int o, int v, T T2m) {
   const size_t total = (size_t)o * o * v * v;
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (size_t)gridDim.x * blockDim.x) {
@@ -102,7 +101,7 @@ __global__ void k_prep_t2m(const double *__restrict__ t2, int o, int v, T *__res
 }
 
 template <class T>
-__global__ void k_prep_oot(const double *__restrict__ ovoo, int o, int v, T *__restrict__ OOt) {
+__global__ void k_prep_oot(const double *__restrict__ ovoo, int o, int v, T OOt) {
   const size_t total = (size_t)o * o * v * o;
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (size_t)gridDim.x * blockDim.x) {
@@ -121,9 +120,10 @@ static unsigned grid_for(size_t n) {
   return (unsigned)b;
 }
 
+// T: the store sink (a pointer, or Planar3)
 template <class T>
 static hipError_t prepare_as(const double *ovvv, int packed, const double *t2, const double *ovoo,
-                             int o, int v, T *Vp, T *T2m, T *OOt, hipStream_t s) {
+                             int o, int v, T Vp, T T2m, T OOt, hipStream_t s) {
   hipLaunchKernelGGL(k_prep_vp<T>, dim3(grid_for((size_t)o * v * v * v)), dim3(256), 0, s, ovvv,
                      packed, o, v, Vp);
   hipLaunchKernelGGL(k_prep_t2m<T>, dim3(grid_for((size_t)o * o * v * v)), dim3(256), 0, s, t2, o,
@@ -133,17 +133,22 @@ static hipError_t prepare_as(const double *ovvv, int packed, const double *t2, c
   return hipGetLastError();
 }
 
-// split_bf16: write packed (hi, mid, lo) bf16 triplets instead of FP64 copies
+// split == kSplitBf16x3: three bf16 planes (hi, mid, lo) per array, each plane
+// as long as the array (in the first 6 of the 8 bytes per element allocated)
 hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, const double *ovoo,
                           int o, int v, double *Vp, double *T2m, double *OOt, int split,
                           hipStream_t s) {
   if (split == kSplitBf16x3) {
-    typedef unsigned long long u64;
-    return prepare_as<u64>(ovvv, packed, t2, ovoo, o, v, (u64 *)Vp, (u64 *)T2m, (u64 *)OOt, s);
+    const long long n_vp = (long long)o * v * v * v, n_t2m = (long long)o * o * v * v,
+                    n_oot = (long long)o * o * v * o;
+    return prepare_as<Planar3>(ovvv, packed, t2, ovoo, o, v,
+                               Planar3{(unsigned short *)Vp, n_vp},
+                               Planar3{(unsigned short *)T2m, n_t2m},
+                               Planar3{(unsigned short *)OOt, n_oot}, s);
   }
   if (split == kSplitF32x2)
-    return prepare_as<F2>(ovvv, packed, t2, ovoo, o, v, (F2 *)Vp, (F2 *)T2m, (F2 *)OOt, s);
-  return prepare_as<double>(ovvv, packed, t2, ovoo, o, v, Vp, T2m, OOt, s);
+    return prepare_as<F2 *>(ovvv, packed, t2, ovoo, o, v, (F2 *)Vp, (F2 *)T2m, (F2 *)OOt, s);
+  return prepare_as<double *>(ovvv, packed, t2, ovoo, o, v, Vp, T2m, OOt, s);
 }
 
 __global__ void k_split_f32x2(const double *__restrict__ x, size_t n, F2 *__restrict__ out) {
@@ -727,8 +732,10 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
           }
           const double vbar = r3Q * rD;
           const double wbar = A.ebar * pw + vbar;
-          if (A.Wb8)
-            A.Wb8[idx] = A.wsplit == kSplitF32x2 ? split2(wbar).u : split3(wbar);
+          if (A.Wb3)
+            put3(A.Wb3, v3, idx, wbar);
+          else if (A.Wb8)
+            A.Wb8[idx] = split2(wbar).u;
           else
             A.W[idx] = wbar;
           A.Vb[idx] = vbar;
@@ -739,8 +746,10 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
             const double r3W = 4.0 * w0 + w1 + w2 - 2.0 * (w3 + w4 + w5);
             const double vbar = g * r3W * rD;
             const double wbar = g * Z * rD + vbar;
-            if (A.Wb8)
-              A.Wb8[idx] = A.wsplit == kSplitF32x2 ? split2(wbar).u : split3(wbar);
+            if (A.Wb3)
+              put3(A.Wb3, v3, idx, wbar);
+            else if (A.Wb8)
+              A.Wb8[idx] = split2(wbar).u;
             else
               A.W[idx] = wbar;
             A.Vb[idx] = vbar;

@@ -1,0 +1,476 @@
+// Split-BF16x3 contraction engine with direct-to-LDS operand loads (gfx950).
+//
+// Same problem and descriptors as the register-staged split-BF16x3 engine
+// (bf16_kernels.hip): C (=, +=, split-K partial) sum_s alpha_s A_s B_s, each
+// product as the six bf16 MFMA terms hi*hi + hi*mid + mid*hi + hi*lo + lo*hi +
+// mid*mid summed in FP32 per workgroup, alpha_s = +-1.  Operands are planar
+// (OpDesc::ps): the hi, mid and lo pieces of an element sit in three bf16
+// planes of the FP64 array's layout.  Every lane moves 16 bytes (8 consecutive
+// bf16 of one plane) per `global_load_lds_dwordx4`, which writes LDS at (wave
+// base + 16 * lane) without passing through registers; the LDS layout is set
+// by which global chunk each lane fetches.  Per plane of an R-row x 32-k stage:
+//
+//   K-contiguous operand: [r][32 k], 64 bytes per row; the chunk holding k
+//     8c..8c+7 of row r is stored at chunk c ^ ksw(r).  The fragment read
+//     (16 rows, one chunk, 16 bytes per lane) then touches 16 distinct 16-byte
+//     slots of every 256-byte window, and 8 distinct ones of every 128-byte
+//     window for the first 8 lanes;
+//   row-contiguous operand: [32 k][R], 2R bytes per k-row; the chunk holding
+//     rows 8c..8c+7 of k-row kk is stored at chunk c ^ rsw(kk).  A 16x32 MFMA
+//     operand (row = lane % 16, k = 8 (lane / 16) + 0..7) is read with two
+//     ds_read_b64_tr_b16: within each 16-lane group that instruction hands
+//     lane j element j % 4 of the 8 bytes read by lane 4e + j / 4, as element
+//     e.  Lane 4e + q of group G therefore reads rows 4q..4q+3 of k-row
+//     8G + e (then 8G + 4 + e), and lane j receives row j at k 8G + e.
+//
+// Two LDS stages: while stage g is multiplied, the loads of stage g + 1 are
+// in flight.  Each iteration waits for its own loads (s_waitcnt vmcnt(0)),
+// then one s_barrier (no memory fence) makes every wave's loads visible and
+// retires the stage read one iteration earlier, which the new loads overwrite.
+//
+// Eligibility (gemm_b3p_ok): 8 consecutive elements along the fast index are
+// adjacent and 16-byte aligned for every row and k the kernel addresses: fast
+// stride 1, every other stride, divisor, plane stride and batch offset a
+// multiple of 8, the base 16-byte aligned; K-contiguous operands need
+// segment lengths that are multiples of 8, row-contiguous ones a row count
+// that is.  Rows past M or N and k past L are clamped to valid elements
+// (finite inputs); k past L is zeroed in the A fragments, rows past M or N
+// are never stored.
+#include <cstdlib>
+#include "ccsdt_common.h"
+#include "engine_staging.h"
+
+namespace pamd {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short v4i16 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) void *lds_ptr_t;
+typedef __attribute__((address_space(3))) v4i16 lds_v4i16;
+typedef unsigned short bf16_t;
+
+constexpr int kPK = 32;      // k per stage: one 16x16x32 MFMA step
+constexpr int kPStages = 2;
+constexpr int kPThreads = 512;
+
+// s_waitcnt with only the vector-memory counter constrained (gfx9 encoding:
+// vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+__device__ __forceinline__ int ksw(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
+template <int R>
+__device__ __forceinline__ int rsw(int kk) {
+  if constexpr (R % 128 == 0) return 2 * (kk & 3) + 8 * ((kk >> 3) & 1);
+  if constexpr (R % 64 == 0) return 2 * (kk & 3);
+  return 0;  // 2R = 32 x odd bytes per k-row: consecutive k-rows already fall in distinct banks
+}
+
+// per-thread load plan of one operand (R rows, 3 planes) for one segment
+template <int R>
+struct PPlan {
+  static constexpr int CH = 12 * R;                                   // chunks per stage
+  static constexpr int NI = (CH + kPThreads - 1) / kPThreads;         // loads per thread
+  static constexpr int SLOTS = NI * kPThreads;                        // LDS chunks (with padding)
+  const bf16_t *ptr[NI];  // source of load q at the segment's k = 0, used by full stages
+};
+
+// slot of load q: plane and, within the plane's tile, the row (K-contiguous)
+// or first row of 8 (row-contiguous) rl, and the chunk-local k kl.  Loads past
+// the 12 R chunks fill padding with a valid chunk.
+template <int R>
+__device__ __forceinline__ void pslot(int q, int tid, int rc, int &plane, int &rl, int &kl) {
+  const int id = q * kPThreads + tid;
+  if (id >= 12 * R) {
+    plane = 0;
+    rl = 0;
+    kl = 0;
+    return;
+  }
+  plane = id / (4 * R);
+  const int rem = id - plane * 4 * R;
+  if (!rc) {
+    rl = rem >> 2;
+    kl = 8 * ((rem & 3) ^ ksw(rl));
+  } else {
+    constexpr int CPR = R / 8;
+    const int kk = rem / CPR, pc = rem - kk * CPR;
+    rl = 8 * (pc ^ rsw<R>(kk));
+    kl = kk;
+  }
+}
+
+// element offset of row r (clamped: K-contiguous to Rtot - 1, row-contiguous
+// chunk starts to Rtot - 8)
+__device__ __forceinline__ long long prow(const OpDesc &d, int rc, int Rtot, int r) {
+  const int lim = rc ? Rtot - 8 : Rtot - 1;
+  if (r > lim) r = lim;
+  const int q = r / d.rdiv;
+  return (long long)q * d.rx + (long long)(r - q * d.rdiv) * d.ry;
+}
+
+// element offset of contraction index k (clamped: K-contiguous chunk starts
+// to L - 8, row-contiguous k-rows to L - 1)
+__device__ __forceinline__ long long pkoff(const OpDesc &d, int rc, int L, int k) {
+  const int lim = rc ? L - 1 : L - 8;
+  if (k > lim) k = lim;
+  const int q = k / d.kdiv;
+  return (long long)q * d.kx + (long long)(k - q * d.kdiv) * d.ky;
+}
+
+template <int R>
+__device__ __forceinline__ void pplan(const OpDesc &d, int rc, int Rtot, int r0, int tid,
+                                      PPlan<R> &pl) {
+  const bf16_t *p = reinterpret_cast<const bf16_t *>(d.p);
+#pragma unroll
+  for (int q = 0; q < PPlan<R>::NI; ++q) {
+    int plane, rl, kl;
+    pslot<R>(q, tid, rc, plane, rl, kl);
+    pl.ptr[q] = p + plane * d.ps + prow(d, rc, Rtot, r0 + rl) + (long long)kl * d.ky;
+  }
+}
+
+// the loads of one stage (k = kappa .. kappa + 31, kp = divmod(kappa, kdiv))
+template <int R>
+__device__ __forceinline__ void pissue(const OpDesc &d, int rc, int Rtot, int r0, int tid, int L,
+                                       int kappa, KPos kp, const PPlan<R> &pl, bf16_t *lds,
+                                       int wave) {
+  const bool full = kappa + kPK <= L && (d.kdiv >= (1 << 30) || kp.r + kPK <= d.kdiv);
+  if (full) {
+    const long long kb = (long long)kp.q * d.kx + (long long)kp.r * d.ky;  // wave-uniform
+#pragma unroll
+    for (int q = 0; q < PPlan<R>::NI; ++q)
+      __builtin_amdgcn_global_load_lds((const void *)(pl.ptr[q] + kb),
+                                       (lds_ptr_t)(lds + (q * kPThreads + wave * 64) * 8), 16, 0, 0);
+    return;
+  }
+  const bf16_t *p = reinterpret_cast<const bf16_t *>(d.p);
+#pragma unroll
+  for (int q = 0; q < PPlan<R>::NI; ++q) {
+    int plane, rl, kl;
+    pslot<R>(q, tid, rc, plane, rl, kl);
+    const bf16_t *src = p + plane * d.ps + prow(d, rc, Rtot, r0 + rl) + pkoff(d, rc, L, kappa + kl);
+    __builtin_amdgcn_global_load_lds((const void *)src,
+                                     (lds_ptr_t)(lds + (q * kPThreads + wave * 64) * 8), 16, 0, 0);
+  }
+}
+
+__device__ __forceinline__ void kadvance(KPos &kp, int kdiv) {
+  kp.r += kPK;
+  while (kp.r >= kdiv) {
+    kp.r -= kdiv;
+    ++kp.q;
+  }
+}
+
+// MFMA operand (rows r16 .. r16 + 15, k 8 (lane / 16) .. + 7) of one plane
+template <int R>
+__device__ __forceinline__ u32x4 frag_k(const bf16_t *P, int r16, int lane) {
+  const int r = r16 + (lane & 15), G = lane >> 4;
+  return *(const u32x4 *)(P + r * kPK + 8 * (G ^ ksw(r)));
+}
+
+template <int R>
+__device__ __forceinline__ u32x4 frag_r(const bf16_t *P, int r16, int lane) {
+  const int G = lane >> 4, i = lane & 15, e = i >> 2, q = i & 3;
+  const int rr = r16 + 4 * q;
+  const int c = rr >> 3, half = rr & 7;
+  const int k0 = 8 * G + e, k1 = k0 + 4;
+  const v4i16 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+      (lds_v4i16 *)(P + k0 * R + 8 * (c ^ rsw<R>(k0)) + half));
+  const v4i16 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+      (lds_v4i16 *)(P + k1 * R + 8 * (c ^ rsw<R>(k1)) + half));
+  const u32x2 l2 = __builtin_bit_cast(u32x2, lo), h2 = __builtin_bit_cast(u32x2, hi);
+  return u32x4{l2[0], l2[1], h2[0], h2[1]};
+}
+
+template <int R>
+__device__ __forceinline__ void frags(const bf16_t *T, int rc, int r16, int lane, u32x4 &h,
+                                      u32x4 &m, u32x4 &l) {
+  constexpr int PL = R * kPK;  // bf16 per plane
+  if (rc) {
+    h = frag_r<R>(T, r16, lane);
+    m = frag_r<R>(T + PL, r16, lane);
+    l = frag_r<R>(T + 2 * PL, r16, lane);
+  } else {
+    h = frag_k<R>(T, r16, lane);
+    m = frag_k<R>(T + PL, r16, lane);
+    l = frag_k<R>(T + 2 * PL, r16, lane);
+  }
+}
+
+template <int MT, int NT, int WM, int WN>
+__global__ __launch_bounds__(kPThreads, 4) void k_gemm_b3p(GemmDesc d, GemmBatch bt, int n_fast) {
+  static_assert(WM * WN == 8, "8 waves");
+  constexpr int WTM = MT / WM, WTN = NT / WN;
+  constexpr int TM = WTM / 16, TN = WTN / 16;
+  static_assert(TM * 16 == WTM && TN * 16 == WTN, "16x16 MFMA tiles");
+  constexpr int NIA = PPlan<MT>::NI, NIB = PPlan<NT>::NI;
+  constexpr int SA = PPlan<MT>::SLOTS * 8, SB = PPlan<NT>::SLOTS * 8, SS = SA + SB;  // bf16
+  __shared__ __align__(16) bf16_t S[kPStages * SS];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+  const int mt = (d.M + MT - 1) / MT, nt = (d.N + NT - 1) / NT;
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const int per = mt * nt * d.splits;  // workgroups of one product of the batch
+  const int batch = logical / per;
+  const int rem = logical - batch * per;
+  const int split = rem / (mt * nt);
+  const int tile = rem - split * (mt * nt);
+  const int tm = n_fast ? tile / nt : tile % mt;
+  const int tn = n_fast ? tile % nt : tile / mt;
+  const int m0 = tm * MT, n0 = tn * NT;
+  const int total = d.chunk_start[d.nseg];
+  const int cps = (total + d.splits - 1) / d.splits;
+  const int g0 = split * cps;
+  const int g1 = min(total, g0 + cps);
+
+  f32x4 acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  // fetch tracker: segment, stage start and K position of the next stage to load
+  int fs = 0;
+  while (fs + 1 < d.nseg && g0 >= d.chunk_start[fs + 1]) ++fs;
+  int fkappa = (g0 - d.chunk_start[fs]) * kPK;
+  KPos fka{fkappa / d.seg[fs].A.kdiv, fkappa % d.seg[fs].A.kdiv};
+  KPos fkb{fkappa / d.seg[fs].B.kdiv, fkappa % d.seg[fs].B.kdiv};
+  OpDesc FA = d.seg[fs].A, FB = d.seg[fs].B;
+  FA.p = reinterpret_cast<const double *>(reinterpret_cast<const bf16_t *>(FA.p) + batch * bt.a[fs]);
+  FB.p = reinterpret_cast<const double *>(reinterpret_cast<const bf16_t *>(FB.p) + batch * bt.bo[fs]);
+  int FL = d.seg[fs].L, fend = d.chunk_start[fs + 1];
+  PPlan<MT> pa;
+  PPlan<NT> pb;
+  pplan<MT>(FA, FA.rfast, d.M, m0, tid, pa);
+  pplan<NT>(FB, FB.rfast, d.N, n0, tid, pb);
+  // compute tracker
+  int cs = fs, ckappa = fkappa, cend = d.chunk_start[fs + 1];
+  int la = d.seg[fs].A.rfast, lb = d.seg[fs].B.rfast, cL = d.seg[fs].L;
+  bool neg = d.seg[fs].alpha < 0.0;
+
+  auto issue = [&](int g) {  // loads of stage g (the fetch tracker's stage)
+    bf16_t *st = S + ((g - g0) % kPStages) * SS;
+    pissue<MT>(FA, FA.rfast, d.M, m0, tid, FL, fkappa, fka, pa, st, wave);
+    pissue<NT>(FB, FB.rfast, d.N, n0, tid, FL, fkappa, fkb, pb, st + SA, wave);
+    if (g + 1 < g1) {
+      if (g + 1 >= fend) {
+        while (g + 1 >= d.chunk_start[fs + 1]) ++fs;  // (skips segments of length 0)
+        FA = d.seg[fs].A;
+        FB = d.seg[fs].B;
+        FA.p = reinterpret_cast<const double *>(reinterpret_cast<const bf16_t *>(FA.p) +
+                                                batch * bt.a[fs]);
+        FB.p = reinterpret_cast<const double *>(reinterpret_cast<const bf16_t *>(FB.p) +
+                                                batch * bt.bo[fs]);
+        FL = d.seg[fs].L;
+        fend = d.chunk_start[fs + 1];
+        fkappa = 0;
+        fka = KPos{0, 0};
+        fkb = KPos{0, 0};
+        pplan<MT>(FA, FA.rfast, d.M, m0, tid, pa);
+        pplan<NT>(FB, FB.rfast, d.N, n0, tid, pb);
+      } else {
+        fkappa += kPK;
+        kadvance(fka, FA.kdiv);
+        kadvance(fkb, FB.kdiv);
+      }
+    }
+  };
+
+  if (g0 < g1) issue(g0);
+  const int G = lane >> 4;
+  for (int g = g0; g < g1; ++g) {
+    wait_vmcnt<0>();
+    __builtin_amdgcn_s_barrier();
+    if (g + 1 < g1) issue(g + 1);
+
+    if (g >= cend) {
+      while (g >= d.chunk_start[cs + 1]) ++cs;
+      ckappa = 0;
+      cend = d.chunk_start[cs + 1];
+      la = d.seg[cs].A.rfast;
+      lb = d.seg[cs].B.rfast;
+      neg = d.seg[cs].alpha < 0.0;
+      cL = d.seg[cs].L;
+    }
+    const int kv = cL - ckappa;  // valid k in this stage (>= 32: all)
+    ckappa += kPK;
+    const bf16_t *As = S + ((g - g0) % kPStages) * SS;
+    const bf16_t *Bs = As + SA;
+    // k past the segment is zeroed on the A side (B holds clamped, finite
+    // values); alpha = -1 flips the sign bits of the A pieces
+    u32x4 keep = {~0u, ~0u, ~0u, ~0u};
+    if (kv < kPK) {
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        const int k = 8 * G + 2 * w;
+        keep[w] = (k < kv ? 0xffffu : 0u) | (k + 1 < kv ? 0xffff0000u : 0u);
+      }
+    }
+    const unsigned sg = neg ? 0x80008000u : 0u;
+    const u32x4 sgn = {sg, sg, sg, sg};
+    // the fragments of the narrower side stay in registers; the other side is
+    // read two tiles at a time, and the six products of those two tiles
+    // alternate between their accumulators
+    if constexpr (TM <= TN) {
+      u32x4 ah[TM], am[TM], al[TM];
+#pragma unroll
+      for (int t = 0; t < TM; ++t) {
+        frags<MT>(As, la, wm * WTM + 16 * t, lane, ah[t], am[t], al[t]);
+        ah[t] = (ah[t] ^ sgn) & keep;
+        am[t] = (am[t] ^ sgn) & keep;
+        al[t] = (al[t] ^ sgn) & keep;
+      }
+#pragma unroll
+      for (int b0 = 0; b0 < TN; b0 += 2) {
+        constexpr int W2 = 2;
+        u32x4 bh[W2], bm[W2], bl[W2];
+#pragma unroll
+        for (int u = 0; u < W2; ++u)
+          if (b0 + u < TN) frags<NT>(Bs, lb, wn * WTN + 16 * (b0 + u), lane, bh[u], bm[u], bl[u]);
+#pragma unroll
+        for (int a = 0; a < TM; ++a) {
+#define B3P_PRODUCT(AX, BX)                                                                   \
+  _Pragma("unroll") for (int u = 0; u < W2; ++u) if (b0 + u < TN) acc[a][b0 + u] =            \
+      __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, AX),                 \
+                                              __builtin_bit_cast(bf16x8, BX[u]), acc[a][b0 + u], 0, 0, 0);
+          // smallest terms first
+          B3P_PRODUCT(am[a], bm)
+          B3P_PRODUCT(ah[a], bl)
+          B3P_PRODUCT(al[a], bh)
+          B3P_PRODUCT(ah[a], bm)
+          B3P_PRODUCT(am[a], bh)
+          B3P_PRODUCT(ah[a], bh)
+#undef B3P_PRODUCT
+        }
+      }
+    } else {
+      u32x4 bh[TN], bm[TN], bl[TN];
+#pragma unroll
+      for (int t = 0; t < TN; ++t) frags<NT>(Bs, lb, wn * WTN + 16 * t, lane, bh[t], bm[t], bl[t]);
+#pragma unroll
+      for (int a0 = 0; a0 < TM; a0 += 2) {
+        constexpr int W2 = 2;
+        u32x4 ah[W2], am[W2], al[W2];
+#pragma unroll
+        for (int u = 0; u < W2; ++u)
+          if (a0 + u < TM) {
+            frags<MT>(As, la, wm * WTM + 16 * (a0 + u), lane, ah[u], am[u], al[u]);
+            ah[u] = (ah[u] ^ sgn) & keep;
+            am[u] = (am[u] ^ sgn) & keep;
+            al[u] = (al[u] ^ sgn) & keep;
+          }
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+#define B3P_PRODUCT(AX, BX)                                                                   \
+  _Pragma("unroll") for (int u = 0; u < W2; ++u) if (a0 + u < TM) acc[a0 + u][b] =            \
+      __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, AX[u]),              \
+                                              __builtin_bit_cast(bf16x8, BX), acc[a0 + u][b], 0, 0, 0);
+          B3P_PRODUCT(am, bm[b])
+          B3P_PRODUCT(ah, bl[b])
+          B3P_PRODUCT(al, bh[b])
+          B3P_PRODUCT(ah, bm[b])
+          B3P_PRODUCT(am, bh[b])
+          B3P_PRODUCT(ah, bh[b])
+#undef B3P_PRODUCT
+        }
+      }
+    }
+  }
+
+  // D layout of v_mfma_f32_16x16x32_bf16: lane l, element r -> row 4 (l / 16) + r, column l % 16
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ml = wm * WTM + 16 * a + 4 * (lane >> 4) + r;
+        const int nl = wn * WTN + 16 * b + (lane & 15);
+        const bool part = d.mode == kPartial;
+        const OutPos pm = out_pos<MT>(tm, ml, part ? (1 << 30) : d.C.mdiv);
+        const OutPos pn = out_pos<NT>(tn, nl, part ? (1 << 30) : d.C.ndiv);
+        const int m = pm.idx, n = pn.idx;
+        if (m >= d.M || n >= d.N) continue;
+        const double val = (double)acc[a][b][r];
+        if (part) {
+          d.partial[((size_t)split * d.M + m) * d.N + n] = val;
+        } else {
+          double *c = d.C.p + batch * bt.c + (long long)pm.q * d.C.mx + (long long)pm.r * d.C.my +
+                      (long long)pn.q * d.C.nx + (long long)pn.r * d.C.ny;
+          if (d.mode == kAccumulate)
+            *c += val;
+          else
+            *c = val;
+        }
+      }
+}
+
+// An operand can be staged with 16-byte direct loads of 8 bf16 along its fast
+// index, for every row and k the kernel may address (rows clamped to
+// [0, rows), k to [0, L)).
+bool op_b3p_ok(const OpDesc &o, int rows, int L) {
+  auto m8 = [](long long x) { return (x & 7) == 0; };
+  const bool rsplit = o.rdiv < (1 << 30), ksplit = o.kdiv < (1 << 30);
+  if (((uintptr_t)o.p & 15) != 0 || !m8(o.ps) || o.ps <= 0) return false;
+  if (o.rfast) {
+    if (o.ry != 1 || rows < 8 || (rows & 7)) return false;
+    if (rsplit && ((o.rdiv & 7) || !m8(o.rx))) return false;
+    if (!m8(o.ky) || (ksplit && !m8(o.kx))) return false;
+  } else {
+    if (o.ky != 1 || L < 8 || (L & 7)) return false;
+    if (ksplit && ((o.kdiv & 7) || !m8(o.kx))) return false;
+    if (!m8(o.ry) || (rsplit && !m8(o.rx))) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+bool gemm_b3p_ok(const GemmDesc &d, const GemmBatch &bt) {
+  if (d.M <= 0 || d.N <= 0 || d.nseg <= 0) return false;
+  for (int q = 0; q < d.nseg; ++q) {
+    const SegDesc &s = d.seg[q];
+    if (s.alpha != 1.0 && s.alpha != -1.0) return false;
+    if (!op_b3p_ok(s.A, d.M, s.L) || !op_b3p_ok(s.B, d.N, s.L)) return false;
+    if (bt.nb > 1 && ((bt.a[q] | bt.bo[q]) & 7)) return false;
+  }
+  return true;
+}
+
+bool gemm_b3p_enabled() {
+  static const bool on = [] {
+    const char *e = getenv("PYSCFAD_AMD_B3_DIRECT");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+// d: chunk_start in stages of 32; shape as in the register-staged engine
+hipError_t launch_gemm_b3p(const GemmDesc &d, int MT, int NT, hipStream_t s, const GemmBatch &bt) {
+  const unsigned grid =
+      (unsigned)((long long)((d.M + MT - 1) / MT) * ((d.N + NT - 1) / NT) * d.splits * bt.nb);
+  const int n_fast = d.M >= d.N ? 1 : 0;
+  if (MT == 128 && NT == 80)
+    hipLaunchKernelGGL((k_gemm_b3p<128, 80, 8, 1>), dim3(grid), dim3(kPThreads), 0, s, d, bt, n_fast);
+  else if (MT == 80 && NT == 128)
+    hipLaunchKernelGGL((k_gemm_b3p<80, 128, 1, 8>), dim3(grid), dim3(kPThreads), 0, s, d, bt, n_fast);
+  else if (MT == 128 && NT == 48)
+    hipLaunchKernelGGL((k_gemm_b3p<128, 48, 8, 1>), dim3(grid), dim3(kPThreads), 0, s, d, bt, n_fast);
+  else if (MT == 48 && NT == 128)
+    hipLaunchKernelGGL((k_gemm_b3p<48, 128, 1, 8>), dim3(grid), dim3(kPThreads), 0, s, d, bt, n_fast);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+}  // namespace pamd

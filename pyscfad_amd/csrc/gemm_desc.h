@@ -24,6 +24,7 @@ inline OpDesc op(const double *p, long long rx, long long ry, int rdiv, long lon
   d.kdiv = kdiv;
   d.rfast = rfast;
   d.pad_ = 0;
+  d.ps = 0;
   return d;
 }
 // row index is a single index with stride ry; K single with stride ky

@@ -20,7 +20,8 @@ BIG = 1 << 30
 class OpDesc(ctypes.Structure):
     _fields_ = [('p', ctypes.c_uint64), ('rx', ctypes.c_longlong), ('ry', ctypes.c_longlong),
                 ('kx', ctypes.c_longlong), ('ky', ctypes.c_longlong), ('rdiv', ctypes.c_int),
-                ('kdiv', ctypes.c_int), ('rfast', ctypes.c_int), ('pad_', ctypes.c_int)]
+                ('kdiv', ctypes.c_int), ('rfast', ctypes.c_int), ('pad_', ctypes.c_int),
+                ('ps', ctypes.c_longlong)]
 
 
 class SegDesc(ctypes.Structure):
