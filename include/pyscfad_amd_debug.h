@@ -25,6 +25,14 @@ int pamd_debug_gemm_descriptors(int nocc, int nvir, int i, int j, int k, int wit
  * 0 when it runs on the register-staged path. */
 int pamd_debug_gemm_direct_ok(const void *desc);
 
+/* 1 when a recorded descriptor, moved onto planar split-BF16x3 operands (each
+ * FP64 element address 8 e becomes the bf16 address 2 e, bases being multiples
+ * of 64 bytes, with plane stride plane_stride), runs on the split-BF16x3
+ * engine's direct-to-LDS path (16-byte runs of 8 bf16: fast strides 1, other
+ * strides, divisors, segment lengths or row counts multiples of 8), 0 when it
+ * runs on the register-staged path. */
+int pamd_debug_gemm_b3p_ok(const void *desc, long long plane_stride);
+
 /* Split-K factors the adjoint uses for (particle, hole) partial buffers. */
 int pamd_debug_split_factors(int nocc, int nvir, int *S_p, int *S_h);
 

@@ -31,7 +31,7 @@ EXPORTS = ('pamd_version', 'pamd_ccsdt_workspace_size', 'pamd_ccsdt_energy',
            'pamd_ccsdt_projected_energy_vjp', 'pamd_profile_enable',
            'pamd_profile_reset', 'pamd_profile_progress', 'pamd_profile_num_kinds', 'pamd_profile_kind_name',
            'pamd_profile_read', 'pamd_debug_gemm_descriptors', 'pamd_debug_split_factors',
-           'pamd_debug_split_factors_prec', 'pamd_debug_gemm_direct_ok',
+           'pamd_debug_split_factors_prec', 'pamd_debug_gemm_direct_ok', 'pamd_debug_gemm_b3p_ok',
            'pamd_bf16_gemm_tn', 'pamd_bf16_act_fwd', 'pamd_bf16_act_bwd', 'pamd_adam_step',
            'pamd_mse', 'pamd_f32_to_bf16', 'pamd_mp2_init_amps', 'pamd_df_eris_workspace_size',
            'pamd_df_eris', 'pamd_df_eris_vjp', 'pamd_debug_df_descriptors')
@@ -110,6 +110,8 @@ def load():
     lib.pamd_df_eris_vjp.argtypes = [i, i, i] + [vp] * 6 + [sz, vp]
     lib.pamd_debug_gemm_direct_ok.restype = i
     lib.pamd_debug_gemm_direct_ok.argtypes = [vp]
+    lib.pamd_debug_gemm_b3p_ok.restype = i
+    lib.pamd_debug_gemm_b3p_ok.argtypes = [vp, ctypes.c_longlong]
     lib.pamd_debug_df_descriptors.restype = i
     lib.pamd_debug_df_descriptors.argtypes = [i, i, i, i, i, vp, vp, i, ctypes.POINTER(i),
                                               ctypes.POINTER(i)]

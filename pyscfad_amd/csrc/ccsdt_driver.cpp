@@ -1221,6 +1221,19 @@ int pamd_debug_gemm_direct_ok(const void *desc) {
   return gemm_direct_ok(d) ? 1 : 0;
 }
 
+int pamd_debug_gemm_b3p_ok(const void *desc, long long plane_stride) {
+  if (!desc) return -PAMD_ERR_ARG;
+  GemmDesc d;
+  memcpy(&d, desc, sizeof(d));
+  for (int q = 0; q < d.nseg; ++q)
+    for (OpDesc *op : {&d.seg[q].A, &d.seg[q].B}) {
+      // FP64 element address e * 8 -> bf16 element address e * 2
+      op->p = (const double *)((uintptr_t)op->p / 4);
+      op->ps = plane_stride;
+    }
+  return gemm_b3p_ok(d) ? 1 : 0;
+}
+
 int pamd_debug_gemm_descriptors(int nocc, int nvir, int i, int j, int k, int with_vjp,
                                 const unsigned long long *bases, void *out, int max_desc,
                                 int *n_desc, int *desc_bytes) {

@@ -126,6 +126,32 @@ def test_direct_engine_eligibility():
     assert 0 in [lib.pamd_debug_gemm_direct_ok(ctypes.byref(d)) for d in descs]
 
 
+def test_bf16x3_direct_engine_eligibility():
+    """Which contractions run on the split-BF16x3 engine's direct-to-LDS path
+    (16-byte runs of 8 bf16 along each operand's fast index): every W-build and
+    adjoint contraction when nocc and nvir are multiples of 8 (the benchmark
+    and the adjoint-check sizes); when nocc is not (10, 60), the contractions
+    with hole terms stay on the register-staged path (row strides that are
+    multiples of nocc are not 16-byte aligned), while the ovvv and particle
+    t2 adjoint contractions still qualify."""
+    import ctypes
+    from pyscfad_amd import _lib
+    lib = _lib.load()
+
+    def ok(o, v):
+        return [lib.pamd_debug_gemm_b3p_ok(ctypes.byref(d), o * v ** 3)
+                for d in record(o, v, 0, 1, 2, with_vjp=True)]
+    for o, v in ((40, 400), (24, 240), (16, 160), (8, 40)):
+        assert ok(o, v) == [1] * len(ok(o, v)), (o, v)
+    # descriptor order: W build (3), ovvv adjoint (3), particle t2 (6), hole ovoo (6), hole t2 (3)
+    for o, v in ((10, 400), (60, 600)):
+        assert ok(o, v) == [0] * 3 + [1] * 9 + [0] * 9, (o, v)
+    assert 0 in ok(6, 20)
+    # a plane stride that breaks 16-byte alignment of the mid and lo planes
+    d = record(40, 400, 0, 1, 2, with_vjp=False)[0]
+    assert lib.pamd_debug_gemm_b3p_ok(ctypes.byref(d), 40 * 400 ** 3 + 4) == 0
+
+
 def _direct_stage_map(R, rfast):
     """LDS slot (in doubles) of each tile element (r, k), r < R, k < 16, as the
     direct-load plan of gemm_direct.hip places it: thread t, load q writes 16
