@@ -105,7 +105,7 @@ def engine_roofline(prof, precision):
     fp64_equiv = g_fl / (g_ms * 1e-3) / 1e12 if g_ms > 0 else None
     if precision == 'bf16x3':
         # six bf16 MFMA partial products per FP64-equivalent multiply-add
-        kernel, mfma_factor, peak = 'k_gemm_b3 / k_gemm_b3n', 6.0, PEAKS['bf16']
+        kernel, mfma_factor, peak = 'k_gemm_b3p (k_gemm_b3n for operands without 16-byte runs)', 6.0, PEAKS['bf16']
     elif precision == 'fp32x2':
         # three fp32 MFMA products (hi*hi, hi*lo, lo*hi) per multiply-add
         kernel, mfma_factor, peak = 'k_gemm_f64 (FP32 hi/lo mode)', 3.0, PEAKS['fp32']
@@ -260,6 +260,10 @@ def main():
     ap.add_argument('--no-df', action='store_true',
                     help='skip the timing of the DF integral producer and its adjoint')
     ap.add_argument('--energy-only', action='store_true', help='time E(T) without the VJP')
+    ap.add_argument('--walk-shards', type=int, default=1,
+                    help='time shard 0 of this many round-robin shards of the occupied-triple '
+                         'walk instead of the whole step (short runs under hardware counters; '
+                         'the line is then not the benchmark metric)')
     ap.add_argument('--out', default=None, help='also write the JSON line to this file')
     ap.add_argument('--calib-copy-mib', type=int, default=0,
                     help='run one plain device copy of this many MiB before the timed region '
@@ -333,6 +337,10 @@ def main():
     ws = inp.workspace(not args.energy_only)
 
     def step():
+        if args.walk_shards > 1:
+            if args.energy_only:
+                return _ops.energy_shard(inp, 0, args.walk_shards), None
+            return _ops.energy_vjp_shard(inp, 0, args.walk_shards, et_bar=1.0, workspace=ws)
         if args.energy_only:
             return _ops.energy(inp), None
         return _ops.energy_vjp(inp, et_bar=1.0, workspace=ws)
@@ -452,7 +460,9 @@ def main():
         'dtype': 'f64' if args.precision == 'fp64' else args.precision,
         'data': 'synthetic (seeded DF-like generator; see pyscfad_amd/gen.py)',
         'config': {'workload': f'RHF (T) E+VJP nocc={o} nvir={v} naux={naux} seed={args.seed}'
-                   + (' (energy only)' if args.energy_only else ''),
+                   + (' (energy only)' if args.energy_only else '')
+                   + (f' (walk shard 0 of {args.walk_shards}: not the metric)'
+                      if args.walk_shards > 1 else ''),
                    'precision': args.precision, 'ovvv': 'packed',
                    'warmup_problem': 'o3/v40/naux160 (library load, every kernel once)'},
         'e_t': e_val,

@@ -633,13 +633,18 @@ static int b3_pick(int M, int N) {
   return best;
 }
 
+// (sides that are multiples of 8 normally run on the direct engine, with its own tiles)
 int gemm_tiles_b3(int M, int N) {
+  if (gemm_b3p_enabled() && M % 8 == 0 && N % 8 == 0) return gemm_b3p_tiles(M, N);
   const int c = b3_pick(M, N);
   const int MT = kB3Shape[c][0], NT = kB3Shape[c][1];
   return ((M + MT - 1) / MT) * ((N + NT - 1) / NT);
 }
 
-int gemm_wgs_per_cu_b3(int M, int N) { return kB3Shape[b3_pick(M, N)][2]; }
+int gemm_wgs_per_cu_b3(int M, int N) {
+  if (gemm_b3p_enabled() && M % 8 == 0 && N % 8 == 0) return 2;
+  return kB3Shape[b3_pick(M, N)][2];
+}
 
 hipError_t launch_gemm_b3(const GemmDesc &d0, hipStream_t s, const GemmBatch &bt) {
   if (d0.M <= 0 || d0.N <= 0) return hipSuccess;
@@ -648,10 +653,9 @@ hipError_t launch_gemm_b3(const GemmDesc &d0, hipStream_t s, const GemmBatch &bt
   d.chunk_start[0] = 0;
   for (int q = 0; q < d.nseg; ++q) d.chunk_start[q + 1] = d.chunk_start[q] + (d.seg[q].L + BKC - 1) / BKC;
   for (int q = d.nseg + 1; q <= kMaxSeg; ++q) d.chunk_start[q] = d.chunk_start[d.nseg];
+  if (gemm_b3p_enabled() && gemm_b3p_ok(d, bt)) return launch_gemm_b3p(d, s, bt);
   const int c = b3_pick(d.M, d.N);
   const int MT = kB3Shape[c][0], NT = kB3Shape[c][1];
-  if (c < kB3Wide && gemm_b3p_enabled() && gemm_b3p_ok(d, bt))
-    return launch_gemm_b3p(d, MT, NT, s, bt);
   const unsigned grid =
       (unsigned)((long long)((d.M + MT - 1) / MT) * ((d.N + NT - 1) / NT) * d.splits * bt.nb);
   const int n_fast = d.M >= d.N ? 1 : 0;

@@ -152,11 +152,11 @@ hipError_t launch_split_bf16x3(const double *x, size_t n, unsigned short *out, h
 hipError_t launch_gemm_bf16x3(const GemmDesc &d, hipStream_t s, const GemmBatch &bt = no_batch());
 // split-BF16x3 engine with direct-to-LDS operand loads (gemm_b3p.hip), on
 // descriptors whose chunk_start counts stages of 32; launch_gemm_bf16x3 picks
-// it when gemm_b3p_ok(d) and the tile shape is one of its four
+// it when gemm_b3p_ok(d)
 bool gemm_b3p_ok(const GemmDesc &d, const GemmBatch &bt = no_batch());
 bool gemm_b3p_enabled();  // env PYSCFAD_AMD_B3_DIRECT=0 turns it off
-hipError_t launch_gemm_b3p(const GemmDesc &d, int MT, int NT, hipStream_t s,
-                           const GemmBatch &bt = no_batch());
+int gemm_b3p_tiles(int M, int N);  // workgroup tiles of one split
+hipError_t launch_gemm_b3p(const GemmDesc &d, hipStream_t s, const GemmBatch &bt = no_batch());
 hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s, const GemmBatch &bt = no_batch());
 // FP64 / FP32 hi/lo engine with direct-to-LDS operand loads (gemm_direct.hip):
 // usable when gemm_direct_ok(d); launch_gemm_f64 / launch_gemm_f32x2 pick it then

@@ -19,6 +19,7 @@
 //        ovov_bar, t1_bar, fvo_bar -- Vbar contractions (k_vr_*) + gather.
 #include <hip/hip_runtime.h>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <algorithm>
 #include <vector>
@@ -142,6 +143,9 @@ struct Layout {
   size_t off_Vp, off_T2m, off_OOt, off_OVS, off_T2S, off_orb, off_W, off_tri, off_part_t, off_T2g, off_W8;
   int wbatch;  // W blocks of the energy walk's batched W build (1 with the adjoint)
   size_t off_Vb, off_Vpb, off_T2mb, off_part_mo, off_R, off_part_p, off_part_h, off_wt;
+  // pipelined adjoint walk: second copies of the per-triple buffers
+  int pipe;
+  size_t off_W2, off_W8b, off_R2, off_part_p2, off_part_h2;
   size_t total;
   int S_p, S_h;
 };
@@ -199,7 +203,9 @@ static int energy_batch(int o, int v, int prec) {
 // Vp, T2m and OOt hold FP64 copies, or their split forms: three bf16 planes
 // (split-BF16x3, 6 of the 8 bytes per element) or packed FP32 hi/lo pairs; the
 // split precisions add split copies of t2 and of Wbar.
-static Layout make_layout(int o, int v, int vjp, int prec) {
+// pipe: the adjoint walk runs two triples in flight (run_vjp_pipelined) and
+// needs second copies of W, Wbar, the V-bar scratch and the split-K partials
+static Layout make_layout(int o, int v, int vjp, int prec, bool pipe = false) {
   Layout L;
   memset(&L, 0, sizeof(L));
   const size_t d = sizeof(double);
@@ -240,6 +246,14 @@ static Layout make_layout(int o, int v, int vjp, int prec) {
     L.off_part_p = take((size_t)6 * L.S_p * v2);
     L.off_part_h = take((size_t)6 * L.S_h * v * o);
   }
+  L.pipe = (vjp && pipe) ? 1 : 0;
+  if (L.pipe) {
+    L.off_W2 = take(v3);
+    if (prec == PAMD_PREC_BF16X3 || prec == PAMD_PREC_FP32X2) L.off_W8b = take(v3);
+    L.off_R2 = take(vr_scratch_doubles(v));
+    L.off_part_p2 = take((size_t)6 * L.S_p * v2);
+    L.off_part_h2 = take((size_t)6 * L.S_h * v * o);
+  }
   L.total = off;
   return L;
 }
@@ -272,6 +286,9 @@ struct Ctx {
   const double *wt_host;  // occupied weights m (host) of a diagonal projector, or null
   double *wt;             // device copy of m
   PlanarMap planes;       // split-BF16x3 operand arrays
+  // pipelined adjoint walk (pipe = 1): the second set of per-triple buffers
+  int pipe;
+  double *W2, *gW2, *R2, *part_p2, *part_h2;
   // outputs
   double *t1_bar, *t2_bar, *ovoo_bar, *ovov_bar, *fvo_bar;
 };
@@ -384,21 +401,13 @@ static int build_w(const Ctx &c, int i, int j, int k, int nb = 1) {
 }
 
 // ---------------------------------------------------------------------------
-// adjoint contractions for one triple (c.W holds Wbar, c.Vb holds Vbar)
+// adjoint of one triple (c.gW holds Wbar, c.Vb holds Vbar): the V-bar
+// reductions into c.R, the adjoint contractions (stream c.s) into Vpb, T2mb
+// and the split-K partials c.part_p, c.part_h, and the gather of R and the
+// partials into the cotangents
 // ---------------------------------------------------------------------------
-static int adjoint(const Ctx &c, int i, int j, int k) {
-  const long long o = c.o, v = c.v;
-  const long long v2 = v * v, v3 = v2 * v, vo = v * o;
-  const int V = c.v, O = c.o;
-  const double *Wb = c.gW;
-  auto Vp = [&](int x) { return c.Vp + x * v3; };
-  auto Vpb = [&](int x) { return c.Vpb + x * v3; };
-  auto T2 = [&](int x, int y) { return c.gT2 + (x * o + y) * v2; };
-  auto T2m = [&](int x) { return c.T2m + x * v2 * o; };
-  auto T2mb = [&](int x) { return c.T2mb + x * v2 * o; };
-  auto OOt = [&](int x, int y) { return c.OOt + (x * o + y) * vo; };
-
-  // V-term contractions (needs Vbar only)
+static int adjoint_vr(const Ctx &c, int i, int j, int k, hipStream_t st) {
+  const double v3 = (double)c.v * c.v * c.v;
   VRArgs vr;
   vr.Vb = c.Vb;
   vr.t1 = c.t1;
@@ -413,9 +422,23 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
   vr.has_fvo = c.has_fvo;
   vr.R = c.R;
   if (!g_record) {
-    const double vb = 2.0 * 8.0 * (double)v3;  // Vbar read twice
-    PAMD_CHECK(timed(c.s, kProfVR, 0.0, vb, [&] { return launch_vr(vr, c.s); }));
+    const double vb = 2.0 * 8.0 * v3;  // Vbar read twice
+    PAMD_CHECK(timed(st, kProfVR, 0.0, vb, [&] { return launch_vr(vr, st); }));
   }
+  return PAMD_OK;
+}
+
+static int adjoint_gemms(const Ctx &c, int i, int j, int k) {
+  const long long o = c.o, v = c.v;
+  const long long v2 = v * v, v3 = v2 * v, vo = v * o;
+  const int V = c.v, O = c.o;
+  const double *Wb = c.gW;
+  auto Vp = [&](int x) { return c.Vp + x * v3; };
+  auto Vpb = [&](int x) { return c.Vpb + x * v3; };
+  auto T2 = [&](int x, int y) { return c.gT2 + (x * o + y) * v2; };
+  auto T2m = [&](int x) { return c.T2m + x * v2 * o; };
+  auto T2mb = [&](int x) { return c.T2mb + x * v2 * o; };
+  auto OOt = [&](int x, int y) { return c.OOt + (x * o + y) * vo; };
 
   // ovvv_bar, rows = virtual pairs, cols = f
   {  // Vpb[i][a][b|c][f]: P1 (K=c), P2 (K=b)
@@ -512,7 +535,11 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
                       out(T2mb(k), o, vo, V, 0, 1, kBig));
     PAMD_CHECK(run_gemm(c, d, kProfAdjT2H));
   }
-  // gather into the cotangents
+  return PAMD_OK;
+}
+
+static int adjoint_combine(const Ctx &c, int i, int j, int k, hipStream_t st) {
+  const double v = c.v, o = c.o, v2 = v * v;
   CombineArgs ca;
   ca.R = c.R;
   ca.part_p = c.part_p;
@@ -532,9 +559,16 @@ static int adjoint(const Ctx &c, int i, int j, int k) {
   ca.fvo_bar = c.fvo_bar;
   if (!g_record) {
     const double cb = 8.0 * (6.0 * c.S_p * v2 + 6.0 * c.S_h * v * o + 12.0 * v2);
-    PAMD_CHECK(timed(c.s, kProfCombine, 0.0, cb, [&] { return launch_combine(ca, c.s); }));
+    PAMD_CHECK(timed(st, kProfCombine, 0.0, cb, [&] { return launch_combine(ca, st); }));
   }
   return PAMD_OK;
+}
+
+static int adjoint(const Ctx &c, int i, int j, int k) {
+  int st = adjoint_vr(c, i, j, k, c.s);
+  if (st == PAMD_OK) st = adjoint_gemms(c, i, j, k);
+  if (st == PAMD_OK) st = adjoint_combine(c, i, j, k, c.s);
+  return st;
 }
 
 // operand format of a precision
@@ -696,6 +730,152 @@ static int run_projected(Ctx &c, const double *ulo, int nfrag, double *accA, dou
   return PAMD_OK;
 }
 
+// Side stream and events of the pipelined adjoint walk, per host thread and
+// device: created once (default priority, non-blocking) and kept for the life
+// of the process.
+struct PipeRes {
+  hipStream_t e;
+  hipEvent_t ev[10];  // start, end, W built [2], V-bar reduced [2], adjoint done [2], gathered [2]
+  bool ok;
+};
+
+static PipeRes *pipe_resources() {
+  constexpr int kMaxDev = 64;
+  static thread_local PipeRes res[kMaxDev];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return nullptr;
+  PipeRes &r = res[dev];
+  if (!r.ok) {
+    if (hipStreamCreateWithFlags(&r.e, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    for (hipEvent_t &e : r.ev)
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    r.ok = true;
+  }
+  return &r;
+}
+
+// The adjoint walk with two triples in flight.  The contractions (W builds
+// and adjoint GEMMs) run in order on the call's stream G; the memory-bound
+// epilogue of each triple (orbit epilogue, triple energy, V-bar reductions,
+// gather into the cotangents) runs on the side stream E while G builds the
+// next triple's W:
+//   G: WB(0) WB(1) ADJ(0) WB(2) ADJ(1) WB(3) ADJ(2) ...
+//   E:   ORB(0) VR(0) | COMB(0) ORB(1) VR(1) | COMB(1) ...
+// Triple t uses buffer set t % 2 (W, Wbar, V-bar scratch R, split-K
+// partials).  Waits: ORB(t) after WB(t); ADJ(t) after VR(t) and after COMB(t-2)
+// (which read the partials ADJ(t) overwrites); COMB(t) after ADJ(t).  The
+// orders within G and E cover the rest: WB(t+2) follows ADJ(t), which followed
+// ORB(t) (the last reader of W[t % 2]); ORB(t+2) follows COMB(t+1) and so
+// ADJ(t) (the last reader of Wbar[t % 2]); VR(t+2) follows COMB(t) (the last
+// reader of R[t % 2]).  The accumulators shared by all triples (Vpb and T2mb
+// on G; the cotangents, triple energies and dE/de partials on E) are each
+// written by one stream, in walk order, so the sums are the sequential walk's.
+static int run_vjp_pipelined(Ctx &c, long long norb, PipeRes &pr) {
+  const int o = c.o, fmt = split_format(c.prec);
+  std::vector<int> tr;  // triples of this call in walk order, i | j << 10 | k << 20
+  {
+    long long windex = -1;
+    for (int i = 0; i < o; ++i)
+      for (int j = i; j < o; ++j)
+        for (int k = j; k < o; ++k) {
+          if (i == j && j == k) continue;
+          ++windex;
+          if (c.nshards > 1 && windex % c.nshards != c.shard) continue;
+          tr.push_back(i | (j << 10) | (k << 20));
+        }
+  }
+  Ctx cx[2] = {c, c};
+  cx[1].W = c.W2;
+  cx[1].gW = c.gW2;
+  cx[1].R = c.R2;
+  cx[1].part_p = c.part_p2;
+  cx[1].part_h = c.part_h2;
+  for (int a = 0; a < cx[1].planes.count; ++a)
+    if (cx[1].planes.base[a] == (const char *)c.gW) cx[1].planes.base[a] = (const char *)c.gW2;
+  const hipStream_t G = c.s, E = pr.e;
+  hipEvent_t *wb = pr.ev + 2, *vr = pr.ev + 4, *adj = pr.ev + 6, *cmb = pr.ev + 8;
+  PAMD_CHECK(hipEventRecord(pr.ev[0], G));
+  PAMD_CHECK(hipStreamWaitEvent(E, pr.ev[0], 0));
+  const long long n = (long long)tr.size();
+  const long long ntrip = (long long)o * (o + 1) * (o + 2) / 6 - o;
+  const auto t_start = std::chrono::steady_clock::now();
+  auto walk = [&]() -> int {
+    if (n == 0) return PAMD_OK;
+    auto ijk = [&](long long t, int &i, int &j, int &k) {
+      i = tr[t] & 1023;
+      j = (tr[t] >> 10) & 1023;
+      k = tr[t] >> 20;
+    };
+    int i, j, k, st;
+    g_prof_on = g_prof_every > 0;
+    ijk(0, i, j, k);
+    st = build_w(cx[0], i, j, k);
+    if (st != PAMD_OK) return st;
+    PAMD_CHECK(hipEventRecord(wb[0], G));
+    for (long long t = 0; t < n; ++t) {
+      const int p = (int)(t & 1);
+      const Ctx &C = cx[p];
+      g_prof_on = g_prof_every > 0 && (t % g_prof_every) == 0;
+      ijk(t, i, j, k);
+      // E: epilogue and V-bar reductions of t
+      PAMD_CHECK(hipStreamWaitEvent(E, wb[p], 0));
+      OrbitArgs oa = orbit_args(C, i, j, k, norb);
+      wbar_out(C, fmt, oa);
+      oa.vjp = 1;
+      oa.omega = (i == j || j == k) ? 1.0 : 2.0;
+      oa.ebar = c.wt_host ? c.ebar * ((c.wt_host[i] + c.wt_host[j] + c.wt_host[k]) / 3.0) : c.ebar;
+      {
+        const double ob = 8.0 * (double)c.v * c.v * c.v * 3.0;  // W read, Wbar and Vbar written
+        PAMD_CHECK(timed(E, kProfOrbit, 0.0, ob, [&] { return launch_orbit(oa, C.G, E); }));
+      }
+      PAMD_CHECK(launch_triple_energy(C.part_t, C.G, C.tri + ((size_t)i * o + j) * o + k, E));
+      st = adjoint_vr(C, i, j, k, E);
+      if (st != PAMD_OK) return st;
+      PAMD_CHECK(hipEventRecord(vr[p], E));
+      // G: W build of t + 1
+      if (t + 1 < n) {
+        int i1, j1, k1;
+        ijk(t + 1, i1, j1, k1);
+        st = build_w(cx[1 - p], i1, j1, k1);
+        if (st != PAMD_OK) return st;
+        PAMD_CHECK(hipEventRecord(wb[1 - p], G));
+      }
+      // G: adjoint contractions of t
+      PAMD_CHECK(hipStreamWaitEvent(G, vr[p], 0));
+      if (t >= 2) PAMD_CHECK(hipStreamWaitEvent(G, cmb[p], 0));
+      st = adjoint_gemms(C, i, j, k);
+      if (st != PAMD_OK) return st;
+      PAMD_CHECK(hipEventRecord(adj[p], G));
+      // E: gather of t
+      PAMD_CHECK(hipStreamWaitEvent(E, adj[p], 0));
+      st = adjoint_combine(C, i, j, k, E);
+      if (st != PAMD_OK) return st;
+      PAMD_CHECK(hipEventRecord(cmb[p], E));
+      if (g_progress_every > 0 && (t + 1) % g_progress_every == 0) {
+        PAMD_CHECK(hipStreamSynchronize(E));
+        PAMD_CHECK(hipStreamSynchronize(G));
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() -
+                                                         t_start).count();
+        fprintf(stderr, "pyscfad_amd: (T)+VJP triples %lld/%lld done, %.1f s\n", t + 1,
+                c.nshards > 1 ? n : ntrip, el);
+        fflush(stderr);
+      }
+    }
+    return PAMD_OK;
+  };
+  int st = walk();
+  // the call's stream waits for the side stream, also after an error
+  hipError_t e = hipEventRecord(pr.ev[1], E);
+  if (e == hipSuccess) e = hipStreamWaitEvent(G, pr.ev[1], 0);
+  if (e != hipSuccess && st == PAMD_OK) {
+    fprintf(stderr, "pyscfad_amd: joining the side stream failed: %s\n", hipGetErrorString(e));
+    st = PAMD_ERR_HIP;
+  }
+  g_prof_on = false;
+  if (st == PAMD_OK && !g_prof_recs.empty()) PAMD_CHECK(prof_collect());
+  return st;
+}
+
 static int run(Ctx &c) {
   const int o = c.o, v = c.v;
   const int fmt = split_format(c.prec);
@@ -707,6 +887,14 @@ static int run(Ctx &c) {
   if (c.vjp) {
     const int st0 = zero_vjp(c);
     if (st0 != PAMD_OK) return st0;
+  }
+  static const bool pipe_on = [] {
+    const char *e = getenv("PYSCFAD_AMD_PIPELINE");
+    return !(e && e[0] == '0');
+  }();
+  if (c.vjp && c.pipe && pipe_on && !g_record) {
+    PipeRes *pr = pipe_resources();
+    if (pr) return run_vjp_pipelined(c, norb, *pr);
   }
   long long tcount = 0, windex = -1;
   const long long ntrip = (long long)o * (o + 1) * (o + 2) / 6 - o;
@@ -845,6 +1033,14 @@ static void bind_workspace(Ctx &c, const Layout &L, void *ws) {
     c.R = (double *)(b + L.off_R);
     c.part_p = (double *)(b + L.off_part_p);
     c.part_h = (double *)(b + L.off_part_h);
+    c.pipe = L.pipe;
+    if (L.pipe) {
+      c.W2 = (double *)(b + L.off_W2);
+      c.gW2 = packed_ops ? (double *)(b + L.off_W8b) : c.W2;
+      c.R2 = (double *)(b + L.off_R2);
+      c.part_p2 = (double *)(b + L.off_part_p2);
+      c.part_h2 = (double *)(b + L.off_part_h2);
+    }
   } else {
     c.Vb = c.Vpb = c.T2mb = c.part_mo = c.R = c.part_p = c.part_h = nullptr;
   }
@@ -902,7 +1098,7 @@ const char *pamd_version(void) { return "pyscfad_amd 0.1.0 (gfx950)"; }
 
 int pamd_ccsdt_workspace_size(const pamd_ccsdt_config *cfg, int with_vjp, size_t *bytes) {
   if (!cfg || !bytes || cfg->nocc < 1 || cfg->nvir < 1) return PAMD_ERR_ARG;
-  *bytes = make_layout(cfg->nocc, cfg->nvir, with_vjp ? 1 : 0, cfg->precision).total;
+  *bytes = make_layout(cfg->nocc, cfg->nvir, with_vjp ? 1 : 0, cfg->precision, true).total;
   return PAMD_OK;
 }
 
@@ -990,13 +1186,13 @@ static int vjp_impl(const pamd_ccsdt_config *cfg, const double *mo_energy, const
                     size_t workspace_bytes, double *e_out, double *mo_energy_bar, double *t1_bar,
                     double *t2_bar, double *ovvv_bar, double *ovoo_bar, double *ovov_bar,
                     double *fvo_bar, double *wt_bar, void *stream, int shard = 0,
-                    int nshards = 1) {
+                    int nshards = 1, bool pipe = true) {
   int st = check_common(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo);
   if (st != PAMD_OK) return st;
   if (!e_out || !mo_energy_bar || !t1_bar || !t2_bar || !ovvv_bar || !ovoo_bar || !ovov_bar ||
       !fvo_bar)
     return PAMD_ERR_ARG;
-  Layout L = make_layout(cfg->nocc, cfg->nvir, 1, cfg->precision);
+  Layout L = make_layout(cfg->nocc, cfg->nvir, 1, cfg->precision, pipe);
   if (!workspace || workspace_bytes < L.total) return PAMD_ERR_WORKSPACE;
   Ctx c;
   memset(&c, 0, sizeof(c));
@@ -1088,9 +1284,10 @@ int pamd_ccsdt_energy_vjp_noalloc(const pamd_ccsdt_config *cfg, const double *mo
   st = scratch_on_current_device(&ws);
   if (st != PAMD_OK) return st;
   g_prof_blocked = true;
+  // one triple in flight: no second stream, events or buffer copies
   st = vjp_impl(cfg, mo_energy, t1, t2, ovvv, ovoo, ovov, fvo, nullptr, et_bar, ws,
                 kNoAllocScratchBytes, e_out, mo_energy_bar, t1_bar, t2_bar, ovvv_bar, ovoo_bar,
-                ovov_bar, fvo_bar, nullptr, stream);
+                ovov_bar, fvo_bar, nullptr, stream, 0, 1, false);
   g_prof_blocked = false;
   return st;
 }

@@ -55,7 +55,6 @@ typedef unsigned short bf16_t;
 
 constexpr int kPK = 32;      // k per stage: one 16x16x32 MFMA step
 constexpr int kPStages = 2;
-constexpr int kPThreads = 512;
 
 // s_waitcnt with only the vector-memory counter constrained (gfx9 encoding:
 // vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
@@ -73,21 +72,22 @@ __device__ __forceinline__ int rsw(int kk) {
   return 0;  // 2R = 32 x odd bytes per k-row: consecutive k-rows already fall in distinct banks
 }
 
-// per-thread load plan of one operand (R rows, 3 planes) for one segment
-template <int R>
+// per-thread load plan of one operand (R rows, 3 planes) for one segment, in
+// a workgroup of T threads
+template <int R, int T>
 struct PPlan {
-  static constexpr int CH = 12 * R;                                   // chunks per stage
-  static constexpr int NI = (CH + kPThreads - 1) / kPThreads;         // loads per thread
-  static constexpr int SLOTS = NI * kPThreads;                        // LDS chunks (with padding)
+  static constexpr int CH = 12 * R;             // chunks per stage
+  static constexpr int NI = (CH + T - 1) / T;   // loads per thread
+  static constexpr int SLOTS = NI * T;          // LDS chunks (with padding)
   const bf16_t *ptr[NI];  // source of load q at the segment's k = 0, used by full stages
 };
 
 // slot of load q: plane and, within the plane's tile, the row (K-contiguous)
 // or first row of 8 (row-contiguous) rl, and the chunk-local k kl.  Loads past
 // the 12 R chunks fill padding with a valid chunk.
-template <int R>
+template <int R, int T>
 __device__ __forceinline__ void pslot(int q, int tid, int rc, int &plane, int &rl, int &kl) {
-  const int id = q * kPThreads + tid;
+  const int id = q * T + tid;
   if (id >= 12 * R) {
     plane = 0;
     rl = 0;
@@ -125,40 +125,40 @@ __device__ __forceinline__ long long pkoff(const OpDesc &d, int rc, int L, int k
   return (long long)q * d.kx + (long long)(k - q * d.kdiv) * d.ky;
 }
 
-template <int R>
+template <int R, int T>
 __device__ __forceinline__ void pplan(const OpDesc &d, int rc, int Rtot, int r0, int tid,
-                                      PPlan<R> &pl) {
+                                      PPlan<R, T> &pl) {
   const bf16_t *p = reinterpret_cast<const bf16_t *>(d.p);
 #pragma unroll
-  for (int q = 0; q < PPlan<R>::NI; ++q) {
+  for (int q = 0; q < PPlan<R, T>::NI; ++q) {
     int plane, rl, kl;
-    pslot<R>(q, tid, rc, plane, rl, kl);
+    pslot<R, T>(q, tid, rc, plane, rl, kl);
     pl.ptr[q] = p + plane * d.ps + prow(d, rc, Rtot, r0 + rl) + (long long)kl * d.ky;
   }
 }
 
 // the loads of one stage (k = kappa .. kappa + 31, kp = divmod(kappa, kdiv))
-template <int R>
+template <int R, int T>
 __device__ __forceinline__ void pissue(const OpDesc &d, int rc, int Rtot, int r0, int tid, int L,
-                                       int kappa, KPos kp, const PPlan<R> &pl, bf16_t *lds,
+                                       int kappa, KPos kp, const PPlan<R, T> &pl, bf16_t *lds,
                                        int wave) {
   const bool full = kappa + kPK <= L && (d.kdiv >= (1 << 30) || kp.r + kPK <= d.kdiv);
   if (full) {
     const long long kb = (long long)kp.q * d.kx + (long long)kp.r * d.ky;  // wave-uniform
 #pragma unroll
-    for (int q = 0; q < PPlan<R>::NI; ++q)
+    for (int q = 0; q < PPlan<R, T>::NI; ++q)
       __builtin_amdgcn_global_load_lds((const void *)(pl.ptr[q] + kb),
-                                       (lds_ptr_t)(lds + (q * kPThreads + wave * 64) * 8), 16, 0, 0);
+                                       (lds_ptr_t)(lds + (q * T + wave * 64) * 8), 16, 0, 0);
     return;
   }
   const bf16_t *p = reinterpret_cast<const bf16_t *>(d.p);
 #pragma unroll
-  for (int q = 0; q < PPlan<R>::NI; ++q) {
+  for (int q = 0; q < PPlan<R, T>::NI; ++q) {
     int plane, rl, kl;
-    pslot<R>(q, tid, rc, plane, rl, kl);
+    pslot<R, T>(q, tid, rc, plane, rl, kl);
     const bf16_t *src = p + plane * d.ps + prow(d, rc, Rtot, r0 + rl) + pkoff(d, rc, L, kappa + kl);
     __builtin_amdgcn_global_load_lds((const void *)src,
-                                     (lds_ptr_t)(lds + (q * kPThreads + wave * 64) * 8), 16, 0, 0);
+                                     (lds_ptr_t)(lds + (q * T + wave * 64) * 8), 16, 0, 0);
   }
 }
 
@@ -206,14 +206,15 @@ __device__ __forceinline__ void frags(const bf16_t *T, int rc, int r16, int lane
   }
 }
 
-template <int MT, int NT, int WM, int WN>
-__global__ __launch_bounds__(kPThreads, 4) void k_gemm_b3p(GemmDesc d, GemmBatch bt, int n_fast) {
-  static_assert(WM * WN == 8, "8 waves");
+// WM x WN waves; two workgroups per CU (80 KB of LDS each at most)
+template <int MT, int NT, int WM, int WN, int MINW>
+__global__ __launch_bounds__(64 * WM * WN, MINW) void k_gemm_b3p(GemmDesc d, GemmBatch bt, int n_fast) {
+  constexpr int T = 64 * WM * WN;
   constexpr int WTM = MT / WM, WTN = NT / WN;
   constexpr int TM = WTM / 16, TN = WTN / 16;
   static_assert(TM * 16 == WTM && TN * 16 == WTN, "16x16 MFMA tiles");
-  constexpr int NIA = PPlan<MT>::NI, NIB = PPlan<NT>::NI;
-  constexpr int SA = PPlan<MT>::SLOTS * 8, SB = PPlan<NT>::SLOTS * 8, SS = SA + SB;  // bf16
+  constexpr int SA = PPlan<MT, T>::SLOTS * 8, SB = PPlan<NT, T>::SLOTS * 8, SS = SA + SB;  // bf16
+  static_assert(2 * kPStages * SS * 2 <= 160 * 1024, "LDS of two workgroups per CU");
   __shared__ __align__(16) bf16_t S[kPStages * SS];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -249,10 +250,10 @@ __global__ __launch_bounds__(kPThreads, 4) void k_gemm_b3p(GemmDesc d, GemmBatch
   FA.p = reinterpret_cast<const double *>(reinterpret_cast<const bf16_t *>(FA.p) + batch * bt.a[fs]);
   FB.p = reinterpret_cast<const double *>(reinterpret_cast<const bf16_t *>(FB.p) + batch * bt.bo[fs]);
   int FL = d.seg[fs].L, fend = d.chunk_start[fs + 1];
-  PPlan<MT> pa;
-  PPlan<NT> pb;
-  pplan<MT>(FA, FA.rfast, d.M, m0, tid, pa);
-  pplan<NT>(FB, FB.rfast, d.N, n0, tid, pb);
+  PPlan<MT, T> pa;
+  PPlan<NT, T> pb;
+  pplan<MT, T>(FA, FA.rfast, d.M, m0, tid, pa);
+  pplan<NT, T>(FB, FB.rfast, d.N, n0, tid, pb);
   // compute tracker
   int cs = fs, ckappa = fkappa, cend = d.chunk_start[fs + 1];
   int la = d.seg[fs].A.rfast, lb = d.seg[fs].B.rfast, cL = d.seg[fs].L;
@@ -260,8 +261,8 @@ __global__ __launch_bounds__(kPThreads, 4) void k_gemm_b3p(GemmDesc d, GemmBatch
 
   auto issue = [&](int g) {  // loads of stage g (the fetch tracker's stage)
     bf16_t *st = S + ((g - g0) % kPStages) * SS;
-    pissue<MT>(FA, FA.rfast, d.M, m0, tid, FL, fkappa, fka, pa, st, wave);
-    pissue<NT>(FB, FB.rfast, d.N, n0, tid, FL, fkappa, fkb, pb, st + SA, wave);
+    pissue<MT, T>(FA, FA.rfast, d.M, m0, tid, FL, fkappa, fka, pa, st, wave);
+    pissue<NT, T>(FB, FB.rfast, d.N, n0, tid, FL, fkappa, fkb, pb, st + SA, wave);
     if (g + 1 < g1) {
       if (g + 1 >= fend) {
         while (g + 1 >= d.chunk_start[fs + 1]) ++fs;  // (skips segments of length 0)
@@ -276,8 +277,8 @@ __global__ __launch_bounds__(kPThreads, 4) void k_gemm_b3p(GemmDesc d, GemmBatch
         fkappa = 0;
         fka = KPos{0, 0};
         fkb = KPos{0, 0};
-        pplan<MT>(FA, FA.rfast, d.M, m0, tid, pa);
-        pplan<NT>(FB, FB.rfast, d.N, n0, tid, pb);
+        pplan<MT, T>(FA, FA.rfast, d.M, m0, tid, pa);
+        pplan<NT, T>(FB, FB.rfast, d.N, n0, tid, pb);
       } else {
         fkappa += kPK;
         kadvance(fka, FA.kdiv);
@@ -455,21 +456,55 @@ bool gemm_b3p_enabled() {
   return on;
 }
 
-// d: chunk_start in stages of 32; shape as in the register-staged engine
-hipError_t launch_gemm_b3p(const GemmDesc &d, int MT, int NT, hipStream_t s, const GemmBatch &bt) {
+// fraction of a padded MT x NT tiling that is real output
+static double b3p_eff(int M, int N, int MT, int NT) {
+  const double pm = (double)((M + MT - 1) / MT) * MT, pn = (double)((N + NT - 1) / NT) * NT;
+  return (double)M * N / (pm * pn);
+}
+
+// Tile shapes {MT, NT}: 8-wave 128 x 80 / 128 x 48 (and transposes), and
+// 5-wave 80 x 48 (and transpose) for outputs with one side a multiple of 80
+// and the other short (400 x 40 at nvir = 400, nocc = 40: 83% of the padded
+// tiling is real output, against 65% with 128 x 48).  The pick maximises the
+// padded tiling's efficiency times the tile's flops per staged byte,
+// MT NT / (MT + NT).  (A 5-wave 80 x 80 tile for 400 x 400 was measured at
+// half the 128 x 80 rate: ten waves per CU instead of sixteen.)
+static const int kB3pShape[][2] = {{128, 80}, {80, 128}, {128, 48}, {48, 128}, {80, 48}, {48, 80}};
+
+int gemm_b3p_pick(int M, int N) {
+  int best = 0;
+  double bs = -1.0;
+  for (int c = 0; c < (int)(sizeof(kB3pShape) / sizeof(kB3pShape[0])); ++c) {
+    const int MT = kB3pShape[c][0], NT = kB3pShape[c][1];
+    const double sc = b3p_eff(M, N, MT, NT) * (double)MT * NT / (MT + NT);
+    if (sc > bs + 1e-9) {
+      bs = sc;
+      best = c;
+    }
+  }
+  return best;
+}
+
+int gemm_b3p_tiles(int M, int N) {
+  const int c = gemm_b3p_pick(M, N);
+  return ((M + kB3pShape[c][0] - 1) / kB3pShape[c][0]) * ((N + kB3pShape[c][1] - 1) / kB3pShape[c][1]);
+}
+
+// d: chunk_start in stages of 32
+hipError_t launch_gemm_b3p(const GemmDesc &d, hipStream_t s, const GemmBatch &bt) {
+  const int c = gemm_b3p_pick(d.M, d.N);
+  const int MT = kB3pShape[c][0], NT = kB3pShape[c][1];
   const unsigned grid =
       (unsigned)((long long)((d.M + MT - 1) / MT) * ((d.N + NT - 1) / NT) * d.splits * bt.nb);
   const int n_fast = d.M >= d.N ? 1 : 0;
-  if (MT == 128 && NT == 80)
-    hipLaunchKernelGGL((k_gemm_b3p<128, 80, 8, 1>), dim3(grid), dim3(kPThreads), 0, s, d, bt, n_fast);
-  else if (MT == 80 && NT == 128)
-    hipLaunchKernelGGL((k_gemm_b3p<80, 128, 1, 8>), dim3(grid), dim3(kPThreads), 0, s, d, bt, n_fast);
-  else if (MT == 128 && NT == 48)
-    hipLaunchKernelGGL((k_gemm_b3p<128, 48, 8, 1>), dim3(grid), dim3(kPThreads), 0, s, d, bt, n_fast);
-  else if (MT == 48 && NT == 128)
-    hipLaunchKernelGGL((k_gemm_b3p<48, 128, 1, 8>), dim3(grid), dim3(kPThreads), 0, s, d, bt, n_fast);
-  else
-    return hipErrorInvalidValue;
+  switch (c) {
+    case 0: hipLaunchKernelGGL((k_gemm_b3p<128, 80, 8, 1, 4>), dim3(grid), dim3(512), 0, s, d, bt, n_fast); break;
+    case 1: hipLaunchKernelGGL((k_gemm_b3p<80, 128, 1, 8, 4>), dim3(grid), dim3(512), 0, s, d, bt, n_fast); break;
+    case 2: hipLaunchKernelGGL((k_gemm_b3p<128, 48, 8, 1, 4>), dim3(grid), dim3(512), 0, s, d, bt, n_fast); break;
+    case 3: hipLaunchKernelGGL((k_gemm_b3p<48, 128, 1, 8, 4>), dim3(grid), dim3(512), 0, s, d, bt, n_fast); break;
+    case 4: hipLaunchKernelGGL((k_gemm_b3p<80, 48, 5, 1, 3>), dim3(grid), dim3(320), 0, s, d, bt, n_fast); break;
+    default: hipLaunchKernelGGL((k_gemm_b3p<48, 80, 3, 1, 2>), dim3(grid), dim3(192), 0, s, d, bt, n_fast); break;
+  }
   return hipGetLastError();
 }
 
