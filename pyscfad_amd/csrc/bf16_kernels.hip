@@ -36,6 +36,59 @@ hipError_t launch_split_bf16x3(const double *x, size_t n, unsigned short *out, h
   return hipGetLastError();
 }
 
+// VpT[x][f][p][q] (mode 0) or VpU[x][f][q][p] (mode 1) from Vp[x][p][q][f]; all
+// three are planar split-BF16x3 arrays of o v^3 elements (plane stride o v^3).
+// One 64 x 64 tile transpose through LDS per block: y is the index that keeps
+// its place (p for mode 0, q for mode 1), a the one that moves innermost;
+// reads run along f and writes along a, 64 consecutive bf16 per wave.
+__global__ __launch_bounds__(256) void k_vp_transpose(const unsigned short *__restrict__ src,
+                                                      int o, int v, int mode,
+                                                      unsigned short *__restrict__ dst) {
+  __shared__ unsigned short t[3][64][65];
+  const long long v2 = (long long)v * v, v3 = v2 * v, n = (long long)o * v3;
+  const int nt = (v + 63) / 64;
+  long long b = blockIdx.x;
+  const int tf = (int)(b % nt);
+  b /= nt;
+  const int ta = (int)(b % nt);
+  b /= nt;
+  const int y = (int)(b % v);
+  const long long x = b / v;
+  const int a0 = ta * 64, f0 = tf * 64;
+  const unsigned short *S = src + x * v3;
+  unsigned short *D = dst + x * v3;
+  for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
+    const int al = e >> 6, fl = e & 63;
+    const int a = a0 + al, f = f0 + fl;
+    if (a < v && f < v) {
+      const long long so = (mode == 0 ? y * v2 + a * (long long)v : a * v2 + y * (long long)v) + f;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) t[q][al][fl] = S[so + q * n];
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
+    const int fl = e >> 6, al = e & 63;
+    const int a = a0 + al, f = f0 + fl;
+    if (a < v && f < v) {
+      const long long dof = f * v2 + y * (long long)v + a;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) D[dof + q * n] = t[q][al][fl];
+    }
+  }
+}
+
+hipError_t launch_vp_transposes(const unsigned short *Vp, int o, int v, unsigned short *VpT,
+                                unsigned short *VpU, hipStream_t s) {
+  const long long nt = (v + 63) / 64;
+  const long long blocks = (long long)o * v * nt * nt;
+  if (blocks <= 0) return hipSuccess;
+  if (blocks >= (1LL << 31)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_vp_transpose, dim3((unsigned)blocks), dim3(256), 0, s, Vp, o, v, 0, VpT);
+  hipLaunchKernelGGL(k_vp_transpose, dim3((unsigned)blocks), dim3(256), 0, s, Vp, o, v, 1, VpU);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Split-BF16x3 engine, K stages of 32 with planar LDS.
 //

@@ -149,6 +149,9 @@ hipError_t launch_prepare(const double *ovvv, int packed, const double *t2, cons
 hipError_t launch_split_f32x2(const double *x, size_t n, unsigned long long *out, hipStream_t s);
 hipError_t launch_gemm_f32x2(const GemmDesc &d, hipStream_t s, const GemmBatch &bt = no_batch());
 hipError_t launch_split_bf16x3(const double *x, size_t n, unsigned short *out, hipStream_t s);
+// planar split-BF16x3 VpT[x][f][p][q] and VpU[x][f][q][p] from Vp[x][p][q][f]
+hipError_t launch_vp_transposes(const unsigned short *Vp, int o, int v, unsigned short *VpT,
+                                unsigned short *VpU, hipStream_t s);
 hipError_t launch_gemm_bf16x3(const GemmDesc &d, hipStream_t s, const GemmBatch &bt = no_batch());
 // split-BF16x3 engine with direct-to-LDS operand loads (gemm_b3p.hip), on
 // descriptors whose chunk_start counts stages of 32; launch_gemm_bf16x3 picks

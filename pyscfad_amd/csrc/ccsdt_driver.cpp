@@ -143,6 +143,9 @@ struct Layout {
   size_t off_Vp, off_T2m, off_OOt, off_OVS, off_T2S, off_orb, off_W, off_tri, off_part_t, off_T2g, off_W8;
   int wbatch;  // W blocks of the energy walk's batched W build (1 with the adjoint)
   size_t off_Vb, off_Vpb, off_T2mb, off_part_mo, off_R, off_part_p, off_part_h, off_wt;
+  // transposed planar copies of Vp (split-BF16x3 W build; see vpt_fits)
+  int vpt;
+  size_t off_VpT, off_VpU;
   // pipelined adjoint walk: second copies of the per-triple buffers
   int pipe;
   size_t off_W2, off_W8b, off_R2, off_part_p2, off_part_h2;
@@ -203,6 +206,17 @@ static int energy_batch(int o, int v, int prec) {
 // Vp, T2m and OOt hold FP64 copies, or their split forms: three bf16 planes
 // (split-BF16x3, 6 of the 8 bytes per element) or packed FP32 hi/lo pairs; the
 // split precisions add split copies of t2 and of Wbar.
+// Transposed planar copies of Vp make every particle-term ovvv operand of the
+// split-BF16x3 W build row-contiguous, the faster of the direct engine's two
+// staging forms (the big operand of all three passes).  They take two o v^3
+// arrays; held when the direct engine runs the W build (nocc and nvir
+// multiples of 8) and the copies stay within 48 GB.
+static bool vpt_fits(int o, int v, int prec) {
+  const double bytes = 2.0 * 8.0 * o * (double)v * v * v;
+  return prec == PAMD_PREC_BF16X3 && o % 8 == 0 && v % 8 == 0 && bytes <= 48e9 &&
+         gemm_b3p_enabled();
+}
+
 // pipe: the adjoint walk runs two triples in flight (run_vjp_pipelined) and
 // needs second copies of W, Wbar, the V-bar scratch and the split-K partials
 static Layout make_layout(int o, int v, int vjp, int prec, bool pipe = false) {
@@ -246,6 +260,11 @@ static Layout make_layout(int o, int v, int vjp, int prec, bool pipe = false) {
     L.off_part_p = take((size_t)6 * L.S_p * v2);
     L.off_part_h = take((size_t)6 * L.S_h * v * o);
   }
+  L.vpt = vpt_fits(o, v, prec) ? 1 : 0;
+  if (L.vpt) {
+    L.off_VpT = take((size_t)o * v3);
+    L.off_VpU = take((size_t)o * v3);
+  }
   L.pipe = (vjp && pipe) ? 1 : 0;
   if (L.pipe) {
     L.off_W2 = take(v3);
@@ -265,8 +284,8 @@ static Layout make_layout(int o, int v, int vjp, int prec, bool pipe = false) {
 // e of an array becomes bf16 element e of its hi plane, with the array's
 // element count as the plane stride.
 struct PlanarMap {
-  const char *base[5];
-  long long n[5];
+  const char *base[7];
+  long long n[7];
   int count;
 };
 
@@ -286,6 +305,9 @@ struct Ctx {
   const double *wt_host;  // occupied weights m (host) of a diagonal projector, or null
   double *wt;             // device copy of m
   PlanarMap planes;       // split-BF16x3 operand arrays
+  // transposed planar copies of Vp for the W build (split-BF16x3, when the
+  // layout holds them): VpT[x][f][p][q] = VpU[x][f][q][p] = Vp[x][p][q][f]
+  double *VpT, *VpU;
   // pipelined adjoint walk (pipe = 1): the second set of per-triple buffers
   int pipe;
   double *W2, *gW2, *R2, *part_p2, *part_h2;
@@ -361,11 +383,21 @@ static int build_w(const Ctx &c, int i, int j, int k, int nb = 1) {
   auto T2m = [&](int x) { return c.T2m + x * v2 * o; };
   auto OOt = [&](int x, int y) { return c.OOt + (x * o + y) * vo; };
   const int V = c.v, O = c.o;
+  // the ovvv slab of a particle term with its row pair (q, s) read as
+  // Vp(x)[q][s][f] ("qs") or Vp(x)[s][q][f] ("sq"), K = f: from Vp itself
+  // (K-contiguous), or, when the call holds the transposed copies, from
+  // VpT(x)[f][q][s] / VpU(x)[f][q][s] (row-contiguous)
+  auto Pqs = [&](int x) {
+    return c.VpT ? opR(c.VpT + x * v3, v, 1, V, v2, 1) : opR(Vp(x), v2, v, V, 1, 0);
+  };
+  auto Psq = [&](int x) {
+    return c.VpU ? opR(c.VpU + x * v3, v, 1, V, v2, 1) : opR(Vp(x), v, v2, V, 1, 0);
+  };
   // pass C: rows (a,b), cols c
   {
     GemmDesc d = gemm(V * V, V, kStore,
-                      {seg(opR(Vp(i), v2, v, V, 1, 0), op1(T2(k, j), v, 1, 0), V, 1.0),      // P1
-                       seg(opR(Vp(j), v, v2, V, 1, 0), op1(T2(k, i), v, 1, 0), V, 1.0),      // P3
+                      {seg(Pqs(i), op1(T2(k, j), v, 1, 0), V, 1.0),      // P1
+                       seg(Psq(j), op1(T2(k, i), v, 1, 0), V, 1.0),      // P3
                        seg(opR(T2m(j), o, vo, V, 1, 0), op1(OOt(k, i), o, 1, 0), O, -1.0),   // H5
                        seg(opR(T2m(i), vo, o, V, 1, 0), op1(OOt(k, j), o, 1, 0), O, -1.0)},  // H6
                       out(c.W, v2, v, V, 0, 1, kBig));
@@ -376,8 +408,8 @@ static int build_w(const Ctx &c, int i, int j, int k, int nb = 1) {
   // pass B: rows b, cols (a,c)
   {
     GemmDesc d = gemm(V, V * V, kAccumulate,
-                      {seg(op1(T2(j, k), v, 1, 0), opR(Vp(i), v2, v, V, 1, 0), V, 1.0),      // P2
-                       seg(op1(T2(j, i), v, 1, 0), opR(Vp(k), v, v2, V, 1, 0), V, 1.0),      // P5
+                      {seg(op1(T2(j, k), v, 1, 0), Pqs(i), V, 1.0),      // P2
+                       seg(op1(T2(j, i), v, 1, 0), Psq(k), V, 1.0),      // P5
                        seg(op1(OOt(j, i), o, 1, 0), opR(T2m(k), o, vo, V, 1, 0), O, -1.0),   // H3
                        seg(op1(OOt(j, k), o, 1, 0), opR(T2m(i), vo, o, V, 1, 0), O, -1.0)},  // H4
                       out(c.W, 0, v, kBig, v2, 1, V));
@@ -388,8 +420,8 @@ static int build_w(const Ctx &c, int i, int j, int k, int nb = 1) {
   // pass A: rows a, cols (b,c)
   {
     GemmDesc d = gemm(V, V * V, kAccumulate,
-                      {seg(op1(T2(i, k), v, 1, 0), opR(Vp(j), v2, v, V, 1, 0), V, 1.0),      // P4
-                       seg(op1(T2(i, j), v, 1, 0), opR(Vp(k), v, v2, V, 1, 0), V, 1.0),      // P6
+                      {seg(op1(T2(i, k), v, 1, 0), Pqs(j), V, 1.0),      // P4
+                       seg(op1(T2(i, j), v, 1, 0), Psq(k), V, 1.0),      // P6
                        seg(op1(OOt(i, j), o, 1, 0), opR(T2m(k), o, vo, V, 1, 0), O, -1.0),   // H1
                        seg(op1(OOt(i, k), o, 1, 0), opR(T2m(j), vo, o, V, 1, 0), O, -1.0)},  // H2
                       out(c.W, 0, v2, kBig, v, 1, V));
@@ -608,6 +640,9 @@ static int setup(Ctx &c, long long &norb) {
   PAMD_CHECK(launch_prep_sym(c.t2, o, v, 1, c.T2S, c.s));
   if (fmt == kSplitBf16x3)
     PAMD_CHECK(launch_split_bf16x3(c.t2, (size_t)o * o * v * v, (unsigned short *)c.gT2, c.s));
+  if (c.VpT)
+    PAMD_CHECK(launch_vp_transposes((const unsigned short *)c.Vp, o, v, (unsigned short *)c.VpT,
+                                    (unsigned short *)c.VpU, c.s));
   if (fmt == kSplitF32x2)
     PAMD_CHECK(launch_split_f32x2(c.t2, (size_t)o * o * v * v, (unsigned long long *)c.gT2, c.s));
   PAMD_CHECK(launch_fill(c.tri, (size_t)o * o * o, 0.0, c.s));
@@ -1010,6 +1045,8 @@ static void bind_workspace(Ctx &c, const Layout &L, void *ws) {
   const bool packed_ops = c.prec == PAMD_PREC_BF16X3 || c.prec == PAMD_PREC_FP32X2;
   c.gT2 = packed_ops ? (const double *)(b + L.off_T2g) : c.t2;
   c.gW = (packed_ops && c.vjp) ? (double *)(b + L.off_W8) : c.W;
+  c.VpT = L.vpt ? (double *)(b + L.off_VpT) : nullptr;
+  c.VpU = L.vpt ? (double *)(b + L.off_VpU) : nullptr;
   {
     const long long o = c.o, v = c.v, v3 = v * v * v;
     PlanarMap &pm = c.planes;
@@ -1024,6 +1061,10 @@ static void bind_workspace(Ctx &c, const Layout &L, void *ws) {
     add(c.OOt, o * o * o * v);
     add(c.gT2, o * o * v * v);
     if (c.vjp) add(c.gW, v3);
+    if (c.VpT) {
+      add(c.VpT, o * v3);
+      add(c.VpU, o * v3);
+    }
   }
   if (c.vjp) {
     c.Vb = (double *)(b + L.off_Vb);
