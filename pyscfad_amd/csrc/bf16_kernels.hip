@@ -727,4 +727,26 @@ hipError_t launch_gemm_bf16x3(const GemmDesc &d, hipStream_t s, const GemmBatch 
   return launch_gemm_b3(d, s, bt);
 }
 
+// chunk_start of a descriptor in stages of BKC
+static GemmDesc to_stages(const GemmDesc &d0) {
+  GemmDesc d = d0;
+  d.chunk_start[0] = 0;
+  for (int q = 0; q < d.nseg; ++q) d.chunk_start[q + 1] = d.chunk_start[q] + (d.seg[q].L + BKC - 1) / BKC;
+  for (int q = d.nseg + 1; q <= kMaxSeg; ++q) d.chunk_start[q] = d.chunk_start[d.nseg];
+  return d;
+}
+
+hipError_t launch_gemm_bf16x3_group(const GemmDesc *ds, int n, hipStream_t s) {
+  if (n >= 2 && n <= kMaxGroup && gemm_b3p_enabled()) {
+    GemmDesc g[kMaxGroup];
+    for (int p = 0; p < n; ++p) g[p] = to_stages(ds[p]);
+    if (gemm_b3p_group_ok(g, n)) return launch_gemm_b3p_group(g, n, s);
+  }
+  for (int p = 0; p < n; ++p) {
+    const hipError_t e = launch_gemm_b3(ds[p], s, no_batch());
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 }  // namespace pamd

@@ -60,6 +60,23 @@ struct GemmDesc {
   double *partial;               // kPartial: partial[(split * M + m) * N + n]
 };
 
+// A descriptor of at most two segments (same fields as GemmDesc), for group
+// launches whose kernel argument holds several of them.
+struct GemmDescS {
+  int M, N, nseg, mode;
+  int chunk_start[3];
+  int splits;
+  SegDesc seg[2];
+  OutDesc C;
+  double *partial;
+};
+constexpr int kMaxGroup = 6;
+struct GemmGroup {
+  GemmDescS d[kMaxGroup];
+  int n;
+  int pad_;
+};
+
 // A batch of nb products that share one descriptor up to pointer offsets:
 // product b reads seg[s].A.p + b * a[s] and seg[s].B.p + b * bo[s] and writes
 // C.p + b * c (element offsets; not for the split-K partial mode).  One launch
@@ -160,6 +177,13 @@ bool gemm_b3p_ok(const GemmDesc &d, const GemmBatch &bt = no_batch());
 bool gemm_b3p_enabled();  // env PYSCFAD_AMD_B3_DIRECT=0 turns it off
 int gemm_b3p_tiles(int M, int N);  // workgroup tiles of one split
 hipError_t launch_gemm_b3p(const GemmDesc &d, hipStream_t s, const GemmBatch &bt = no_batch());
+// n independent products in one launch of the direct engine (chunk_start in
+// stages of 32), when gemm_b3p_group_ok (see gemm_b3p.hip)
+bool gemm_b3p_group_ok(const GemmDesc *ds, int n);
+hipError_t launch_gemm_b3p_group(const GemmDesc *ds, int n, hipStream_t s);
+// split-BF16x3 group launch (chunk_start in chunks of KC): one launch when the
+// products form a group on the direct engine, else one launch each
+hipError_t launch_gemm_bf16x3_group(const GemmDesc *ds, int n, hipStream_t s);
 hipError_t launch_gemm_f64(const GemmDesc &d, hipStream_t s, const GemmBatch &bt = no_batch());
 // FP64 / FP32 hi/lo engine with direct-to-LDS operand loads (gemm_direct.hip):
 // usable when gemm_direct_ok(d); launch_gemm_f64 / launch_gemm_f32x2 pick it then

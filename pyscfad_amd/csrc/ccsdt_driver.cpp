@@ -348,6 +348,47 @@ static hipError_t run_gemm(const Ctx &c, const GemmDesc &d0, int kind,
   return timed(s, kind, fl, 0.0, [&] { return launch_gemm_f64(d0, s, bt); });
 }
 
+// Group launches of the adjoint families: env PYSCFAD_AMD_GROUP = 0 never,
+// 1 always; unset: when the three bf16 planes of Wbar (6 v^3 bytes, read by
+// every product of a family) fit in half of the GPU's 256 MB last-level
+// cache.  Measured E+VJP: o16/v160 0.634 -> 0.602 s and o24/v240 5.61 ->
+// 5.44 s with groups, o40/v400 134.7 -> 138.6 s (the products of a group then
+// stream different ovvv slabs and Wbar orders through the cache at once).
+static int group_mode() {
+  static const int m = [] {
+    const char *e = getenv("PYSCFAD_AMD_GROUP");
+    return e ? (e[0] == '0' ? 0 : 1) : -1;
+  }();
+  return m;
+}
+constexpr double kGroupWbarBytes = 128.0 * (1 << 20);
+
+// n independent contractions of one kind: in the split-BF16x3 form one group
+// launch when their outputs are disjoint (out_disjoint), they fit the direct
+// engine's group and grouping is on for their size, else one launch each
+static hipError_t run_gemm_group(const Ctx &c, const GemmDesc *ds, int n, int kind,
+                                 bool out_disjoint) {
+  const int gm = group_mode();
+  const bool group = gm == 1 || (gm < 0 && 6.0 * c.v * (double)c.v * c.v <= kGroupWbarBytes);
+  if (g_record || c.prec != PAMD_PREC_BF16X3 || !out_disjoint || n > kMaxGroup || !group) {
+    for (int q = 0; q < n; ++q) {
+      const hipError_t e = run_gemm(c, ds[q], kind);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+  GemmDesc d[kMaxGroup];
+  double fl = 0.0;
+  for (int q = 0; q < n; ++q) {
+    d[q] = ds[q];
+    for (int p = 0; p < d[q].nseg; ++p)
+      if (!planarize(c.planes, d[q].seg[p].A) || !planarize(c.planes, d[q].seg[p].B))
+        return hipErrorInvalidValue;
+    fl += gemm_flops(d[q]);
+  }
+  return timed(c.s, kind, fl, 0.0, [&] { return launch_gemm_bf16x3_group(d, n, c.s); });
+}
+
 #define PAMD_CHECK(expr)                                                      \
   do {                                                                        \
     hipError_t e_ = (expr);                                                   \
@@ -472,27 +513,29 @@ static int adjoint_gemms(const Ctx &c, int i, int j, int k) {
   auto T2mb = [&](int x) { return c.T2mb + x * v2 * o; };
   auto OOt = [&](int x, int y) { return c.OOt + (x * o + y) * vo; };
 
+  // the three adjoints into Vpb[i], Vpb[j], Vpb[k] and into T2mb[i], T2mb[j],
+  // T2mb[k] write disjoint slabs unless two occupied indices coincide
+  const bool distinct = i != j && j != k;
   // ovvv_bar, rows = virtual pairs, cols = f
-  {  // Vpb[i][a][b|c][f]: P1 (K=c), P2 (K=b)
-    GemmDesc d = gemm(V * V, V, kAccumulate,
-                      {seg(opR(Wb, v2, v, V, 1, 0), op1(T2(k, j), 1, v, 1), V, 1.0),
-                       seg(opR(Wb, v2, 1, V, v, 1), op1(T2(j, k), 1, v, 1), V, 1.0)},
-                      out(Vpb(i), v2, v, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(c, d, kProfAdjOvvv));
-  }
-  {  // Vpb[j][b][a|c][f]: P3 (K=c), P4 (K=a)
-    GemmDesc d = gemm(V * V, V, kAccumulate,
-                      {seg(opR(Wb, v, v2, V, 1, 0), op1(T2(k, i), 1, v, 1), V, 1.0),
-                       seg(opR(Wb, v, 1, V, v2, 1), op1(T2(i, k), 1, v, 1), V, 1.0)},
-                      out(Vpb(j), v2, v, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(c, d, kProfAdjOvvv));
-  }
-  {  // Vpb[k][c][a|b][f]: rows m = y*v + x with x = c: P5 (K=b), P6 (K=a)
-    GemmDesc d = gemm(V * V, V, kAccumulate,
-                      {seg(opR(Wb, v2, 1, V, v, 1), op1(T2(j, i), 1, v, 1), V, 1.0),
-                       seg(opR(Wb, v, 1, V, v2, 1), op1(T2(i, j), 1, v, 1), V, 1.0)},
-                      out(Vpb(k), v, v2, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(c, d, kProfAdjOvvv));
+  {
+    const GemmDesc ds[3] = {
+        // Vpb[i][a][b|c][f]: P1 (K=c), P2 (K=b)
+        gemm(V * V, V, kAccumulate,
+             {seg(opR(Wb, v2, v, V, 1, 0), op1(T2(k, j), 1, v, 1), V, 1.0),
+              seg(opR(Wb, v2, 1, V, v, 1), op1(T2(j, k), 1, v, 1), V, 1.0)},
+             out(Vpb(i), v2, v, V, 0, 1, kBig)),
+        // Vpb[j][b][a|c][f]: P3 (K=c), P4 (K=a)
+        gemm(V * V, V, kAccumulate,
+             {seg(opR(Wb, v, v2, V, 1, 0), op1(T2(k, i), 1, v, 1), V, 1.0),
+              seg(opR(Wb, v, 1, V, v2, 1), op1(T2(i, k), 1, v, 1), V, 1.0)},
+             out(Vpb(j), v2, v, V, 0, 1, kBig)),
+        // Vpb[k][c][a|b][f]: rows m = y*v + x with x = c: P5 (K=b), P6 (K=a)
+        gemm(V * V, V, kAccumulate,
+             {seg(opR(Wb, v2, 1, V, v, 1), op1(T2(j, i), 1, v, 1), V, 1.0),
+              seg(opR(Wb, v, 1, V, v2, 1), op1(T2(i, j), 1, v, 1), V, 1.0)},
+             out(Vpb(k), v, v2, V, 0, 1, kBig)),
+    };
+    PAMD_CHECK(run_gemm_group(c, ds, 3, kProfAdjOvvv, distinct));
   }
 
   // particle t2_bar pieces: split-K over the virtual pair, partials per problem
@@ -518,7 +561,7 @@ static int adjoint_gemms(const Ctx &c, int i, int j, int k) {
         gemm(V, V, kPartial, {seg(opK(Wb, v2, v, 1, V, 0), opK(Vp(k), 1, v, v2, V, 1), V * V, 1.0)},
              out(nullptr, 0, 0, 1, 0, 0, 1), c.S_p, c.part_p + 5 * pp),
     };
-    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(c, ds[q], kProfAdjT2P));
+    PAMD_CHECK(run_gemm_group(c, ds, 6, kProfAdjT2P, true));
   }
   // hole ovoo_bar pieces: split-K over the virtual pair
   const size_t ph = (size_t)c.S_h * v * o;
@@ -543,29 +586,28 @@ static int adjoint_gemms(const Ctx &c, int i, int j, int k) {
         gemm(V, O, kPartial, {seg(opK(Wb, 1, v2, v, V, 1), opK(T2m(i), 1, vo, o, V, 1), V * V, -1.0)},
              out(nullptr, 0, 0, 1, 0, 0, 1), c.S_h, c.part_h + 5 * ph),
     };
-    for (int q = 0; q < 6; ++q) PAMD_CHECK(run_gemm(c, ds[q], kProfAdjOvoo));
+    PAMD_CHECK(run_gemm_group(c, ds, 6, kProfAdjOvoo, true));
   }
   // hole-term t2 reads: T2mb[x][p][q][m] (= dE/d t2[x,m,p,q])
-  {  // T2mb[i]: H4 (p=a,q=c,K=b), H6 (p=a,q=b,K=c)
-    GemmDesc d = gemm(V * V, O, kAccumulate,
-                      {seg(opR(Wb, v2, 1, V, v, 1), op1(OOt(j, k), 1, o, 1), V, -1.0),
-                       seg(opR(Wb, v2, v, V, 1, 0), op1(OOt(k, j), 1, o, 1), V, -1.0)},
-                      out(T2mb(i), vo, o, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(c, d, kProfAdjT2H));
-  }
-  {  // T2mb[j]: H2 (p=b,q=c,K=a), H5 (p=b,q=a,K=c)
-    GemmDesc d = gemm(V * V, O, kAccumulate,
-                      {seg(opR(Wb, v, 1, V, v2, 1), op1(OOt(i, k), 1, o, 1), V, -1.0),
-                       seg(opR(Wb, v, v2, V, 1, 0), op1(OOt(k, i), 1, o, 1), V, -1.0)},
-                      out(T2mb(j), vo, o, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(c, d, kProfAdjT2H));
-  }
-  {  // T2mb[k]: rows m = q*v + p with p = c: H1 (q=b,K=a), H3 (q=a,K=b)
-    GemmDesc d = gemm(V * V, O, kAccumulate,
-                      {seg(opR(Wb, v, 1, V, v2, 1), op1(OOt(i, j), 1, o, 1), V, -1.0),
-                       seg(opR(Wb, v2, 1, V, v, 1), op1(OOt(j, i), 1, o, 1), V, -1.0)},
-                      out(T2mb(k), o, vo, V, 0, 1, kBig));
-    PAMD_CHECK(run_gemm(c, d, kProfAdjT2H));
+  {
+    const GemmDesc ds[3] = {
+        // T2mb[i]: H4 (p=a,q=c,K=b), H6 (p=a,q=b,K=c)
+        gemm(V * V, O, kAccumulate,
+             {seg(opR(Wb, v2, 1, V, v, 1), op1(OOt(j, k), 1, o, 1), V, -1.0),
+              seg(opR(Wb, v2, v, V, 1, 0), op1(OOt(k, j), 1, o, 1), V, -1.0)},
+             out(T2mb(i), vo, o, V, 0, 1, kBig)),
+        // T2mb[j]: H2 (p=b,q=c,K=a), H5 (p=b,q=a,K=c)
+        gemm(V * V, O, kAccumulate,
+             {seg(opR(Wb, v, 1, V, v2, 1), op1(OOt(i, k), 1, o, 1), V, -1.0),
+              seg(opR(Wb, v, v2, V, 1, 0), op1(OOt(k, i), 1, o, 1), V, -1.0)},
+             out(T2mb(j), vo, o, V, 0, 1, kBig)),
+        // T2mb[k]: rows m = q*v + p with p = c: H1 (q=b,K=a), H3 (q=a,K=b)
+        gemm(V * V, O, kAccumulate,
+             {seg(opR(Wb, v, 1, V, v2, 1), op1(OOt(i, j), 1, o, 1), V, -1.0),
+              seg(opR(Wb, v2, 1, V, v, 1), op1(OOt(j, i), 1, o, 1), V, -1.0)},
+             out(T2mb(k), o, vo, V, 0, 1, kBig)),
+    };
+    PAMD_CHECK(run_gemm_group(c, ds, 3, kProfAdjT2H, distinct));
   }
   return PAMD_OK;
 }

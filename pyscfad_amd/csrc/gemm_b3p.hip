@@ -37,6 +37,7 @@
 // (finite inputs); k past L is zeroed in the A fragments, rows past M or N
 // are never stored.
 #include <cstdlib>
+#include <cstring>
 #include "ccsdt_common.h"
 #include "engine_staging.h"
 
@@ -207,20 +208,29 @@ __device__ __forceinline__ void frags(const bf16_t *T, int rc, int r16, int lane
 }
 
 // WM x WN waves; two workgroups per CU (80 KB of LDS each at most)
-template <int MT, int NT, int WM, int WN, int MINW>
-__global__ __launch_bounds__(64 * WM * WN, MINW) void k_gemm_b3p(GemmDesc d, GemmBatch bt, int n_fast) {
+// LDS of one workgroup (bf16 elements): two stages of the A and B tiles
+template <int MT, int NT, int WM, int WN>
+struct B3pLds {
+  static constexpr int T = 64 * WM * WN;
+  static constexpr int SA = PPlan<MT, T>::SLOTS * 8, SB = PPlan<NT, T>::SLOTS * 8, SS = SA + SB;
+  static constexpr int N = kPStages * SS;
+  static_assert(2 * N * 2 <= 160 * 1024, "LDS of two workgroups per CU");
+};
+
+// One workgroup's tile of one product: workgroup number `logical` of the
+// product's (or the batch's) grid.  D is GemmDesc or GemmDescS (same fields).
+template <int MT, int NT, int WM, int WN, class D>
+__device__ __forceinline__ void b3p_tile(const D &d, const GemmBatch &bt, int n_fast, int logical,
+                                         bf16_t *S) {
   constexpr int T = 64 * WM * WN;
   constexpr int WTM = MT / WM, WTN = NT / WN;
   constexpr int TM = WTM / 16, TN = WTN / 16;
   static_assert(TM * 16 == WTM && TN * 16 == WTN, "16x16 MFMA tiles");
-  constexpr int SA = PPlan<MT, T>::SLOTS * 8, SB = PPlan<NT, T>::SLOTS * 8, SS = SA + SB;  // bf16
-  static_assert(2 * kPStages * SS * 2 <= 160 * 1024, "LDS of two workgroups per CU");
-  __shared__ __align__(16) bf16_t S[kPStages * SS];
+  constexpr int SA = B3pLds<MT, NT, WM, WN>::SA, SS = B3pLds<MT, NT, WM, WN>::SS;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;
   const int mt = (d.M + MT - 1) / MT, nt = (d.N + NT - 1) / NT;
-  const int logical = xcd_remap(blockIdx.x, gridDim.x);
   const int per = mt * nt * d.splits;  // workgroups of one product of the batch
   const int batch = logical / per;
   const int rem = logical - batch * per;
@@ -416,6 +426,28 @@ __global__ __launch_bounds__(64 * WM * WN, MINW) void k_gemm_b3p(GemmDesc d, Gem
       }
 }
 
+template <int MT, int NT, int WM, int WN, int MINW>
+__global__ __launch_bounds__(64 * WM * WN, MINW) void k_gemm_b3p(GemmDesc d, GemmBatch bt, int n_fast) {
+  __shared__ __align__(16) bf16_t S[B3pLds<MT, NT, WM, WN>::N];
+  b3p_tile<MT, NT, WM, WN>(d, bt, n_fast, xcd_remap(blockIdx.x, gridDim.x), S);
+}
+
+// A group of up to kMaxGroup independent products with equal M, N, split-K
+// factor and mode, each of at most two segments, in one launch: the grid is
+// the products' grids one after the other.
+template <int MT, int NT, int WM, int WN, int MINW>
+__global__ __launch_bounds__(64 * WM * WN, MINW) void k_gemm_b3p_group(GemmGroup g, int n_fast) {
+  __shared__ __align__(16) bf16_t S[B3pLds<MT, NT, WM, WN>::N];
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const int per = (int)(gridDim.x / (unsigned)g.n);
+  const int p = logical / per;
+  GemmBatch bt;
+  for (int q = 0; q < kMaxSeg; ++q) bt.a[q] = bt.bo[q] = 0;
+  bt.c = 0;
+  bt.nb = 1;
+  b3p_tile<MT, NT, WM, WN>(g.d[p], bt, n_fast, logical - p * per, S);
+}
+
 // An operand can be staged with 16-byte direct loads of 8 bf16 along its fast
 // index, for every row and k the kernel may address (rows clamped to
 // [0, rows), k to [0, L)).
@@ -437,7 +469,8 @@ bool op_b3p_ok(const OpDesc &o, int rows, int L) {
 
 }  // namespace
 
-bool gemm_b3p_ok(const GemmDesc &d, const GemmBatch &bt) {
+template <class D>
+static bool b3p_ok(const D &d, const GemmBatch &bt) {
   if (d.M <= 0 || d.N <= 0 || d.nseg <= 0) return false;
   for (int q = 0; q < d.nseg; ++q) {
     const SegDesc &s = d.seg[q];
@@ -447,6 +480,8 @@ bool gemm_b3p_ok(const GemmDesc &d, const GemmBatch &bt) {
   }
   return true;
 }
+
+bool gemm_b3p_ok(const GemmDesc &d, const GemmBatch &bt) { return b3p_ok(d, bt); }
 
 bool gemm_b3p_enabled() {
   static const bool on = [] {
@@ -488,6 +523,61 @@ int gemm_b3p_pick(int M, int N) {
 int gemm_b3p_tiles(int M, int N) {
   const int c = gemm_b3p_pick(M, N);
   return ((M + kB3pShape[c][0] - 1) / kB3pShape[c][0]) * ((N + kB3pShape[c][1] - 1) / kB3pShape[c][1]);
+}
+
+// The products form a group: equal sizes, split-K factor and mode, at most two
+// segments each, at most kMaxGroup of them, each eligible for direct loads.
+bool gemm_b3p_group_ok(const GemmDesc *ds, int n) {
+  if (n < 1 || n > kMaxGroup) return false;
+  for (int p = 0; p < n; ++p) {
+    const GemmDesc &d = ds[p];
+    if (d.nseg > 2 || d.M != ds[0].M || d.N != ds[0].N || d.splits != ds[0].splits ||
+        d.mode != ds[0].mode || !gemm_b3p_ok(d))
+      return false;
+  }
+  const GemmDesc &d = ds[0];
+  const int c = gemm_b3p_pick(d.M, d.N);
+  const long long per = (long long)((d.M + kB3pShape[c][0] - 1) / kB3pShape[c][0]) *
+                        ((d.N + kB3pShape[c][1] - 1) / kB3pShape[c][1]) * d.splits;
+  return per * n < (1LL << 31);
+}
+
+// Group launch of n products (chunk_start of each in stages of 32) that
+// satisfy gemm_b3p_group_ok.
+hipError_t launch_gemm_b3p_group(const GemmDesc *ds, int n, hipStream_t s) {
+  if (!gemm_b3p_group_ok(ds, n)) return hipErrorInvalidValue;
+  GemmGroup g;
+  memset(&g, 0, sizeof(g));
+  g.n = n;
+  for (int p = 0; p < n; ++p) {
+    const GemmDesc &d = ds[p];
+    GemmDescS &e = g.d[p];
+    e.M = d.M;
+    e.N = d.N;
+    e.nseg = d.nseg;
+    e.mode = d.mode;
+    for (int q = 0; q <= d.nseg; ++q) e.chunk_start[q] = d.chunk_start[q];
+    for (int q = d.nseg + 1; q <= 2; ++q) e.chunk_start[q] = d.chunk_start[d.nseg];
+    e.splits = d.splits;
+    for (int q = 0; q < d.nseg; ++q) e.seg[q] = d.seg[q];
+    e.C = d.C;
+    e.partial = d.partial;
+  }
+  const GemmDesc &d = ds[0];
+  const int c = gemm_b3p_pick(d.M, d.N);
+  const int MT = kB3pShape[c][0], NT = kB3pShape[c][1];
+  const long long per = (long long)((d.M + MT - 1) / MT) * ((d.N + NT - 1) / NT) * d.splits;
+  const unsigned grid = (unsigned)(per * n);
+  const int n_fast = d.M >= d.N ? 1 : 0;
+  switch (c) {
+    case 0: hipLaunchKernelGGL((k_gemm_b3p_group<128, 80, 8, 1, 4>), dim3(grid), dim3(512), 0, s, g, n_fast); break;
+    case 1: hipLaunchKernelGGL((k_gemm_b3p_group<80, 128, 1, 8, 4>), dim3(grid), dim3(512), 0, s, g, n_fast); break;
+    case 2: hipLaunchKernelGGL((k_gemm_b3p_group<128, 48, 8, 1, 4>), dim3(grid), dim3(512), 0, s, g, n_fast); break;
+    case 3: hipLaunchKernelGGL((k_gemm_b3p_group<48, 128, 1, 8, 4>), dim3(grid), dim3(512), 0, s, g, n_fast); break;
+    case 4: hipLaunchKernelGGL((k_gemm_b3p_group<80, 48, 5, 1, 3>), dim3(grid), dim3(320), 0, s, g, n_fast); break;
+    default: hipLaunchKernelGGL((k_gemm_b3p_group<48, 80, 3, 1, 2>), dim3(grid), dim3(192), 0, s, g, n_fast); break;
+  }
+  return hipGetLastError();
 }
 
 // d: chunk_start in stages of 32
