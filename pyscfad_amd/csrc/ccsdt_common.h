@@ -191,7 +191,8 @@ bool gemm_direct_ok(const GemmDesc &d, const GemmBatch &bt = no_batch());
 hipError_t launch_gemm_f64_direct(const GemmDesc &d, hipStream_t s, bool fx2,
                                   const GemmBatch &bt = no_batch());
 bool gemm_direct_enabled();  // env PYSCFAD_AMD_GEMM_DIRECT=0 turns the direct engine off
-int gemm_tiles(int M, int N);     // FP64 engine: workgroup tiles of one split
+int gemm_direct_tiles(int M, int N, bool fx2);  // direct FP64 / FP32 hi/lo engine: tiles of one split
+int gemm_tiles(int M, int N, bool fx2 = false);  // FP64 / FP32 hi/lo engine: tiles of one split
 int gemm_tiles_b3(int M, int N);  // split-BF16x3 engine: workgroup tiles of one split
 int gemm_wgs_per_cu_b3(int M, int N);  // split-BF16x3 engine: resident workgroups per CU
 hipError_t launch_orbit(const OrbitArgs &a, int grid, hipStream_t s, int mode = kOrbitPlain);

@@ -180,7 +180,7 @@ static int device_cus() {
 constexpr int kB3MaxK = 8192;
 static int splits_for(int M, int N, int chunks, int prec) {
   const bool b3 = prec == PAMD_PREC_BF16X3;
-  const int tiles = std::max(1, b3 ? gemm_tiles_b3(M, N) : gemm_tiles(M, N));
+  const int tiles = std::max(1, b3 ? gemm_tiles_b3(M, N) : gemm_tiles(M, N, prec == PAMD_PREC_FP32X2));
   const int slots = (b3 ? gemm_wgs_per_cu_b3(M, N) : 2) * device_cus();
   int s = slots / tiles;
   if (b3) s = std::max(s, (int)(((long long)chunks * KC + kB3MaxK - 1) / kB3MaxK));
@@ -194,7 +194,8 @@ static int splits_for(int M, int N, int chunks, int prec) {
 // their W blocks within 2 GB.  (At nvir = 400 one triple already has ~6000
 // tiles: no batching.)
 static int energy_batch(int o, int v, int prec) {
-  const long long tiles = prec == PAMD_PREC_BF16X3 ? gemm_tiles_b3(v * v, v) : gemm_tiles(v * v, v);
+  const long long tiles = prec == PAMD_PREC_BF16X3 ? gemm_tiles_b3(v * v, v)
+                                                    : gemm_tiles(v * v, v, prec == PAMD_PREC_FP32X2);
   const long long slots = 2LL * device_cus();
   long long b = (4 * slots + tiles - 1) / std::max(1LL, tiles);
   b = std::max(1LL, std::min(8LL, b));

@@ -476,7 +476,9 @@ hipError_t launch_gemm_f32x2(const GemmDesc &d, hipStream_t s, const GemmBatch &
   return hipGetLastError();
 }
 
-int gemm_tiles(int M, int N) {
+// (the direct engine's tiles when it is on: it takes every descriptor of even sizes)
+int gemm_tiles(int M, int N, bool fx2) {
+  if (gemm_direct_enabled()) return gemm_direct_tiles(M, N, fx2);
   return M >= N ? ((M + 127) / 128) * ((N + 63) / 64) : ((M + 63) / 64) * ((N + 127) / 128);
 }
 

@@ -54,6 +54,22 @@ __device__ __forceinline__ void wait_vmcnt() {
 }
 
 __device__ __forceinline__ int kfast_swz(int r) { return (r ^ (r >> 3)) & 7; }
+// row-contiguous tile of R rows: 16-byte chunk p (rows 2p, 2p+1) of k-row k is
+// stored at p ^ rfast_swz<R>(k), so the four k of a fragment read (k = 4g + s,
+// g = 0..3) sit at different chunk offsets; the XOR stays below R / 2
+template <int R>
+__device__ __forceinline__ int rfast_swz(int k) {
+  return (R % 64 == 0) ? (((k >> 2) & 3) << 3) : (((k >> 2) & 3) << 1);
+}
+
+// 16-byte chunks of an R-row x 16-k tile, and loads per thread of 512 (the
+// last load of a thread past the tile's 8 R chunks fills padding)
+template <int R>
+struct DTile {
+  static constexpr int CH = 8 * R;
+  static constexpr int NI = (CH + 511) / 512;
+  static constexpr int SLOTS = NI * 512;
+};
 
 // per-thread load plan of one operand (R rows) for one segment
 // When no K chunk of the segment straddles a divisor boundary of the K index
@@ -62,7 +78,7 @@ __device__ __forceinline__ int kfast_swz(int r) { return (r ^ (r >> 3)) & 7; }
 // segment) plus one wave-uniform offset of the chunk start: ptr[q].
 template <int R>
 struct DPlan {
-  static constexpr int NI = R / 64;  // 16-byte loads per thread per chunk
+  static constexpr int NI = DTile<R>::NI;  // 16-byte loads per thread per chunk
   const double *ptr[NI];             // p + row offset + kl * ky (used when `fast`)
   bool fast;
 };
@@ -72,15 +88,15 @@ template <int R>
 __device__ __forceinline__ void dslot(const OpDesc &d, int Rtot, int r0, int tid, int q, int &roff,
                                       int &kl) {
   int r;
+  int id = q * 512 + tid;
+  if (id >= DTile<R>::CH) id = 0;  // padding: a valid chunk, never read
   if (d.rfast) {
-    const int id = q * 512 + tid;
     const int k = id / (R / 2), pc = id % (R / 2);
-    const int p = pc ^ (((k >> 2) & 3) << 3);
+    const int p = pc ^ rfast_swz<R>(k);
     r = r0 + 2 * p;
     if (r > Rtot - 2) r = Rtot - 2;  // Rtot is even for a row-contiguous operand
     kl = k;
   } else {
-    const int id = q * 512 + tid;
     const int rl = id >> 3, pc = id & 7;
     kl = 2 * (pc ^ kfast_swz(rl));
     r = r0 + rl;
@@ -155,8 +171,9 @@ __global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, GemmBatch bt, 
   static_assert(WM * WN == 8, "8 waves");
   constexpr int WTM = MT / WM, WTN = NT / WN;
   constexpr int TM = WTM / 16, TN = WTN / 16;
-  constexpr int NIA = MT / 64, NIB = NT / 64, NI = NIA + NIB;
-  constexpr int SA = MT * kDK, SB = NT * kDK, SS = SA + SB;  // doubles per stage
+  constexpr int NIA = DTile<MT>::NI, NIB = DTile<NT>::NI, NI = NIA + NIB;
+  constexpr int SA = DTile<MT>::SLOTS * 2, SB = DTile<NT>::SLOTS * 2, SS = SA + SB;  // doubles per stage
+  static_assert(2 * NST * SS * 8 <= 160 * 1024, "LDS of two workgroups per CU");
   constexpr int kFlush2 = 8;
   __shared__ __align__(16) double S[NST * SS];
   const int tid = threadIdx.x;
@@ -266,87 +283,84 @@ __global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, GemmBatch bt, 
     ckappa += kDK;
     const double *As = S + ((g - g0) % NST) * SS;
     const double *Bs = As + SA;
-    double fa[4][TM], fb[4][TN];
+    // the chunk's 16 k in two halves of two MFMA steps each (s = 2h, 2h + 1),
+    // which keeps half the fragments live at a time
 #pragma unroll
-    for (int t = 0; t < TM; ++t) {
-      const int r = wm * WTM + t * 16 + (lane & 15);
-      if (la) {
+    for (int h = 0; h < 2; ++h) {
+      double fa[2][TM], fb[2][TN];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const int k = g4 + s;
-          const int c = (r >> 1) ^ (((k >> 2) & 3) << 3);
-          fa[s][t] = As[k * MT + 2 * c + (r & 1)];
-        }
-      } else {
-        const int f = kfast_swz(r);
-        const d2v x0 = *(const d2v *)(As + r * kDK + 2 * ((g4 >> 1) ^ f));
-        const d2v x1 = *(const d2v *)(As + r * kDK + 2 * (((g4 >> 1) + 1) ^ f));
-        fa[0][t] = x0[0];
-        fa[1][t] = x0[1];
-        fa[2][t] = x1[0];
-        fa[3][t] = x1[1];
-      }
-    }
+      for (int t = 0; t < TM; ++t) {
+        const int r = wm * WTM + t * 16 + (lane & 15);
+        if (la) {
 #pragma unroll
-    for (int t = 0; t < TN; ++t) {
-      const int r = wn * WTN + t * 16 + (lane & 15);
-      if (lb) {
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const int k = g4 + s;
-          const int c = (r >> 1) ^ (((k >> 2) & 3) << 3);
-          fb[s][t] = Bs[k * NT + 2 * c + (r & 1)];
-        }
-      } else {
-        const int f = kfast_swz(r);
-        const d2v x0 = *(const d2v *)(Bs + r * kDK + 2 * ((g4 >> 1) ^ f));
-        const d2v x1 = *(const d2v *)(Bs + r * kDK + 2 * (((g4 >> 1) + 1) ^ f));
-        fb[0][t] = x0[0];
-        fb[1][t] = x0[1];
-        fb[2][t] = x1[0];
-        fb[3][t] = x1[1];
-      }
-    }
-    if (kv < kDK) {  // k past the segment: zero the A side (B holds clamped, finite values)
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-        if (g4 + s >= kv) {
-#pragma unroll
-          for (int t = 0; t < TM; ++t) fa[s][t] = 0.0;
-        }
-    }
-    if constexpr (FX2) {
-      if (alpha < 0.0) {  // FP32 hi/lo pairs: flip both signs
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-#pragma unroll
-          for (int t = 0; t < TM; ++t)
-            fa[s][t] = __longlong_as_double(__double_as_longlong(fa[s][t]) ^ 0x8000000080000000ll);
-      }
-    } else if (alpha != 1.0) {
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int t = 0; t < TM; ++t) fa[s][t] *= alpha;
-    }
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-      for (int x = 0; x < TM; ++x)
-#pragma unroll
-        for (int y = 0; y < TN; ++y) {
-          if constexpr (FX2) {
-            const unsigned long long ua = __double_as_longlong(fa[s][x]);
-            const unsigned long long ub = __double_as_longlong(fb[s][y]);
-            const float ah = __uint_as_float((unsigned)ua), al = __uint_as_float((unsigned)(ua >> 32));
-            const float bh = __uint_as_float((unsigned)ub), bl = __uint_as_float((unsigned)(ub >> 32));
-            facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, facc[x][y], 0, 0, 0);
-            facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, facc[x][y], 0, 0, 0);
-            facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, facc[x][y], 0, 0, 0);
-          } else {
-            acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s][x], fb[s][y], acc[x][y], 0, 0, 0);
+          for (int u = 0; u < 2; ++u) {
+            const int k = g4 + 2 * h + u;
+            const int c = (r >> 1) ^ rfast_swz<MT>(k);
+            fa[u][t] = As[k * MT + 2 * c + (r & 1)];
           }
+        } else {
+          const d2v x = *(const d2v *)(As + r * kDK + 2 * (((g4 >> 1) + h) ^ kfast_swz(r)));
+          fa[0][t] = x[0];
+          fa[1][t] = x[1];
         }
+      }
+#pragma unroll
+      for (int t = 0; t < TN; ++t) {
+        const int r = wn * WTN + t * 16 + (lane & 15);
+        if (lb) {
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int k = g4 + 2 * h + u;
+            const int c = (r >> 1) ^ rfast_swz<NT>(k);
+            fb[u][t] = Bs[k * NT + 2 * c + (r & 1)];
+          }
+        } else {
+          const d2v x = *(const d2v *)(Bs + r * kDK + 2 * (((g4 >> 1) + h) ^ kfast_swz(r)));
+          fb[0][t] = x[0];
+          fb[1][t] = x[1];
+        }
+      }
+      if (kv < kDK) {  // k past the segment: zero the A side (B holds clamped, finite values)
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+          if (g4 + 2 * h + u >= kv) {
+#pragma unroll
+            for (int t = 0; t < TM; ++t) fa[u][t] = 0.0;
+          }
+      }
+      if constexpr (FX2) {
+        if (alpha < 0.0) {  // FP32 hi/lo pairs: flip both signs
+#pragma unroll
+          for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int t = 0; t < TM; ++t)
+              fa[u][t] = __longlong_as_double(__double_as_longlong(fa[u][t]) ^ 0x8000000080000000ll);
+        }
+      } else if (alpha != 1.0) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+          for (int t = 0; t < TM; ++t) fa[u][t] *= alpha;
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+#pragma unroll
+          for (int y = 0; y < TN; ++y) {
+            if constexpr (FX2) {
+              const unsigned long long ua = __double_as_longlong(fa[u][x]);
+              const unsigned long long ub = __double_as_longlong(fb[u][y]);
+              const float ah = __uint_as_float((unsigned)ua), al = __uint_as_float((unsigned)(ua >> 32));
+              const float bh = __uint_as_float((unsigned)ub), bl = __uint_as_float((unsigned)(ub >> 32));
+              facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, facc[x][y], 0, 0, 0);
+              facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, facc[x][y], 0, 0, 0);
+              facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, facc[x][y], 0, 0, 0);
+            } else {
+              acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[u][x], fb[u][y], acc[x][y], 0, 0, 0);
+            }
+          }
+    }
     if constexpr (FX2) {
       if (++nfl == kFlush2 || g + 1 == g1) {
         nfl = 0;
@@ -423,16 +437,56 @@ bool gemm_direct_ok(const GemmDesc &d, const GemmBatch &bt) {
   return true;
 }
 
-template <bool FX2, int NST>
+// Tile shapes {MT, NT}: 128 x 64 / 64 x 128 (4 x 2 / 2 x 4 waves, three LDS
+// stages) and 128 x 80 / 80 x 128 (8 x 1 / 1 x 8 waves, two LDS stages: the
+// 80-wide tiles cover nvir = 400 or 160 without padding).  The pick maximises
+// the padded tiling's efficiency times MT NT / (MT + NT).
+static const int kF64dShape[4][2] = {{128, 64}, {64, 128}, {128, 80}, {80, 128}};
+
+static double f64d_eff(int M, int N, int MT, int NT) {
+  const double pm = (double)((M + MT - 1) / MT) * MT, pn = (double)((N + NT - 1) / NT) * NT;
+  return (double)M * N / (pm * pn);
+}
+
+// fx2: the FP32 hi/lo form keeps FP32 and FP64 accumulators, which do not fit
+// the 80-wide tiles' registers; it takes the 64-wide shapes only
+int gemm_direct_pick(int M, int N, bool fx2) {
+  int best = 0;
+  double bs = -1.0;
+  for (int c = 0; c < (fx2 ? 2 : 4); ++c) {
+    const int MT = kF64dShape[c][0], NT = kF64dShape[c][1];
+    const double sc = f64d_eff(M, N, MT, NT) * (double)MT * NT / (MT + NT);
+    if (sc > bs + 1e-9) {
+      bs = sc;
+      best = c;
+    }
+  }
+  return best;
+}
+
+int gemm_direct_tiles(int M, int N, bool fx2) {
+  const int c = gemm_direct_pick(M, N, fx2);
+  return ((M + kF64dShape[c][0] - 1) / kF64dShape[c][0]) * ((N + kF64dShape[c][1] - 1) / kF64dShape[c][1]);
+}
+
+template <bool FX2>
 static void launch_f64d(const GemmDesc &d, const GemmBatch &bt, hipStream_t s) {
-  if (d.M >= d.N) {
-    const long long tiles = (long long)((d.M + 127) / 128) * ((d.N + 63) / 64) * d.splits * bt.nb;
-    hipLaunchKernelGGL((k_gemm_f64d<128, 64, 4, 2, FX2, NST>), dim3((unsigned)tiles), dim3(512), 0,
-                       s, d, bt, 1);
-  } else {
-    const long long tiles = (long long)((d.M + 63) / 64) * ((d.N + 127) / 128) * d.splits * bt.nb;
-    hipLaunchKernelGGL((k_gemm_f64d<64, 128, 2, 4, FX2, NST>), dim3((unsigned)tiles), dim3(512), 0,
-                       s, d, bt, 0);
+  const int c = gemm_direct_pick(d.M, d.N, FX2);
+  const int MT = kF64dShape[c][0], NT = kF64dShape[c][1];
+  const long long tiles = (long long)((d.M + MT - 1) / MT) * ((d.N + NT - 1) / NT) * d.splits * bt.nb;
+  const int n_fast = d.M >= d.N ? 1 : 0;
+  const dim3 g((unsigned)tiles), b(512);
+  switch (c) {
+    case 0: hipLaunchKernelGGL((k_gemm_f64d<128, 64, 4, 2, FX2, kDStages>), g, b, 0, s, d, bt, n_fast); break;
+    case 1: hipLaunchKernelGGL((k_gemm_f64d<64, 128, 2, 4, FX2, kDStages>), g, b, 0, s, d, bt, n_fast); break;
+    default:
+      if constexpr (!FX2) {
+        if (c == 2)
+          hipLaunchKernelGGL((k_gemm_f64d<128, 80, 8, 1, false, 2>), g, b, 0, s, d, bt, n_fast);
+        else
+          hipLaunchKernelGGL((k_gemm_f64d<80, 128, 1, 8, false, 2>), g, b, 0, s, d, bt, n_fast);
+      }
+      break;
   }
 }
 
@@ -440,11 +494,10 @@ hipError_t launch_gemm_f64_direct(const GemmDesc &d, hipStream_t s, bool fx2,
                                   const GemmBatch &bt) {
   if (d.M <= 0 || d.N <= 0) return hipSuccess;
   if (bt.nb < 1 || (bt.nb > 1 && d.mode == kPartial)) return hipErrorInvalidValue;
-  // three LDS stages (two measured the same at o10/v400: 34.1 vs 33.6 TFLOP/s)
   if (fx2)
-    launch_f64d<true, kDStages>(d, bt, s);
+    launch_f64d<true>(d, bt, s);
   else
-    launch_f64d<false, kDStages>(d, bt, s);
+    launch_f64d<false>(d, bt, s);
   return hipGetLastError();
 }
 

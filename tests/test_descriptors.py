@@ -152,18 +152,28 @@ def test_bf16x3_direct_engine_eligibility():
     assert lib.pamd_debug_gemm_b3p_ok(ctypes.byref(d), 40 * 400 ** 3 + 4) == 0
 
 
+def _rfast_swz(R, k):
+    """Chunk XOR of k-row k in a row-contiguous tile of R rows (gemm_direct.hip)."""
+    return ((k >> 2) & 3) << 3 if R % 64 == 0 else ((k >> 2) & 3) << 1
+
+
 def _direct_stage_map(R, rfast):
     """LDS slot (in doubles) of each tile element (r, k), r < R, k < 16, as the
     direct-load plan of gemm_direct.hip places it: thread t, load q writes 16
     bytes at byte 16 (512 q + t); kfast slot = 16 r + 2 (c ^ f(r)) + k % 2,
-    rfast slot = R k + 2 (p ^ sw(k)) + r % 2."""
+    rfast slot = R k + 2 (p ^ sw(k)) + r % 2.  Loads past the tile's 8 R chunks
+    (R = 80: 1024 slots for 640 chunks) fill padding and hold no element."""
     slot = {}
-    for q in range(R // 64):
+    nch = 8 * R
+    for q in range(-(-nch // 512)):
         for t in range(512):
             idx = q * 512 + t
+            if idx >= nch:
+                continue
             if rfast:
                 k, pc = divmod(idx, R // 2)
-                p = pc ^ (((k >> 2) & 3) << 3)
+                p = pc ^ _rfast_swz(R, k)
+                assert p < R // 2
                 for e in range(2):
                     slot[(2 * p + e, k)] = 2 * idx + e
             else:
@@ -174,7 +184,7 @@ def _direct_stage_map(R, rfast):
     return slot
 
 
-@pytest.mark.parametrize('R', [64, 128])
+@pytest.mark.parametrize('R', [64, 80, 128])
 @pytest.mark.parametrize('rfast', [0, 1])
 def test_direct_engine_lds_layout_round_trip(R, rfast):
     """The direct-load plan stores every (row, k) of a chunk exactly once, and
@@ -186,7 +196,7 @@ def test_direct_engine_lds_layout_round_trip(R, rfast):
         for k in range(16):
             g4, s = (k // 4) * 4, k % 4
             if rfast:
-                c = (r >> 1) ^ (((k >> 2) & 3) << 3)
+                c = (r >> 1) ^ _rfast_swz(R, k)
                 read = k * R + 2 * c + (r & 1)
             else:
                 f = (r ^ (r >> 3)) & 7
