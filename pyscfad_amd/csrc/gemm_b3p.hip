@@ -299,6 +299,11 @@ __device__ __forceinline__ void b3p_tile(const D &d, const GemmBatch &bt, int n_
 
   if (g0 < g1) issue(g0);
   const int G = lane >> 4;
+  // a wave whose rows or columns of the tile lie wholly past M or N (the last
+  // tile of a side that is not a multiple of the tile, e.g. 400 = 3 x 128 + 16)
+  // only loads and synchronises: its MFMA issue slots go to the other waves of
+  // its SIMD
+  const bool live = m0 + wm * WTM < d.M && n0 + wn * WTN < d.N;
   for (int g = g0; g < g1; ++g) {
     wait_vmcnt<0>();
     __builtin_amdgcn_s_barrier();
@@ -315,6 +320,7 @@ __device__ __forceinline__ void b3p_tile(const D &d, const GemmBatch &bt, int n_
     }
     const int kv = cL - ckappa;  // valid k in this stage (>= 32: all)
     ckappa += kPK;
+    if (!live) continue;
     const bf16_t *As = S + ((g - g0) % kPStages) * SS;
     const bf16_t *Bs = As + SA;
     // k past the segment is zeroed on the A side (B holds clamped, finite

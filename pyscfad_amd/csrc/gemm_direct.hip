@@ -261,6 +261,9 @@ __global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, GemmBatch bt, 
   if (NST == 3 && g0 + 1 < g1) issue(g0 + 1);
 
   const int g4 = (lane >> 4) * 4;
+  // a wave whose rows or columns of the tile lie wholly past M or N only loads
+  // and synchronises (its MFMA issue slots go to the other waves of its SIMD)
+  const bool live = m0 + wm * WTM < d.M && n0 + wn * WTN < d.N;
   for (int g = g0; g < g1; ++g) {
     if (NST == 3 && g + 1 < g1)
       wait_vmcnt<NI>();
@@ -281,6 +284,7 @@ __global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, GemmBatch bt, 
     }
     const int kv = cL - ckappa;  // valid k in this chunk (>= 16: all)
     ckappa += kDK;
+    if (!live) continue;
     const double *As = S + ((g - g0) % NST) * SS;
     const double *Bs = As + SA;
     // the chunk's 16 k in two halves of two MFMA steps each (s = 2h, 2h + 1),
