@@ -154,6 +154,11 @@ struct CombineArgs {
   int S_p, S_h;
   int o, v, i, j, k, has_fvo;
   double *t2_bar, *ovov_bar, *ovoo_bar, *t1_bar, *fvo_bar;
+  // when part_t is set: *tri_dst = sum of the orbit epilogue's G per-workgroup
+  // energy sums (fixed order), in the same launch
+  const double *part_t;
+  int G;
+  double *tri_dst;
 };
 
 // host-side launchers (ccsdt_kernels.hip)

@@ -613,7 +613,10 @@ static int adjoint_gemms(const Ctx &c, int i, int j, int k) {
   return PAMD_OK;
 }
 
-static int adjoint_combine(const Ctx &c, int i, int j, int k, hipStream_t st) {
+// with_energy: the same launch also sums the orbit epilogue's per-workgroup
+// energies of this triple into tri (the walk then skips launch_triple_energy)
+static int adjoint_combine(const Ctx &c, int i, int j, int k, hipStream_t st,
+                           bool with_energy = false) {
   const double v = c.v, o = c.o, v2 = v * v;
   CombineArgs ca;
   ca.R = c.R;
@@ -632,6 +635,9 @@ static int adjoint_combine(const Ctx &c, int i, int j, int k, hipStream_t st) {
   ca.ovoo_bar = c.ovoo_bar;
   ca.t1_bar = c.t1_bar;
   ca.fvo_bar = c.fvo_bar;
+  ca.part_t = with_energy ? c.part_t : nullptr;
+  ca.G = c.G;
+  ca.tri_dst = with_energy ? c.tri + ((size_t)i * c.o + j) * c.o + k : nullptr;
   if (!g_record) {
     const double cb = 8.0 * (6.0 * c.S_p * v2 + 6.0 * c.S_h * v * o + 12.0 * v2);
     PAMD_CHECK(timed(st, kProfCombine, 0.0, cb, [&] { return launch_combine(ca, st); }));
@@ -639,10 +645,10 @@ static int adjoint_combine(const Ctx &c, int i, int j, int k, hipStream_t st) {
   return PAMD_OK;
 }
 
-static int adjoint(const Ctx &c, int i, int j, int k) {
+static int adjoint(const Ctx &c, int i, int j, int k, bool with_energy = false) {
   int st = adjoint_vr(c, i, j, k, c.s);
   if (st == PAMD_OK) st = adjoint_gemms(c, i, j, k);
-  if (st == PAMD_OK) st = adjoint_combine(c, i, j, k, c.s);
+  if (st == PAMD_OK) st = adjoint_combine(c, i, j, k, c.s, with_energy);
   return st;
 }
 
@@ -906,7 +912,6 @@ static int run_vjp_pipelined(Ctx &c, long long norb, PipeRes &pr) {
         const double ob = 8.0 * (double)c.v * c.v * c.v * 3.0;  // W read, Wbar and Vbar written
         PAMD_CHECK(timed(E, kProfOrbit, 0.0, ob, [&] { return launch_orbit(oa, C.G, E); }));
       }
-      PAMD_CHECK(launch_triple_energy(C.part_t, C.G, C.tri + ((size_t)i * o + j) * o + k, E));
       st = adjoint_vr(C, i, j, k, E);
       if (st != PAMD_OK) return st;
       PAMD_CHECK(hipEventRecord(vr[p], E));
@@ -926,7 +931,7 @@ static int run_vjp_pipelined(Ctx &c, long long norb, PipeRes &pr) {
       PAMD_CHECK(hipEventRecord(adj[p], G));
       // E: gather of t
       PAMD_CHECK(hipStreamWaitEvent(E, adj[p], 0));
-      st = adjoint_combine(C, i, j, k, E);
+      st = adjoint_combine(C, i, j, k, E, true);  // with the triple's energy
       if (st != PAMD_OK) return st;
       PAMD_CHECK(hipEventRecord(cmb[p], E));
       if (g_progress_every > 0 && (t + 1) % g_progress_every == 0) {
@@ -1037,9 +1042,10 @@ static int run(Ctx &c) {
           const double ob = 8.0 * v3 * (c.vjp ? 3.0 : 1.0);  // W read (+ Wbar, Vbar written)
           PAMD_CHECK(timed(c.s, kProfOrbit, 0.0, ob, [&] { return launch_orbit(oa, c.G, c.s); }));
         }
-        PAMD_CHECK(launch_triple_energy(c.part_t, c.G, c.tri + ((size_t)i * o + j) * o + k, c.s));
+        if (!c.vjp)
+          PAMD_CHECK(launch_triple_energy(c.part_t, c.G, c.tri + ((size_t)i * o + j) * o + k, c.s));
         if (c.vjp) {
-          st = adjoint(c, i, j, k);
+          st = adjoint(c, i, j, k, true);  // the gather also sums the triple's energy
           if (st != PAMD_OK) return st;
         }
         if (g_progress_every > 0 && tcount % g_progress_every == 0) {

@@ -15,6 +15,7 @@
 // * prepare/fold -- layout changes at entry/exit.
 #include <type_traits>
 #include <cstdlib>
+#include <algorithm>
 #include "ccsdt_common.h"
 #include "engine_staging.h"
 #include "bf16_split.h"
@@ -1002,12 +1003,18 @@ __device__ __forceinline__ double vr_csum(const double *R, int v, int q, size_t 
   return x;
 }
 
+// LDS of the V-bar reductions (one launch runs both kinds of workgroup)
+struct VRLds {
+  double red[kVRW * kVRC];
+  double cst[4][kVRC];  // per-c constants of the b-sums: t1[k,c], fvo[c,k], S2[a,c], T2[a,c]
+  double sred[kVRW];
+};
+
 template <bool FVO>
-__global__ __launch_bounds__(64 * kVRW) void k_vr_a(VRArgs A) {
-  __shared__ double red[kVRW * kVRC];
-  __shared__ double cst[4][kVRC];  // per-c constants: t1[k,c], fvo[c,k], S2[a,c], T2[a,c]
-  __shared__ double sred[kVRW];
-  const int a = blockIdx.x, part = blockIdx.y, cb = blockIdx.z;
+__device__ __forceinline__ void vr_a_block(const VRArgs &A, VRLds &L, int a, int part, int cb) {
+  double *red = L.red;
+  double (*cst)[kVRC] = L.cst;
+  double *sred = L.sred;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int o = A.o, v = A.v, i = A.i, j = A.j, k = A.k;
   const size_t vv = (size_t)v * v;
@@ -1079,9 +1086,8 @@ __global__ __launch_bounds__(64 * kVRW) void k_vr_a(VRArgs A) {
 }
 
 template <bool FVO>
-__global__ __launch_bounds__(64 * kVRW) void k_vr_b(VRArgs A) {
-  __shared__ double red[kVRW * kVRC];
-  const int b = blockIdx.x, part = blockIdx.y, cb = blockIdx.z;
+__device__ __forceinline__ void vr_b_block(const VRArgs &A, VRLds &L, int b, int part, int cb) {
+  double *red = L.red;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int o = A.o, v = A.v, i = A.i;
   const size_t vv = (size_t)v * v;
@@ -1108,15 +1114,24 @@ __global__ __launch_bounds__(64 * kVRW) void k_vr_b(VRArgs A) {
 }
 #undef PAMD_WAVE_COMBINE
 
+// both V-bar reductions in one launch: blockIdx.y < kVRS runs part y of the
+// b-sums, the rest part y - kVRS of the a-sums
+template <bool FVO>
+__global__ __launch_bounds__(64 * kVRW) void k_vr(VRArgs A) {
+  __shared__ VRLds L;
+  const int y = blockIdx.y;
+  if (y < kVRS)
+    vr_a_block<FVO>(A, L, blockIdx.x, y, blockIdx.z);
+  else
+    vr_b_block<FVO>(A, L, blockIdx.x, y - kVRS, blockIdx.z);
+}
+
 hipError_t launch_vr(const VRArgs &a, hipStream_t s) {
-  const dim3 grid(a.v, kVRS, vr_col_blocks(a.v)), block(64 * kVRW);
-  if (a.has_fvo) {
-    hipLaunchKernelGGL(k_vr_a<true>, grid, block, 0, s, a);
-    hipLaunchKernelGGL(k_vr_b<true>, grid, block, 0, s, a);
-  } else {
-    hipLaunchKernelGGL(k_vr_a<false>, grid, block, 0, s, a);
-    hipLaunchKernelGGL(k_vr_b<false>, grid, block, 0, s, a);
-  }
+  const dim3 grid(a.v, 2 * kVRS, vr_col_blocks(a.v)), block(64 * kVRW);
+  if (a.has_fvo)
+    hipLaunchKernelGGL(k_vr<true>, grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL(k_vr<false>, grid, block, 0, s, a);
   return hipGetLastError();
 }
 
@@ -1125,10 +1140,34 @@ hipError_t launch_vr(const VRArgs &a, hipStream_t s) {
 // of every slab it touches, so coinciding occupied indices (i == j etc.) are
 // summed sequentially in the same thread: no atomics, deterministic.
 // ---------------------------------------------------------------------------
-__global__ void k_combine(CombineArgs A) {
+// Split-K partial sums of the six particle and six hole problems of a triple,
+// in place: part[pr][0][t] = sum_s part[pr][s][t], s = 0..S-1, in a fixed
+// order (four interleaved partial sums); blockIdx.y < 6 picks a particle
+// problem, the rest a hole problem.
+__global__ void k_splitk_reduce(double *part_p, int S_p, size_t T_p, double *part_h, int S_h,
+                                size_t T_h) {
+  const int y = blockIdx.y;
+  const bool hole = y >= 6;
+  const size_t T = hole ? T_h : T_p;
+  const int S = hole ? S_h : S_p;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  double *p = (hole ? part_h + (size_t)(y - 6) * S * T : part_p + (size_t)y * S * T) + t;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int s = 0;
+  for (; s + 4 <= S; s += 4) {
+    a0 += p[(size_t)s * T];
+    a1 += p[(size_t)(s + 1) * T];
+    a2 += p[(size_t)(s + 2) * T];
+    a3 += p[(size_t)(s + 3) * T];
+  }
+  for (; s < S; ++s) a0 += p[(size_t)s * T];
+  p[0] = (a0 + a1) + (a2 + a3);
+}
+
+__device__ __forceinline__ void combine_block(const CombineArgs &A, size_t t) {
   const int o = A.o, v = A.v, i = A.i, j = A.j, k = A.k;
   const size_t vv = (size_t)v * v;
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const double *R = A.R;
   if (t < vv) {
     const int p = (int)(t / v), q = (int)(t % v);
@@ -1179,12 +1218,10 @@ __global__ void k_combine(CombineArgs A) {
 
 // t1_bar / fvo_bar pieces of one triple: column sums over a of four v x v
 // slices of R, one block per 64 columns (the 4 waves split the rows).
-__global__ __launch_bounds__(256) void k_vr_colsums(CombineArgs A) {
-  __shared__ double red[4][4][64];
+__device__ __forceinline__ void colsums_block(const CombineArgs &A, int blk, double (*red)[4][64]) {
   const int o = A.o, v = A.v, i = A.i, j = A.j, k = A.k;
-  const size_t vv = (size_t)v * v;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int c = blockIdx.x * 64 + lane;
+  const int c = blk * 64 + lane;
   const double *R = A.R;
   double sk = 0.0, sj = 0.0, fk = 0.0, fj = 0.0;
   if (c < v) {
@@ -1214,36 +1251,42 @@ __global__ __launch_bounds__(256) void k_vr_colsums(CombineArgs A) {
   }
 }
 
-// Split-K partial sums of six problems, in place: part[pr][0][t] = sum_s
-// part[pr][s][t], s = 0..S-1, in a fixed order (four interleaved partial sums).
-__global__ void k_splitk_reduce(double *part, int S, size_t T) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= T) return;
-  double *p = part + (size_t)blockIdx.y * S * T + t;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  int s = 0;
-  for (; s + 4 <= S; s += 4) {
-    a0 += p[(size_t)s * T];
-    a1 += p[(size_t)(s + 1) * T];
-    a2 += p[(size_t)(s + 2) * T];
-    a3 += p[(size_t)(s + 3) * T];
+// The gather of one triple in one launch (after k_splitk_reduce): blocks
+// [0, nc) gather the split-K sums and the V terms into t2_bar / ovov_bar /
+// ovoo_bar (one thread per
+// position), the next ceil(v / 64) the t1_bar / fvo_bar column sums, and,
+// when A.part_t is set, one more the triple's energy from the orbit
+// epilogue's per-workgroup sums.
+__global__ __launch_bounds__(256) void k_gather(CombineArgs A, int nc) {
+  __shared__ double red[4][4][64];
+  const int blk = blockIdx.x;
+  if (blk < nc) {
+    combine_block(A, (size_t)blk * 256 + threadIdx.x);
+    return;
   }
-  for (; s < S; ++s) a0 += p[(size_t)s * T];
-  p[0] = (a0 + a1) + (a2 + a3);
+  const int ncs = (A.v + 63) / 64;
+  if (blk < nc + ncs) {
+    colsums_block(A, blk - nc, red);
+    return;
+  }
+  double s = 0.0;
+  for (int r = threadIdx.x; r < A.G; r += 256) s += A.part_t[r];
+  s = block_sum256(s, &red[0][0][0]);
+  if (threadIdx.x == 0) *A.tri_dst = s;
 }
 
 hipError_t launch_combine(const CombineArgs &a, hipStream_t s) {
   {
     const size_t Tp = (size_t)a.v * a.v, Th = (size_t)a.v * a.o;
-    hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)((Tp + 255) / 256), 6), dim3(256), 0, s,
-                       const_cast<double *>(a.part_p), a.S_p, Tp);
-    hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)((Th + 255) / 256), 6), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)((std::max(Tp, Th) + 255) / 256), 12),
+                       dim3(256), 0, s, const_cast<double *>(a.part_p), a.S_p, Tp,
                        const_cast<double *>(a.part_h), a.S_h, Th);
   }
   size_t n = (size_t)a.v * a.v;
   if ((size_t)a.v * a.o > n) n = (size_t)a.v * a.o;
-  hipLaunchKernelGGL(k_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(k_vr_colsums, dim3((unsigned)((a.v + 63) / 64)), dim3(256), 0, s, a);
+  const int nc = (int)((n + 255) / 256), ncs = (a.v + 63) / 64;
+  const int blocks = nc + ncs + (a.part_t ? 1 : 0);
+  hipLaunchKernelGGL(k_gather, dim3((unsigned)blocks), dim3(256), 0, s, a, nc);
   return hipGetLastError();
 }
 
