@@ -614,6 +614,24 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
   // W) go to workgroups on the same XCD, so the halves of a line that two
   // workgroups need meet in one L2
   const int first = xcd_remap(blockIdx.x, gridDim.x);
+  // W of the orbit's (up to) six tiles at this thread's position: slots past
+  // the orbit's distinct tiles repeat tile 0, indices past v are clamped (their
+  // values are zeroed where used).  The loads of the next orbit are issued
+  // during the current orbit's phase 2, which issues no loads of its own.
+  auto load_w = [&](long long qq, double (&w)[6]) {
+    const unsigned long long e = ((const OrbitEntryPtr)A.tab)[qq];
+    const int vq[3] = {(int)(e & 0xffff), (int)((e >> 16) & 0xffff), (int)((e >> 32) & 0xffff)};
+    const OrbitPattern &pq = kOrbitPat[(int)(e >> 48)];
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+      const int ps = p < pq.np ? p : 0;
+      const int ar = min(vq[pq.sel[ps][0]] * OT + al, v - 1), br = min(vq[pq.sel[ps][1]] * OT + be, v - 1),
+                cr = min(vq[pq.sel[ps][2]] * OT + ga, v - 1);
+      w[p] = A.W[((long long)ar * v + br) * v + cr];
+    }
+  };
+  double wnext[6];
+  if (first < A.norb) load_w(first, wnext);
   for (long long q = first; q < A.norb; q += gridDim.x) {
     // The entry is read through the constant address space (the table is not
     // written while the kernel runs), so it and the pattern it selects are
@@ -642,11 +660,7 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
       return a < v && b < v && c < v;
     };
 #pragma unroll
-    for (int p = 0; p < 6; ++p) {
-      int ar, br, cr;
-      coords(p, ar, br, cr);
-      wreg[p] = A.W[((long long)ar * v + br) * v + cr];
-    }
+    for (int p = 0; p < 6; ++p) wreg[p] = wnext[p];
     auto v_terms = [&](auto with_fvo) {
 #pragma unroll
       for (int p = 0; p < 6; ++p) {
@@ -683,6 +697,7 @@ __global__ __launch_bounds__(512) void k_orbit(OrbitArgs A) {
       }
     }
     __syncthreads();
+    if (q + gridDim.x < A.norb) load_w(q + gridDim.x, wnext);  // next orbit, in flight during phase 2
     // phase 2
     // dE/de_virtual of this orbit, per lane and per tile range r of the orbit
     // (val[r]): lane l < 8 holds the c-axis sum for offset l, 8 <= l < 16 the
