@@ -1,0 +1,35 @@
+"""The adjoint walk with two triples in flight (pamd_ccsdt_energy_vjp) against
+the walk with one triple in flight (pamd_ccsdt_energy_vjp_noalloc, which never
+pipelines): the energies and every cotangent block are bitwise equal, since
+every accumulator shared by the triples is written by one stream in walk order."""
+import ctypes
+
+import pytest
+import torch
+
+from pyscfad_amd import _lib
+from pyscfad_amd.cc import _ops
+from pyscfad_amd.gen import make_problem
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'bf16x3'])
+@pytest.mark.parametrize('o,v,seed', [(8, 40, 5), (5, 24, 6)])
+def test_pipelined_walk_is_bitwise_the_one_stream_walk(o, v, seed, precision, cuda):
+    print(f'pipeline vs one stream: {precision} o{o} v{v}', flush=True)
+    p = make_problem(o, v, 4 * v, seed=seed, fvo_scale=1e-3)
+    inp = _ops.Inputs(p.mo_energy, p.t1, p.t2, p.ovvv_packed(), p.ovoo, p.ovov, p.fvo,
+                      precision=precision, device=cuda)
+    e_ref, g_ref = _ops.energy_vjp(inp, et_bar=0.7)
+    lib = _lib.load()
+    out = _ops._cotangent_outputs(inp)
+    e = torch.zeros((), dtype=torch.float64, device=inp.device)
+    st = lib.pamd_ccsdt_energy_vjp_noalloc(ctypes.byref(inp.cfg), *inp.args(),
+                                           ctypes.c_double(0.7), _ops._ptr(e),
+                                           *_ops._cotangent_ptrs(out), _ops._stream(inp.device))
+    torch.cuda.synchronize()
+    assert st == _lib.PAMD_OK
+    assert torch.equal(e, e_ref.reshape(e.shape))
+    for k, ref in g_ref.items():
+        assert torch.equal(out[k], ref), k
