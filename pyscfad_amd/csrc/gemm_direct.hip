@@ -22,6 +22,11 @@
 // makes every wave's loads visible and retires the slot read one iteration
 // earlier, which the next loads then overwrite.
 //
+// The kernel is compiled per launch for the operand layouts its segments use
+// (each operand all K-contiguous, all row-contiguous, or mixed with a runtime
+// choice), so the fragment reads and MFMAs of a chunk form one straight-line
+// block: on W-build-shaped descriptors 42-43 -> 46-47 TFLOP/s.
+//
 // A pair of elements must be contiguous and 16-byte aligned, so an operand is
 // eligible when its fast stride is 1, every other stride and index divisor is
 // even, its base is 16-byte aligned, the rows of a row-contiguous operand are
@@ -164,13 +169,113 @@ __device__ __forceinline__ void kadvance(KPos &kp, int kdiv) {
   }
 }
 
+// One K chunk (16 k) of a wave's TM x TN MFMA tiles, in KH parts of SPH = 4 / KH
+// MFMA steps (s = SPH h + u; two parts keep half the fragments live at a
+// time).  LAM / LBM: the A / B stages of every segment of the launch are
+// K-contiguous (0) or row-contiguous (1), or differ between segments (2: the
+// runtime flags la / lb decide).  With the layouts fixed the reads and MFMAs
+// of a part form one block, so the reads overlap the MFMAs.  k past the
+// segment (kv < 16) is zeroed and alpha applied on the A side.
+template <int MT, int NT, int WTM, int WTN, bool FX2, int KH, int LAM, int LBM, class AccD,
+          class AccF>
+__device__ __forceinline__ void f64d_chunk(const double *As, const double *Bs, int wm, int wn,
+                                           int lane, int la, int lb, int kv, double alpha,
+                                           AccD &acc, AccF &facc) {
+  constexpr int TM = WTM / 16, TN = WTN / 16, SPH = 4 / KH;
+  const bool ra = LAM == 2 ? la != 0 : LAM == 1;
+  const bool rb = LBM == 2 ? lb != 0 : LBM == 1;
+  const int g4 = (lane >> 4) * 4;
+#pragma unroll
+  for (int h = 0; h < KH; ++h) {
+    double fa[SPH][TM], fb[SPH][TN];
+#pragma unroll
+    for (int t = 0; t < TM; ++t) {
+      const int r = wm * WTM + t * 16 + (lane & 15);
+      if (ra) {
+#pragma unroll
+        for (int u = 0; u < SPH; ++u) {
+          const int k = g4 + SPH * h + u;
+          fa[u][t] = As[k * MT + 2 * ((r >> 1) ^ rfast_swz<MT>(k)) + (r & 1)];
+        }
+      } else {
+#pragma unroll
+        for (int w = 0; w < SPH / 2; ++w) {
+          const d2v x = *(const d2v *)(As + r * kDK + 2 * (((g4 >> 1) + SPH / 2 * h + w) ^ kfast_swz(r)));
+          fa[2 * w][t] = x[0];
+          fa[2 * w + 1][t] = x[1];
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < TN; ++t) {
+      const int r = wn * WTN + t * 16 + (lane & 15);
+      if (rb) {
+#pragma unroll
+        for (int u = 0; u < SPH; ++u) {
+          const int k = g4 + SPH * h + u;
+          fb[u][t] = Bs[k * NT + 2 * ((r >> 1) ^ rfast_swz<NT>(k)) + (r & 1)];
+        }
+      } else {
+#pragma unroll
+        for (int w = 0; w < SPH / 2; ++w) {
+          const d2v x = *(const d2v *)(Bs + r * kDK + 2 * (((g4 >> 1) + SPH / 2 * h + w) ^ kfast_swz(r)));
+          fb[2 * w][t] = x[0];
+          fb[2 * w + 1][t] = x[1];
+        }
+      }
+    }
+    if (kv < kDK) {  // k past the segment: zero the A side (B holds clamped, finite values)
+#pragma unroll
+      for (int u = 0; u < SPH; ++u)
+        if (g4 + SPH * h + u >= kv) {
+#pragma unroll
+          for (int t = 0; t < TM; ++t) fa[u][t] = 0.0;
+        }
+    }
+    if constexpr (FX2) {
+      if (alpha < 0.0) {  // FP32 hi/lo pairs: flip both signs
+#pragma unroll
+        for (int u = 0; u < SPH; ++u)
+#pragma unroll
+          for (int t = 0; t < TM; ++t)
+            fa[u][t] = __longlong_as_double(__double_as_longlong(fa[u][t]) ^ 0x8000000080000000ll);
+      }
+    } else if (alpha != 1.0) {
+#pragma unroll
+      for (int u = 0; u < SPH; ++u)
+#pragma unroll
+        for (int t = 0; t < TM; ++t) fa[u][t] *= alpha;
+    }
+#pragma unroll
+    for (int u = 0; u < SPH; ++u)
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TN; ++y) {
+          if constexpr (FX2) {
+            const unsigned long long ua = __double_as_longlong(fa[u][x]);
+            const unsigned long long ub = __double_as_longlong(fb[u][y]);
+            const float ah = __uint_as_float((unsigned)ua), al = __uint_as_float((unsigned)(ua >> 32));
+            const float bh = __uint_as_float((unsigned)ub), bl = __uint_as_float((unsigned)(ub >> 32));
+            facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, facc[x][y], 0, 0, 0);
+            facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, facc[x][y], 0, 0, 0);
+            facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, facc[x][y], 0, 0, 0);
+          } else {
+            acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[u][x], fb[u][y], acc[x][y], 0, 0, 0);
+          }
+        }
+  }
+}
+
 }  // namespace
 
-template <int MT, int NT, int WM, int WN, bool FX2, int NST>
+template <int MT, int NT, int WM, int WN, bool FX2, int NST, int LAM, int LBM, int KH = 2>
 __global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, GemmBatch bt, int n_fast) {
   static_assert(WM * WN == 8, "8 waves");
   constexpr int WTM = MT / WM, WTN = NT / WN;
   constexpr int TM = WTM / 16, TN = WTN / 16;
+  static_assert(KH == 1 || KH == 2, "one or two parts per chunk");
+  constexpr int SPH = 4 / KH;  // MFMA k-steps per part
   constexpr int NIA = DTile<MT>::NI, NIB = DTile<NT>::NI, NI = NIA + NIB;
   constexpr int SA = DTile<MT>::SLOTS * 2, SB = DTile<NT>::SLOTS * 2, SS = SA + SB;  // doubles per stage
   static_assert(2 * NST * SS * 8 <= 160 * 1024, "LDS of two workgroups per CU");
@@ -260,7 +365,6 @@ __global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, GemmBatch bt, 
   if (g0 < g1) issue(g0);
   if (NST == 3 && g0 + 1 < g1) issue(g0 + 1);
 
-  const int g4 = (lane >> 4) * 4;
   // a wave whose rows or columns of the tile lie wholly past M or N only loads
   // and synchronises (its MFMA issue slots go to the other waves of its SIMD)
   const bool live = m0 + wm * WTM < d.M && n0 + wn * WTN < d.N;
@@ -287,84 +391,7 @@ __global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, GemmBatch bt, 
     if (!live) continue;
     const double *As = S + ((g - g0) % NST) * SS;
     const double *Bs = As + SA;
-    // the chunk's 16 k in two halves of two MFMA steps each (s = 2h, 2h + 1),
-    // which keeps half the fragments live at a time
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      double fa[2][TM], fb[2][TN];
-#pragma unroll
-      for (int t = 0; t < TM; ++t) {
-        const int r = wm * WTM + t * 16 + (lane & 15);
-        if (la) {
-#pragma unroll
-          for (int u = 0; u < 2; ++u) {
-            const int k = g4 + 2 * h + u;
-            const int c = (r >> 1) ^ rfast_swz<MT>(k);
-            fa[u][t] = As[k * MT + 2 * c + (r & 1)];
-          }
-        } else {
-          const d2v x = *(const d2v *)(As + r * kDK + 2 * (((g4 >> 1) + h) ^ kfast_swz(r)));
-          fa[0][t] = x[0];
-          fa[1][t] = x[1];
-        }
-      }
-#pragma unroll
-      for (int t = 0; t < TN; ++t) {
-        const int r = wn * WTN + t * 16 + (lane & 15);
-        if (lb) {
-#pragma unroll
-          for (int u = 0; u < 2; ++u) {
-            const int k = g4 + 2 * h + u;
-            const int c = (r >> 1) ^ rfast_swz<NT>(k);
-            fb[u][t] = Bs[k * NT + 2 * c + (r & 1)];
-          }
-        } else {
-          const d2v x = *(const d2v *)(Bs + r * kDK + 2 * (((g4 >> 1) + h) ^ kfast_swz(r)));
-          fb[0][t] = x[0];
-          fb[1][t] = x[1];
-        }
-      }
-      if (kv < kDK) {  // k past the segment: zero the A side (B holds clamped, finite values)
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-          if (g4 + 2 * h + u >= kv) {
-#pragma unroll
-            for (int t = 0; t < TM; ++t) fa[u][t] = 0.0;
-          }
-      }
-      if constexpr (FX2) {
-        if (alpha < 0.0) {  // FP32 hi/lo pairs: flip both signs
-#pragma unroll
-          for (int u = 0; u < 2; ++u)
-#pragma unroll
-            for (int t = 0; t < TM; ++t)
-              fa[u][t] = __longlong_as_double(__double_as_longlong(fa[u][t]) ^ 0x8000000080000000ll);
-        }
-      } else if (alpha != 1.0) {
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-#pragma unroll
-          for (int t = 0; t < TM; ++t) fa[u][t] *= alpha;
-      }
-#pragma unroll
-      for (int u = 0; u < 2; ++u)
-#pragma unroll
-        for (int x = 0; x < TM; ++x)
-#pragma unroll
-          for (int y = 0; y < TN; ++y) {
-            if constexpr (FX2) {
-              const unsigned long long ua = __double_as_longlong(fa[u][x]);
-              const unsigned long long ub = __double_as_longlong(fb[u][y]);
-              const float ah = __uint_as_float((unsigned)ua), al = __uint_as_float((unsigned)(ua >> 32));
-              const float bh = __uint_as_float((unsigned)ub), bl = __uint_as_float((unsigned)(ub >> 32));
-              facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, facc[x][y], 0, 0, 0);
-              facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, facc[x][y], 0, 0, 0);
-              facc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, facc[x][y], 0, 0, 0);
-            } else {
-              acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[u][x], fb[u][y], acc[x][y], 0, 0, 0);
-            }
-          }
-    }
+    f64d_chunk<MT, NT, WTM, WTN, FX2, KH, LAM, LBM>(As, Bs, wm, wn, lane, la, lb, kv, alpha, acc, facc);
     if constexpr (FX2) {
       if (++nfl == kFlush2 || g + 1 == g1) {
         nfl = 0;
@@ -473,22 +500,55 @@ int gemm_direct_tiles(int M, int N, bool fx2) {
   return ((M + kF64dShape[c][0] - 1) / kF64dShape[c][0]) * ((N + kF64dShape[c][1] - 1) / kF64dShape[c][1]);
 }
 
+// layout mode of one operand over the descriptor's segments: 0 all
+// K-contiguous, 1 all row-contiguous, 2 mixed
+static int layout_mode(const GemmDesc &d, bool b) {
+  int any0 = 0, any1 = 0;
+  for (int q = 0; q < d.nseg; ++q) {
+    if ((b ? d.seg[q].B.rfast : d.seg[q].A.rfast) != 0)
+      any1 = 1;
+    else
+      any0 = 1;
+  }
+  return any0 && any1 ? 2 : any1;
+}
+
+template <int MT, int NT, int WM, int WN, bool FX2, int NST, int LAM>
+static void launch_f64d_b(int lbm, dim3 g, hipStream_t s, const GemmDesc &d, const GemmBatch &bt,
+                          int n_fast) {
+  switch (lbm) {
+    case 0: hipLaunchKernelGGL((k_gemm_f64d<MT, NT, WM, WN, FX2, NST, LAM, 0>), g, dim3(512), 0, s, d, bt, n_fast); break;
+    case 1: hipLaunchKernelGGL((k_gemm_f64d<MT, NT, WM, WN, FX2, NST, LAM, 1>), g, dim3(512), 0, s, d, bt, n_fast); break;
+    default: hipLaunchKernelGGL((k_gemm_f64d<MT, NT, WM, WN, FX2, NST, LAM, 2>), g, dim3(512), 0, s, d, bt, n_fast); break;
+  }
+}
+
+template <int MT, int NT, int WM, int WN, bool FX2, int NST>
+static void launch_f64d_ab(dim3 g, hipStream_t s, const GemmDesc &d, const GemmBatch &bt, int n_fast) {
+  const int lbm = layout_mode(d, true);
+  switch (layout_mode(d, false)) {
+    case 0: launch_f64d_b<MT, NT, WM, WN, FX2, NST, 0>(lbm, g, s, d, bt, n_fast); break;
+    case 1: launch_f64d_b<MT, NT, WM, WN, FX2, NST, 1>(lbm, g, s, d, bt, n_fast); break;
+    default: launch_f64d_b<MT, NT, WM, WN, FX2, NST, 2>(lbm, g, s, d, bt, n_fast); break;
+  }
+}
+
 template <bool FX2>
 static void launch_f64d(const GemmDesc &d, const GemmBatch &bt, hipStream_t s) {
   const int c = gemm_direct_pick(d.M, d.N, FX2);
   const int MT = kF64dShape[c][0], NT = kF64dShape[c][1];
   const long long tiles = (long long)((d.M + MT - 1) / MT) * ((d.N + NT - 1) / NT) * d.splits * bt.nb;
   const int n_fast = d.M >= d.N ? 1 : 0;
-  const dim3 g((unsigned)tiles), b(512);
+  const dim3 g((unsigned)tiles);
   switch (c) {
-    case 0: hipLaunchKernelGGL((k_gemm_f64d<128, 64, 4, 2, FX2, kDStages>), g, b, 0, s, d, bt, n_fast); break;
-    case 1: hipLaunchKernelGGL((k_gemm_f64d<64, 128, 2, 4, FX2, kDStages>), g, b, 0, s, d, bt, n_fast); break;
+    case 0: launch_f64d_ab<128, 64, 4, 2, FX2, kDStages>(g, s, d, bt, n_fast); break;
+    case 1: launch_f64d_ab<64, 128, 2, 4, FX2, kDStages>(g, s, d, bt, n_fast); break;
     default:
       if constexpr (!FX2) {
         if (c == 2)
-          hipLaunchKernelGGL((k_gemm_f64d<128, 80, 8, 1, false, 2>), g, b, 0, s, d, bt, n_fast);
+          launch_f64d_ab<128, 80, 8, 1, false, 2>(g, s, d, bt, n_fast);
         else
-          hipLaunchKernelGGL((k_gemm_f64d<80, 128, 1, 8, false, 2>), g, b, 0, s, d, bt, n_fast);
+          launch_f64d_ab<80, 128, 1, 8, false, 2>(g, s, d, bt, n_fast);
       }
       break;
   }
