@@ -138,14 +138,38 @@ __device__ __forceinline__ void pplan(const OpDesc &d, int rc, int Rtot, int r0,
   }
 }
 
-// the loads of one stage (k = kappa .. kappa + 31, kp = divmod(kappa, kdiv))
+// K tracker of one operand: the next stage start's remainder r in the
+// operand's K divisor and its element offset kb = q kx + r ky, advanced by
+// adding 32 ky per stage and kx - kdiv ky per wrap of r (no multiplies per
+// stage)
+struct PTrack {
+  int r;
+  long long kb;
+};
+
+__device__ __forceinline__ void ptrack_init(PTrack &t, const OpDesc &d, int kappa) {
+  const int q = kappa / d.kdiv;
+  t.r = kappa - q * d.kdiv;
+  t.kb = (long long)q * d.kx + (long long)t.r * d.ky;
+}
+
+__device__ __forceinline__ void ptrack_next(PTrack &t, const OpDesc &d) {
+  t.r += kPK;
+  t.kb += d.ky << 5;
+  while (t.r >= d.kdiv) {
+    t.r -= d.kdiv;
+    t.kb += d.kx - (long long)d.kdiv * d.ky;
+  }
+}
+
+// the loads of one stage (k = kappa .. kappa + 31, r = kappa mod kdiv, kb =
+// its element offset)
 template <int R, int T>
 __device__ __forceinline__ void pissue(const OpDesc &d, int rc, int Rtot, int r0, int tid, int L,
-                                       int kappa, KPos kp, const PPlan<R, T> &pl, bf16_t *lds,
-                                       int wave) {
-  const bool full = kappa + kPK <= L && (d.kdiv >= (1 << 30) || kp.r + kPK <= d.kdiv);
+                                       int kappa, int r, long long kb, const PPlan<R, T> &pl,
+                                       bf16_t *lds, int wave) {
+  const bool full = kappa + kPK <= L && (d.kdiv >= (1 << 30) || r + kPK <= d.kdiv);
   if (full) {
-    const long long kb = (long long)kp.q * d.kx + (long long)kp.r * d.ky;  // wave-uniform
 #pragma unroll
     for (int q = 0; q < PPlan<R, T>::NI; ++q)
       __builtin_amdgcn_global_load_lds((const void *)(pl.ptr[q] + kb),
@@ -160,14 +184,6 @@ __device__ __forceinline__ void pissue(const OpDesc &d, int rc, int Rtot, int r0
     const bf16_t *src = p + plane * d.ps + prow(d, rc, Rtot, r0 + rl) + pkoff(d, rc, L, kappa + kl);
     __builtin_amdgcn_global_load_lds((const void *)src,
                                      (lds_ptr_t)(lds + (q * T + wave * 64) * 8), 16, 0, 0);
-  }
-}
-
-__device__ __forceinline__ void kadvance(KPos &kp, int kdiv) {
-  kp.r += kPK;
-  while (kp.r >= kdiv) {
-    kp.r -= kdiv;
-    ++kp.q;
   }
 }
 
@@ -254,9 +270,10 @@ __device__ __forceinline__ void b3p_tile(const D &d, const GemmBatch &bt, int n_
   int fs = 0;
   while (fs + 1 < d.nseg && g0 >= d.chunk_start[fs + 1]) ++fs;
   int fkappa = (g0 - d.chunk_start[fs]) * kPK;
-  KPos fka{fkappa / d.seg[fs].A.kdiv, fkappa % d.seg[fs].A.kdiv};
-  KPos fkb{fkappa / d.seg[fs].B.kdiv, fkappa % d.seg[fs].B.kdiv};
   OpDesc FA = d.seg[fs].A, FB = d.seg[fs].B;
+  PTrack ta, tb;
+  ptrack_init(ta, FA, fkappa);
+  ptrack_init(tb, FB, fkappa);
   FA.p = reinterpret_cast<const double *>(reinterpret_cast<const bf16_t *>(FA.p) + batch * bt.a[fs]);
   FB.p = reinterpret_cast<const double *>(reinterpret_cast<const bf16_t *>(FB.p) + batch * bt.bo[fs]);
   int FL = d.seg[fs].L, fend = d.chunk_start[fs + 1];
@@ -270,9 +287,9 @@ __device__ __forceinline__ void b3p_tile(const D &d, const GemmBatch &bt, int n_
   bool neg = d.seg[fs].alpha < 0.0;
 
   auto issue = [&](int g) {  // loads of stage g (the fetch tracker's stage)
-    bf16_t *st = S + ((g - g0) % kPStages) * SS;
-    pissue<MT, T>(FA, FA.rfast, d.M, m0, tid, FL, fkappa, fka, pa, st, wave);
-    pissue<NT, T>(FB, FB.rfast, d.N, n0, tid, FL, fkappa, fkb, pb, st + SA, wave);
+    bf16_t *st = S + ((g - g0) & 1) * SS;
+    pissue<MT, T>(FA, FA.rfast, d.M, m0, tid, FL, fkappa, ta.r, ta.kb, pa, st, wave);
+    pissue<NT, T>(FB, FB.rfast, d.N, n0, tid, FL, fkappa, tb.r, tb.kb, pb, st + SA, wave);
     if (g + 1 < g1) {
       if (g + 1 >= fend) {
         while (g + 1 >= d.chunk_start[fs + 1]) ++fs;  // (skips segments of length 0)
@@ -285,14 +302,14 @@ __device__ __forceinline__ void b3p_tile(const D &d, const GemmBatch &bt, int n_
         FL = d.seg[fs].L;
         fend = d.chunk_start[fs + 1];
         fkappa = 0;
-        fka = KPos{0, 0};
-        fkb = KPos{0, 0};
+        ta = PTrack{0, 0};
+        tb = PTrack{0, 0};
         pplan<MT, T>(FA, FA.rfast, d.M, m0, tid, pa);
         pplan<NT, T>(FB, FB.rfast, d.N, n0, tid, pb);
       } else {
         fkappa += kPK;
-        kadvance(fka, FA.kdiv);
-        kadvance(fkb, FB.kdiv);
+        ptrack_next(ta, FA);
+        ptrack_next(tb, FB);
       }
     }
   };
@@ -321,7 +338,7 @@ __device__ __forceinline__ void b3p_tile(const D &d, const GemmBatch &bt, int n_
     const int kv = cL - ckappa;  // valid k in this stage (>= 32: all)
     ckappa += kPK;
     if (!live) continue;
-    const bf16_t *As = S + ((g - g0) % kPStages) * SS;
+    const bf16_t *As = S + ((g - g0) & 1) * SS;
     const bf16_t *Bs = As + SA;
     // k past the segment is zeroed on the A side (B holds clamped, finite
     // values); alpha = -1 flips the sign bits of the A pieces

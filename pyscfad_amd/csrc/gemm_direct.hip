@@ -25,7 +25,10 @@
 // The kernel is compiled per launch for the operand layouts its segments use
 // (each operand all K-contiguous, all row-contiguous, or mixed with a runtime
 // choice), so the fragment reads and MFMAs of a chunk form one straight-line
-// block: on W-build-shaped descriptors 42-43 -> 46-47 TFLOP/s.
+// block, and the fetch tracker keeps each operand's chunk offset by additions
+// (no 64-bit multiplies per chunk): every wave runs that scalar work in the
+// same phase after the barrier, while the MFMA pipes wait.  On W-build-shaped
+// descriptors 42-43 -> 52 TFLOP/s.
 //
 // A pair of elements must be contiguous and 16-byte aligned, so an operand is
 // eligible when its fast stride is 1, every other stride and index divisor is
@@ -135,16 +138,16 @@ __device__ __forceinline__ int dkoff(const OpDesc &d, int L, int kappa, KPos kp,
   return (int)(q * d.kx + r * d.ky);
 }
 
-// The loads of one chunk.  Full chunks of a `fast` operand use the stored
-// pointers; otherwise (divisor boundary inside a chunk, or the last chunk of a
-// segment whose length is not a multiple of 16) each load's row and k are
+// The loads of one chunk.  Full chunks of a `fast` operand (kappa <= flim)
+// use the stored pointers plus kb, the chunk start's K offset (kept by the
+// tracker); otherwise (divisor boundary inside a chunk, or the last chunk of
+// a segment whose length is not a multiple of 16) each load's row and k are
 // recomputed and k is clamped to the segment.
 template <int R>
 __device__ __forceinline__ void dissue(const OpDesc &d, int Rtot, int r0, int tid, int L,
-                                       int kappa, KPos kp, const DPlan<R> &pl, double *lds,
-                                       int wave) {
-  if (pl.fast && kappa + kDK <= L) {
-    const long long kb = (long long)kp.q * d.kx + (long long)kp.r * d.ky;  // wave-uniform
+                                       int kappa, KPos kp, long long kb, int flim,
+                                       const DPlan<R> &pl, double *lds, int wave) {
+  if (kappa <= flim) {
 #pragma unroll
     for (int q = 0; q < DPlan<R>::NI; ++q)
       __builtin_amdgcn_global_load_lds((const void *)(pl.ptr[q] + kb),
@@ -161,11 +164,30 @@ __device__ __forceinline__ void dissue(const OpDesc &d, int Rtot, int r0, int ti
   }
 }
 
-__device__ __forceinline__ void kadvance(KPos &kp, int kdiv) {
-  kp.r += kDK;
-  while (kp.r >= kdiv) {
-    kp.r -= kdiv;
-    ++kp.q;
+// K tracker of one operand: position (q, r) of the next chunk start in the
+// operand's K divisor and its element offset q kx + r ky, advanced by adding
+// 16 ky per chunk and kx - kdiv ky per wrap of r (no multiplies per chunk)
+struct DTrack {
+  KPos kp;
+  long long off;
+  int flim;  // last chunk start that the stored pointers cover (-1: none)
+};
+
+template <int R>
+__device__ __forceinline__ void dtrack_init(DTrack &t, const OpDesc &d, int kappa, int L,
+                                            const DPlan<R> &pl) {
+  t.kp = KPos{kappa / d.kdiv, kappa % d.kdiv};
+  t.off = (long long)t.kp.q * d.kx + (long long)t.kp.r * d.ky;
+  t.flim = pl.fast ? L - kDK : -1;
+}
+
+__device__ __forceinline__ void dtrack_next(DTrack &t, const OpDesc &d) {
+  t.kp.r += kDK;
+  t.off += d.ky << 4;
+  while (t.kp.r >= d.kdiv) {
+    t.kp.r -= d.kdiv;
+    ++t.kp.q;
+    t.off += d.kx - (long long)d.kdiv * d.ky;
   }
 }
 
@@ -317,8 +339,6 @@ __global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, GemmBatch bt, 
   int fs = 0;
   while (fs + 1 < d.nseg && g0 >= d.chunk_start[fs + 1]) ++fs;
   int fkappa = (g0 - d.chunk_start[fs]) * kDK;
-  KPos fka{fkappa / d.seg[fs].A.kdiv, fkappa % d.seg[fs].A.kdiv};
-  KPos fkb{fkappa / d.seg[fs].B.kdiv, fkappa % d.seg[fs].B.kdiv};
   // fetch-side segment fields (operand pointers moved to this product of the
   // batch), cached while the segment lasts
   OpDesc FA = d.seg[fs].A, FB = d.seg[fs].B;
@@ -329,15 +349,20 @@ __global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, GemmBatch bt, 
   DPlan<NT> pb;
   dplan<MT>(FA, d.M, m0, tid, pa);
   dplan<NT>(FB, d.N, n0, tid, pb);
+  DTrack ta, tb;
+  dtrack_init<MT>(ta, FA, fkappa, FL, pa);
+  dtrack_init<NT>(tb, FB, fkappa, FL, pb);
+  int fst = 0, cst = 0;  // LDS stages of the next chunk to load and to compute
   // compute tracker (segment fields cached while the segment lasts)
   int cs = fs, ckappa = fkappa, cend = d.chunk_start[fs + 1];
   int la = d.seg[fs].A.rfast, lb = d.seg[fs].B.rfast, cL = d.seg[fs].L;
   double alpha = d.seg[fs].alpha;
 
   auto issue = [&](int g) {  // loads of chunk g (the fetch tracker's chunk) into its stage
-    double *st = S + ((g - g0) % NST) * SS;
-    dissue<MT>(FA, d.M, m0, tid, FL, fkappa, fka, pa, st, wave);
-    dissue<NT>(FB, d.N, n0, tid, FL, fkappa, fkb, pb, st + SA, wave);
+    double *st = S + fst * SS;
+    fst = fst + 1 == NST ? 0 : fst + 1;
+    dissue<MT>(FA, d.M, m0, tid, FL, fkappa, ta.kp, ta.off, ta.flim, pa, st, wave);
+    dissue<NT>(FB, d.N, n0, tid, FL, fkappa, tb.kp, tb.off, tb.flim, pb, st + SA, wave);
     // advance the tracker to chunk g + 1
     if (g + 1 < g1) {
       if (g + 1 >= fend) {
@@ -349,14 +374,14 @@ __global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, GemmBatch bt, 
         FL = d.seg[fs].L;
         fend = d.chunk_start[fs + 1];
         fkappa = 0;
-        fka = KPos{0, 0};
-        fkb = KPos{0, 0};
         dplan<MT>(FA, d.M, m0, tid, pa);
         dplan<NT>(FB, d.N, n0, tid, pb);
+        dtrack_init<MT>(ta, FA, 0, FL, pa);
+        dtrack_init<NT>(tb, FB, 0, FL, pb);
       } else {
         fkappa += kDK;
-        kadvance(fka, FA.kdiv);
-        kadvance(fkb, FB.kdiv);
+        dtrack_next(ta, FA);
+        dtrack_next(tb, FB);
       }
     }
   };
@@ -388,8 +413,9 @@ __global__ __launch_bounds__(512, 4) void k_gemm_f64d(GemmDesc d, GemmBatch bt, 
     }
     const int kv = cL - ckappa;  // valid k in this chunk (>= 16: all)
     ckappa += kDK;
+    const double *As = S + cst * SS;
+    cst = cst + 1 == NST ? 0 : cst + 1;
     if (!live) continue;
-    const double *As = S + ((g - g0) % NST) * SS;
     const double *Bs = As + SA;
     f64d_chunk<MT, NT, WTM, WTN, FX2, KH, LAM, LBM>(As, Bs, wm, wn, lane, la, lb, kv, alpha, acc, facc);
     if constexpr (FX2) {
