@@ -34,8 +34,8 @@
 // multiple of 8, the base 16-byte aligned; K-contiguous operands need
 // segment lengths that are multiples of 8, row-contiguous ones a row count
 // that is.  Rows past M or N and k past L are clamped to valid elements
-// (finite inputs); k past L is zeroed in the A fragments, rows past M or N
-// are never stored.
+// (finite inputs); k past L is zeroed in the fragments of one side, rows past
+// M or N are never stored.
 #include <cstdlib>
 #include <cstring>
 #include "ccsdt_common.h"
@@ -340,8 +340,10 @@ __device__ __forceinline__ void b3p_tile(const D &d, const GemmBatch &bt, int n_
     if (!live) continue;
     const bf16_t *As = S + ((g - g0) & 1) * SS;
     const bf16_t *Bs = As + SA;
-    // k past the segment is zeroed on the A side (B holds clamped, finite
-    // values); alpha = -1 flips the sign bits of the A pieces
+    // k past the segment is zeroed and alpha = -1 flips the sign bits on the
+    // side whose fragments stay in registers (both sides hold finite values
+    // there: clamped elements), and only on stages that need it
+    const bool fix = neg || kv < kPK;
     u32x4 keep = {~0u, ~0u, ~0u, ~0u};
     if (kv < kPK) {
 #pragma unroll
@@ -360,9 +362,14 @@ __device__ __forceinline__ void b3p_tile(const D &d, const GemmBatch &bt, int n_
 #pragma unroll
       for (int t = 0; t < TM; ++t) {
         frags<MT>(As, la, wm * WTM + 16 * t, lane, ah[t], am[t], al[t]);
-        ah[t] = (ah[t] ^ sgn) & keep;
-        am[t] = (am[t] ^ sgn) & keep;
-        al[t] = (al[t] ^ sgn) & keep;
+      }
+      if (fix) {
+#pragma unroll
+        for (int t = 0; t < TM; ++t) {
+          ah[t] = (ah[t] ^ sgn) & keep;
+          am[t] = (am[t] ^ sgn) & keep;
+          al[t] = (al[t] ^ sgn) & keep;
+        }
       }
 #pragma unroll
       for (int b0 = 0; b0 < TN; b0 += 2) {
@@ -391,18 +398,21 @@ __device__ __forceinline__ void b3p_tile(const D &d, const GemmBatch &bt, int n_
       u32x4 bh[TN], bm[TN], bl[TN];
 #pragma unroll
       for (int t = 0; t < TN; ++t) frags<NT>(Bs, lb, wn * WTN + 16 * t, lane, bh[t], bm[t], bl[t]);
+      if (fix) {
+#pragma unroll
+        for (int t = 0; t < TN; ++t) {
+          bh[t] = (bh[t] ^ sgn) & keep;
+          bm[t] = (bm[t] ^ sgn) & keep;
+          bl[t] = (bl[t] ^ sgn) & keep;
+        }
+      }
 #pragma unroll
       for (int a0 = 0; a0 < TM; a0 += 2) {
         constexpr int W2 = 2;
         u32x4 ah[W2], am[W2], al[W2];
 #pragma unroll
         for (int u = 0; u < W2; ++u)
-          if (a0 + u < TM) {
-            frags<MT>(As, la, wm * WTM + 16 * (a0 + u), lane, ah[u], am[u], al[u]);
-            ah[u] = (ah[u] ^ sgn) & keep;
-            am[u] = (am[u] ^ sgn) & keep;
-            al[u] = (al[u] ^ sgn) & keep;
-          }
+          if (a0 + u < TM) frags<MT>(As, la, wm * WTM + 16 * (a0 + u), lane, ah[u], am[u], al[u]);
 #pragma unroll
         for (int b = 0; b < TN; ++b) {
 #define B3P_PRODUCT(AX, BX)                                                                   \
