@@ -138,44 +138,56 @@ __device__ __forceinline__ void pplan(const OpDesc &d, int rc, int Rtot, int r0,
   }
 }
 
-// K tracker of one operand: the next stage start's remainder r in the
-// operand's K divisor and its element offset kb = q kx + r ky, advanced by
-// adding 32 ky per stage and kx - kdiv ky per wrap of r (no multiplies per
-// stage)
-struct PTrack {
-  int r;
-  long long kb;
+// Fetch-side state of one operand for the current segment: its K strides and
+// divisor (all that full stages need), whether it is row-contiguous, the next
+// stage start's remainder r in the K divisor and its element offset
+// kb = q kx + r ky, advanced by adding 32 ky per stage and kx - kdiv ky per
+// wrap of r (no multiplies per stage).  The full descriptor is read from the
+// kernel argument only for partial stages and at segment switches.
+struct POp {
+  long long kx, ky, kb;
+  int kdiv, rc, r;
 };
 
-__device__ __forceinline__ void ptrack_init(PTrack &t, const OpDesc &d, int kappa) {
+__device__ __forceinline__ void pop_init(POp &f, const OpDesc &d, int kappa) {
+  f.kx = d.kx;
+  f.ky = d.ky;
+  f.kdiv = d.kdiv;
+  f.rc = d.rfast;
   const int q = kappa / d.kdiv;
-  t.r = kappa - q * d.kdiv;
-  t.kb = (long long)q * d.kx + (long long)t.r * d.ky;
+  f.r = kappa - q * d.kdiv;
+  f.kb = (long long)q * d.kx + (long long)f.r * d.ky;
 }
 
-__device__ __forceinline__ void ptrack_next(PTrack &t, const OpDesc &d) {
-  t.r += kPK;
-  t.kb += d.ky << 5;
-  while (t.r >= d.kdiv) {
-    t.r -= d.kdiv;
-    t.kb += d.kx - (long long)d.kdiv * d.ky;
+__device__ __forceinline__ void pop_next(POp &f) {
+  f.r += kPK;
+  f.kb += f.ky << 5;
+  while (f.r >= f.kdiv) {
+    f.r -= f.kdiv;
+    f.kb += f.kx - (long long)f.kdiv * f.ky;
   }
 }
 
-// the loads of one stage (k = kappa .. kappa + 31, r = kappa mod kdiv, kb =
-// its element offset)
+// stage k = kappa .. kappa + 31 lies in the segment and in one quotient of the
+// K divisor: the plan's pointers plus kb address it
+__device__ __forceinline__ bool pop_full(const POp &f, int kappa, int L) {
+  return kappa + kPK <= L && (f.kdiv >= (1 << 30) || f.r + kPK <= f.kdiv);
+}
+
 template <int R, int T>
-__device__ __forceinline__ void pissue(const OpDesc &d, int rc, int Rtot, int r0, int tid, int L,
-                                       int kappa, int r, long long kb, const PPlan<R, T> &pl,
-                                       bf16_t *lds, int wave) {
-  const bool full = kappa + kPK <= L && (d.kdiv >= (1 << 30) || r + kPK <= d.kdiv);
-  if (full) {
+__device__ __forceinline__ void pissue_full(const PPlan<R, T> &pl, long long kb, bf16_t *lds,
+                                            int wave) {
 #pragma unroll
-    for (int q = 0; q < PPlan<R, T>::NI; ++q)
-      __builtin_amdgcn_global_load_lds((const void *)(pl.ptr[q] + kb),
-                                       (lds_ptr_t)(lds + (q * T + wave * 64) * 8), 16, 0, 0);
-    return;
-  }
+  for (int q = 0; q < PPlan<R, T>::NI; ++q)
+    __builtin_amdgcn_global_load_lds((const void *)(pl.ptr[q] + kb),
+                                     (lds_ptr_t)(lds + (q * T + wave * 64) * 8), 16, 0, 0);
+}
+
+// the loads of a partial stage: each load's row and k recomputed, k clamped
+template <int R, int T>
+__device__ __forceinline__ void pissue_part(const OpDesc &d, int Rtot, int r0, int tid, int L,
+                                            int kappa, bf16_t *lds, int wave) {
+  const int rc = d.rfast;
   const bf16_t *p = reinterpret_cast<const bf16_t *>(d.p);
 #pragma unroll
   for (int q = 0; q < PPlan<R, T>::NI; ++q) {
@@ -270,17 +282,25 @@ __device__ __forceinline__ void b3p_tile(const D &d, const GemmBatch &bt, int n_
   int fs = 0;
   while (fs + 1 < d.nseg && g0 >= d.chunk_start[fs + 1]) ++fs;
   int fkappa = (g0 - d.chunk_start[fs]) * kPK;
-  OpDesc FA = d.seg[fs].A, FB = d.seg[fs].B;
-  PTrack ta, tb;
-  ptrack_init(ta, FA, fkappa);
-  ptrack_init(tb, FB, fkappa);
-  FA.p = reinterpret_cast<const double *>(reinterpret_cast<const bf16_t *>(FA.p) + batch * bt.a[fs]);
-  FB.p = reinterpret_cast<const double *>(reinterpret_cast<const bf16_t *>(FB.p) + batch * bt.bo[fs]);
-  int FL = d.seg[fs].L, fend = d.chunk_start[fs + 1];
+  // operand b (0: A, 1: B) of the fetch segment, moved to this product of the batch
+  auto opdesc = [&](int b) {
+    OpDesc o = b ? d.seg[fs].B : d.seg[fs].A;
+    o.p = reinterpret_cast<const double *>(reinterpret_cast<const bf16_t *>(o.p) +
+                                           batch * (b ? bt.bo[fs] : bt.a[fs]));
+    return o;
+  };
+  POp fa, fb;
   PPlan<MT, T> pa;
   PPlan<NT, T> pb;
-  pplan<MT, T>(FA, FA.rfast, d.M, m0, tid, pa);
-  pplan<NT, T>(FB, FB.rfast, d.N, n0, tid, pb);
+  auto fetch_segment = [&](int kappa) {
+    const OpDesc A = opdesc(0), B = opdesc(1);
+    pop_init(fa, A, kappa);
+    pop_init(fb, B, kappa);
+    pplan<MT, T>(A, A.rfast, d.M, m0, tid, pa);
+    pplan<NT, T>(B, B.rfast, d.N, n0, tid, pb);
+  };
+  fetch_segment(fkappa);
+  int FL = d.seg[fs].L, fend = d.chunk_start[fs + 1];
   // compute tracker
   int cs = fs, ckappa = fkappa, cend = d.chunk_start[fs + 1];
   int la = d.seg[fs].A.rfast, lb = d.seg[fs].B.rfast, cL = d.seg[fs].L;
@@ -288,28 +308,25 @@ __device__ __forceinline__ void b3p_tile(const D &d, const GemmBatch &bt, int n_
 
   auto issue = [&](int g) {  // loads of stage g (the fetch tracker's stage)
     bf16_t *st = S + ((g - g0) & 1) * SS;
-    pissue<MT, T>(FA, FA.rfast, d.M, m0, tid, FL, fkappa, ta.r, ta.kb, pa, st, wave);
-    pissue<NT, T>(FB, FB.rfast, d.N, n0, tid, FL, fkappa, tb.r, tb.kb, pb, st + SA, wave);
+    if (pop_full(fa, fkappa, FL))
+      pissue_full<MT, T>(pa, fa.kb, st, wave);
+    else
+      pissue_part<MT, T>(opdesc(0), d.M, m0, tid, FL, fkappa, st, wave);
+    if (pop_full(fb, fkappa, FL))
+      pissue_full<NT, T>(pb, fb.kb, st + SA, wave);
+    else
+      pissue_part<NT, T>(opdesc(1), d.N, n0, tid, FL, fkappa, st + SA, wave);
     if (g + 1 < g1) {
       if (g + 1 >= fend) {
         while (g + 1 >= d.chunk_start[fs + 1]) ++fs;  // (skips segments of length 0)
-        FA = d.seg[fs].A;
-        FB = d.seg[fs].B;
-        FA.p = reinterpret_cast<const double *>(reinterpret_cast<const bf16_t *>(FA.p) +
-                                                batch * bt.a[fs]);
-        FB.p = reinterpret_cast<const double *>(reinterpret_cast<const bf16_t *>(FB.p) +
-                                                batch * bt.bo[fs]);
         FL = d.seg[fs].L;
         fend = d.chunk_start[fs + 1];
         fkappa = 0;
-        ta = PTrack{0, 0};
-        tb = PTrack{0, 0};
-        pplan<MT, T>(FA, FA.rfast, d.M, m0, tid, pa);
-        pplan<NT, T>(FB, FB.rfast, d.N, n0, tid, pb);
+        fetch_segment(0);
       } else {
         fkappa += kPK;
-        ptrack_next(ta, FA);
-        ptrack_next(tb, FB);
+        pop_next(fa);
+        pop_next(fb);
       }
     }
   };
