@@ -15,8 +15,9 @@
 //     of k-row k stored at chunk p ^ (((k >> 2) & 3) << 3), so the four lane
 //     groups of a fragment read (k = 4g + s) sit 128 bytes apart.
 //
-// Three LDS stages: while chunk g is multiplied, the loads of chunks g+1 and
-// g+2 are in flight.  Each iteration waits for its own chunk's loads
+// Three LDS stages for the 64-wide tiles (two for the 80-wide ones): while
+// chunk g is multiplied, the loads of chunks g+1 and g+2 are in flight.  Each
+// iteration waits for its own chunk's loads
 // (s_waitcnt vmcnt), then one s_barrier (without the memory fence that
 // __syncthreads adds, which would also wait for the chunks still in flight)
 // makes every wave's loads visible and retires the slot read one iteration
