@@ -7,7 +7,7 @@ pair energies + 8 orbital statistics); a 4-member BF16 ensemble of
 instances (600 epochs, the experiment's defaults); the 248 test instances
 screened with the 1e-5 Eh spread gate and the exact kernel as the fallback.
 
-The FP64 labels take about eight minutes (2048 pair-resolved (T) passes of
+The FP64 labels take about six minutes (2048 pair-resolved (T) passes of
 7.1e12 flops each, 1.4e16 in all: 3 minutes even at the FP64 MFMA peak), so
 the test runs only when PYSCFAD_AMD_SCREEN_FULL=1 is set; the default GPU
 suite covers the same code on small instances (test_gpu_screen.py).
@@ -29,7 +29,7 @@ import torch
 pytestmark = [
     pytest.mark.gpu,
     pytest.mark.skipif(os.environ.get('PYSCFAD_AMD_SCREEN_FULL') != '1',
-                       reason='2048 FP64 labels take about 8 minutes; set PYSCFAD_AMD_SCREEN_FULL=1'),
+                       reason='2048 FP64 labels take about 6 minutes; set PYSCFAD_AMD_SCREEN_FULL=1'),
 ]
 
 
