@@ -13,7 +13,8 @@
 //     distinct 16-byte slots of every 256-byte bank window;
 //   row-contiguous operand ("rfast"): [k][r], R doubles per k, the R/2 chunks
 //     of k-row k stored at chunk p ^ (((k >> 2) & 3) << 3), so the four lane
-//     groups of a fragment read (k = 4g + s) sit 128 bytes apart.
+//     groups of a fragment read (k = 4g + s) sit 128 bytes apart (R = 80:
+//     128-byte row blocks placed across k-rows, rfast_at).
 //
 // Three LDS stages for the 64-wide tiles (two for the 80-wide ones): while
 // chunk g is multiplied, the loads of chunks g+1 and g+2 are in flight.  Each
@@ -71,6 +72,37 @@ __device__ __forceinline__ int rfast_swz(int k) {
   return (R % 64 == 0) ? (((k >> 2) & 3) << 3) : (((k >> 2) & 3) << 1);
 }
 
+// LDS offset (doubles) of row r, k-row k in a row-contiguous stage of R rows,
+// and its inverse for the loads (16-byte chunk id -> k, row pair p).  R = 80:
+// a k-row is 640 bytes, so k = 4g + s for the four lane groups g of a fragment
+// read falls on one bank window whatever the XOR within the row; instead the
+// 128-byte blocks (16 rows) of lane groups 0 and 2 take the even block slots
+// and those of groups 1 and 3 the odd ones (block slot 2 (5 kr + r / 16) +
+// g % 2, kr = s + 4 (g / 2)), so the two groups of each half wave read
+// different halves of every 256-byte bank window.  Other R: the XOR above.
+template <int R>
+__device__ __forceinline__ int rfast_at(int k, int r) {
+  if constexpr (R == 80) {
+    const int kr = (k & 3) + 4 * (k >> 3);
+    const int bslot = 2 * (5 * kr + (r >> 4)) + ((k >> 2) & 1);
+    return (bslot * 8 + ((r >> 1) & 7)) * 2 + (r & 1);
+  } else {
+    return k * R + 2 * ((r >> 1) ^ rfast_swz<R>(k)) + (r & 1);
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void rfast_chunk(int id, int &k, int &p) {
+  if constexpr (R == 80) {
+    const int bs = id >> 3, idx = bs >> 1, kr = idx / 5;
+    k = (kr & 3) + 8 * (kr >> 2) + 4 * (bs & 1);
+    p = 8 * (idx - 5 * kr) + (id & 7);
+  } else {
+    k = id / (R / 2);
+    p = (id % (R / 2)) ^ rfast_swz<R>(k);
+  }
+}
+
 // 16-byte chunks of an R-row x 16-k tile, and loads per thread of 512 (the
 // last load of a thread past the tile's 8 R chunks fills padding)
 template <int R>
@@ -100,8 +132,8 @@ __device__ __forceinline__ void dslot(const OpDesc &d, int Rtot, int r0, int tid
   int id = q * 512 + tid;
   if (id >= DTile<R>::CH) id = 0;  // padding: a valid chunk, never read
   if (d.rfast) {
-    const int k = id / (R / 2), pc = id % (R / 2);
-    const int p = pc ^ rfast_swz<R>(k);
+    int k, p;
+    rfast_chunk<R>(id, k, p);
     r = r0 + 2 * p;
     if (r > Rtot - 2) r = Rtot - 2;  // Rtot is even for a row-contiguous operand
     kl = k;
@@ -217,8 +249,7 @@ __device__ __forceinline__ void f64d_chunk(const double *As, const double *Bs, i
       if (ra) {
 #pragma unroll
         for (int u = 0; u < SPH; ++u) {
-          const int k = g4 + SPH * h + u;
-          fa[u][t] = As[k * MT + 2 * ((r >> 1) ^ rfast_swz<MT>(k)) + (r & 1)];
+          fa[u][t] = As[rfast_at<MT>(g4 + SPH * h + u, r)];
         }
       } else {
 #pragma unroll
@@ -235,8 +266,7 @@ __device__ __forceinline__ void f64d_chunk(const double *As, const double *Bs, i
       if (rb) {
 #pragma unroll
         for (int u = 0; u < SPH; ++u) {
-          const int k = g4 + SPH * h + u;
-          fb[u][t] = Bs[k * NT + 2 * ((r >> 1) ^ rfast_swz<NT>(k)) + (r & 1)];
+          fb[u][t] = Bs[rfast_at<NT>(g4 + SPH * h + u, r)];
         }
       } else {
 #pragma unroll

@@ -157,12 +157,30 @@ def _rfast_swz(R, k):
     return ((k >> 2) & 3) << 3 if R % 64 == 0 else ((k >> 2) & 3) << 1
 
 
+def _rfast80_at(k, r):
+    """LDS offset (doubles) of row r, k-row k in an 80-row row-contiguous stage:
+    the 16-row blocks of k-rows with (k >> 2) even take the even 128-byte block
+    slots, the others the odd ones (gemm_direct.hip rfast_at)."""
+    kr = (k & 3) + 4 * (k >> 3)
+    bslot = 2 * (5 * kr + (r >> 4)) + ((k >> 2) & 1)
+    return (bslot * 8 + ((r >> 1) & 7)) * 2 + (r & 1)
+
+
+def _rfast80_chunk(idx):
+    """Inverse of _rfast80_at for the loads: chunk id -> (k, row pair p)."""
+    bs = idx >> 3
+    kr = (bs >> 1) // 5
+    b = (bs >> 1) - 5 * kr
+    return (kr & 3) + 8 * (kr >> 2) + 4 * (bs & 1), 8 * b + (idx & 7)
+
+
 def _direct_stage_map(R, rfast):
     """LDS slot (in doubles) of each tile element (r, k), r < R, k < 16, as the
     direct-load plan of gemm_direct.hip places it: thread t, load q writes 16
     bytes at byte 16 (512 q + t); kfast slot = 16 r + 2 (c ^ f(r)) + k % 2,
-    rfast slot = R k + 2 (p ^ sw(k)) + r % 2.  Loads past the tile's 8 R chunks
-    (R = 80: 1024 slots for 640 chunks) fill padding and hold no element."""
+    rfast slot = R k + 2 (p ^ sw(k)) + r % 2 (R = 80: _rfast80_at).  Loads past
+    the tile's 8 R chunks (R = 80: 1024 slots for 640 chunks) fill padding and
+    hold no element."""
     slot = {}
     nch = 8 * R
     for q in range(-(-nch // 512)):
@@ -170,7 +188,11 @@ def _direct_stage_map(R, rfast):
             idx = q * 512 + t
             if idx >= nch:
                 continue
-            if rfast:
+            if rfast and R == 80:
+                k, p = _rfast80_chunk(idx)
+                for e in range(2):
+                    slot[(2 * p + e, k)] = 2 * idx + e
+            elif rfast:
                 k, pc = divmod(idx, R // 2)
                 p = pc ^ _rfast_swz(R, k)
                 assert p < R // 2
@@ -195,7 +217,9 @@ def test_direct_engine_lds_layout_round_trip(R, rfast):
     for r in range(R):
         for k in range(16):
             g4, s = (k // 4) * 4, k % 4
-            if rfast:
+            if rfast and R == 80:
+                read = _rfast80_at(k, r)
+            elif rfast:
                 c = (r >> 1) ^ _rfast_swz(R, k)
                 read = k * R + 2 * c + (r & 1)
             else:
@@ -203,3 +227,19 @@ def test_direct_engine_lds_layout_round_trip(R, rfast):
                 chunk = ((g4 >> 1) + (s >> 1)) ^ f
                 read = r * 16 + 2 * chunk + (s & 1)
             assert read == slot[(r, k)], (r, k)
+
+
+def test_direct_engine_rfast80_groups_in_distinct_half_windows():
+    """80-row row-contiguous FP64 stage: the lane groups 0/1 (and 2/3) of every
+    fragment read (k = 4 g + s, rows r16 .. r16 + 15) read 128 contiguous bytes
+    each, in different 128-byte halves of the 256-byte bank window, so one half
+    wave's 8-byte reads meet no bank twice."""
+    for r16 in range(0, 80, 16):
+        for s in range(4):
+            for g in (0, 2):
+                for gg in (g, g + 1):
+                    offs = sorted(_rfast80_at(4 * gg + s, r16 + i) * 8 for i in range(16))
+                    assert offs == list(range(offs[0], offs[0] + 128, 8))
+                h0 = (_rfast80_at(4 * g + s, r16) * 8 // 128) % 2
+                h1 = (_rfast80_at(4 * (g + 1) + s, r16) * 8 // 128) % 2
+                assert h0 != h1, (r16, s, g)
