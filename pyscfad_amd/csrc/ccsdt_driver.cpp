@@ -285,8 +285,8 @@ static Layout make_layout(int o, int v, int vjp, int prec, bool pipe = false) {
 // e of an array becomes bf16 element e of its hi plane, with the array's
 // element count as the plane stride.
 struct PlanarMap {
-  const char *base[7];
-  long long n[7];
+  const char *base[8];
+  long long n[8];
   int count;
 };
 
@@ -502,7 +502,13 @@ static int adjoint_vr(const Ctx &c, int i, int j, int k, hipStream_t st) {
   return PAMD_OK;
 }
 
-static int adjoint_gemms(const Ctx &c, int i, int j, int k) {
+// pair: 0 all three ovvv adjoints of the triple; 1 only the one into Vpb[k]
+// (those into Vpb[i] and Vpb[j] wait for the next triple (i, j, k'), which
+// shares them); 2 those into Vpb[i] and Vpb[j] as four-segment launches that
+// also take the terms of the previous triple (i, j, kp) from its own Wbar
+// buffer (prev->gW), plus the one into Vpb[k]
+static int adjoint_gemms(const Ctx &c, int i, int j, int k, int pair = 0, const Ctx *prev = nullptr,
+                         int kp = 0) {
   const long long o = c.o, v = c.v;
   const long long v2 = v * v, v3 = v2 * v, vo = v * o;
   const int V = c.v, O = c.o;
@@ -518,7 +524,30 @@ static int adjoint_gemms(const Ctx &c, int i, int j, int k) {
   // T2mb[k] write disjoint slabs unless two occupied indices coincide
   const bool distinct = i != j && j != k;
   // ovvv_bar, rows = virtual pairs, cols = f
-  {
+  if (pair != 0) {
+    const double *Wq = prev ? prev->gW : nullptr;
+    GemmDesc ds[3];
+    int n = 0;
+    if (pair == 2) {
+      ds[n++] = gemm(V * V, V, kAccumulate,
+                     {seg(opR(Wq, v2, v, V, 1, 0), op1(T2(kp, j), 1, v, 1), V, 1.0),
+                      seg(opR(Wq, v2, 1, V, v, 1), op1(T2(j, kp), 1, v, 1), V, 1.0),
+                      seg(opR(Wb, v2, v, V, 1, 0), op1(T2(k, j), 1, v, 1), V, 1.0),
+                      seg(opR(Wb, v2, 1, V, v, 1), op1(T2(j, k), 1, v, 1), V, 1.0)},
+                     out(Vpb(i), v2, v, V, 0, 1, kBig));
+      ds[n++] = gemm(V * V, V, kAccumulate,
+                     {seg(opR(Wq, v, v2, V, 1, 0), op1(T2(kp, i), 1, v, 1), V, 1.0),
+                      seg(opR(Wq, v, 1, V, v2, 1), op1(T2(i, kp), 1, v, 1), V, 1.0),
+                      seg(opR(Wb, v, v2, V, 1, 0), op1(T2(k, i), 1, v, 1), V, 1.0),
+                      seg(opR(Wb, v, 1, V, v2, 1), op1(T2(i, k), 1, v, 1), V, 1.0)},
+                     out(Vpb(j), v2, v, V, 0, 1, kBig));
+    }
+    ds[n++] = gemm(V * V, V, kAccumulate,
+                   {seg(opR(Wb, v2, 1, V, v, 1), op1(T2(j, i), 1, v, 1), V, 1.0),
+                    seg(opR(Wb, v, 1, V, v2, 1), op1(T2(i, j), 1, v, 1), V, 1.0)},
+                   out(Vpb(k), v, v2, V, 0, 1, kBig));
+    for (int q = 0; q < n; ++q) PAMD_CHECK(run_gemm(c, ds[q], kProfAdjOvvv));
+  } else {
     const GemmDesc ds[3] = {
         // Vpb[i][a][b|c][f]: P1 (K=c), P2 (K=b)
         gemm(V * V, V, kAccumulate,
@@ -876,6 +905,32 @@ static int run_vjp_pipelined(Ctx &c, long long norb, PipeRes &pr) {
   cx[1].part_h = c.part_h2;
   for (int a = 0; a < cx[1].planes.count; ++a)
     if (cx[1].planes.base[a] == (const char *)c.gW) cx[1].planes.base[a] = (const char *)c.gW2;
+  // Paired ovvv adjoints (env PYSCFAD_AMD_PAIR_OVVV=0 turns them off): consecutive triples
+  // (i, j, k), (i, j, k') add into the same Vpb[i] and Vpb[j], so the second
+  // one's launches for them also take the first one's terms, from the other
+  // Wbar buffer.  That buffer is next written by the orbit epilogue two
+  // triples on, which the side stream runs after the gather of the second
+  // triple, which waits for its contractions.  Only where Wbar has its own
+  // buffer (split forms; FP64 writes Wbar over W, which the W build of the
+  // triple after next overwrites before the second triple's contractions).
+  // Measured (full walks, split-BF16x3): o16/v160 E+VJP 0.541 -> 0.532 s,
+  // o24/v240 5.39 -> 5.30 s, o40/v400 127.8 -> 125.2 s.  Only ovvv_bar
+  // changes (the same terms grouped in other FP32 sums, ~1e-8 relative).
+  static const bool pair_env = [] {
+    const char *e = getenv("PYSCFAD_AMD_PAIR_OVVV");
+    return !(e && e[0] == '0');
+  }();
+  const bool pair_on = pair_env && c.gW != c.W && c.gW2 != nullptr;
+  if (pair_on && c.prec == PAMD_PREC_BF16X3) {
+    const long long v3 = (long long)c.v * c.v * c.v;
+    for (int p = 0; p < 2; ++p) {
+      PlanarMap &pm = cx[p].planes;
+      if (pm.count >= 8) return PAMD_ERR_ARG;
+      pm.base[pm.count] = (const char *)cx[1 - p].gW;
+      pm.n[pm.count] = v3;
+      ++pm.count;
+    }
+  }
   const hipStream_t G = c.s, E = pr.e;
   hipEvent_t *wb = pr.ev + 2, *vr = pr.ev + 4, *adj = pr.ev + 6, *cmb = pr.ev + 8;
   PAMD_CHECK(hipEventRecord(pr.ev[0], G));
@@ -891,6 +946,8 @@ static int run_vjp_pipelined(Ctx &c, long long norb, PipeRes &pr) {
       k = tr[t] >> 20;
     };
     int i, j, k, st;
+    bool pend = false;  // the previous triple left its Vpb[i], Vpb[j] adjoints to this one
+    int kpend = 0;
     g_prof_on = g_prof_every > 0;
     ijk(0, i, j, k);
     st = build_w(cx[0], i, j, k);
@@ -926,7 +983,16 @@ static int run_vjp_pipelined(Ctx &c, long long norb, PipeRes &pr) {
       // G: adjoint contractions of t
       PAMD_CHECK(hipStreamWaitEvent(G, vr[p], 0));
       if (t >= 2) PAMD_CHECK(hipStreamWaitEvent(G, cmb[p], 0));
-      st = adjoint_gemms(C, i, j, k);
+      if (pend) {
+        st = adjoint_gemms(C, i, j, k, 2, &cx[1 - p], kpend);
+        pend = false;
+      } else if (pair_on && t + 1 < n && (tr[t + 1] & 0xfffff) == (tr[t] & 0xfffff)) {
+        st = adjoint_gemms(C, i, j, k, 1);
+        pend = true;
+        kpend = k;
+      } else {
+        st = adjoint_gemms(C, i, j, k);
+      }
       if (st != PAMD_OK) return st;
       PAMD_CHECK(hipEventRecord(adj[p], G));
       // E: gather of t

@@ -66,7 +66,11 @@ def test_vjp_shards_add_up(precision, cuda):
             gs[k] += g_s[k]
     assert abs(es - float(e.item())) <= 1e-13 * abs(float(e.item()))
     for k, ref in g.items():
-        assert _rel(gs[k], ref) <= 1e-12, k
+        # the split forms pair the ovvv adjoints of consecutive triples of a
+        # walk, and the shards' walks pair other triples than the whole walk:
+        # the same terms in other FP32 sums
+        tol = 1e-6 if precision != 'fp64' and k == 'ovvv_bar' else 1e-12
+        assert _rel(gs[k], ref) <= tol, (k, _rel(gs[k], ref))
 
 
 def _sin_direction(n, dev):

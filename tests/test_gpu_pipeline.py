@@ -1,7 +1,11 @@
 """The adjoint walk with two triples in flight (pamd_ccsdt_energy_vjp) against
 the walk with one triple in flight (pamd_ccsdt_energy_vjp_noalloc, which never
-pipelines): the energies and every cotangent block are bitwise equal, since
-every accumulator shared by the triples is written by one stream in walk order."""
+pipelines): every accumulator shared by the triples is written by one stream in
+walk order, so the energies and cotangent blocks are bitwise equal, except that
+in the split forms the two-stream walk also pairs the ovvv adjoints of
+consecutive triples (i, j, k), (i, j, k') into one launch each for Vpb[i] and
+Vpb[j]: ovvv_bar then holds the same terms in other FP32 sums (within 1e-6
+relative, about 1e-8 measured); FP64 never pairs."""
 import ctypes
 
 import pytest
@@ -16,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize('precision', ['fp64', 'bf16x3'])
 @pytest.mark.parametrize('o,v,seed', [(8, 40, 5), (5, 24, 6)])
-def test_pipelined_walk_is_bitwise_the_one_stream_walk(o, v, seed, precision, cuda):
+def test_pipelined_walk_matches_the_one_stream_walk(o, v, seed, precision, cuda):
     print(f'pipeline vs one stream: {precision} o{o} v{v}', flush=True)
     p = make_problem(o, v, 4 * v, seed=seed, fvo_scale=1e-3)
     inp = _ops.Inputs(p.mo_energy, p.t1, p.t2, p.ovvv_packed(), p.ovoo, p.ovov, p.fvo,
@@ -32,4 +36,8 @@ def test_pipelined_walk_is_bitwise_the_one_stream_walk(o, v, seed, precision, cu
     assert st == _lib.PAMD_OK
     assert torch.equal(e, e_ref.reshape(e.shape))
     for k, ref in g_ref.items():
-        assert torch.equal(out[k], ref), k
+        if precision != 'fp64' and k == 'ovvv_bar':
+            rel = float((out[k] - ref).norm() / ref.norm())
+            assert rel <= 1e-6, (k, rel)
+        else:
+            assert torch.equal(out[k], ref), k

@@ -1,8 +1,10 @@
 """pamd_ccsdt_energy_vjp_noalloc (library-owned scratch, no device allocation)
 against pamd_ccsdt_energy_vjp with a caller workspace, through the C ABI: two
 calls in a row, the second on a smaller problem, each within 1e-10 Eh in E(T)
-and 1e-9 relative per cotangent block; and a problem larger than the scratch is
-rejected with PAMD_ERR_WORKSPACE."""
+and 1e-9 relative per cotangent block (ovvv_bar in the split form: 1e-6, since
+the workspace entry point's two-stream walk pairs the ovvv adjoints of
+consecutive triples into other FP32 sums); and a problem larger than the
+scratch is rejected with PAMD_ERR_WORKSPACE."""
 import ctypes
 
 import numpy as np
@@ -43,7 +45,8 @@ def test_noalloc_matches_workspace_entry_point(precision, cuda):
         assert st == _lib.PAMD_OK
         assert abs(float(e.item()) - float(e_ref.item())) <= 1e-10
         for k, ref in g_ref.items():
-            assert _rel(g[k], ref) <= 1e-9, (k, _rel(g[k], ref))
+            tol = 1e-6 if precision != 'fp64' and k == 'ovvv_bar' else 1e-9
+            assert _rel(g[k], ref) <= tol, (k, _rel(g[k], ref))
 
 
 def test_noalloc_rejects_problems_beyond_the_scratch(cuda):
