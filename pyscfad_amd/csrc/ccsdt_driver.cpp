@@ -502,11 +502,12 @@ static int adjoint_vr(const Ctx &c, int i, int j, int k, hipStream_t st) {
   return PAMD_OK;
 }
 
-// pair: 0 all three ovvv adjoints of the triple; 1 only the one into Vpb[k]
-// (those into Vpb[i] and Vpb[j] wait for the next triple (i, j, k'), which
-// shares them); 2 those into Vpb[i] and Vpb[j] as four-segment launches that
-// also take the terms of the previous triple (i, j, kp) from its own Wbar
-// buffer (prev->gW), plus the one into Vpb[k]
+// pair: 0 all three ovvv adjoints (and hole-term t2 adjoints) of the triple;
+// 1 only those into Vpb[k] and T2mb[k] (those into index i and j wait for the
+// next triple (i, j, k'), which shares them); 2 those into Vpb[i], Vpb[j],
+// T2mb[i], T2mb[j] as four-segment launches that also take the terms of the
+// previous triple (i, j, kp) from its own Wbar buffer (prev->gW), plus the
+// ones into Vpb[k] and T2mb[k]
 static int adjoint_gemms(const Ctx &c, int i, int j, int k, int pair = 0, const Ctx *prev = nullptr,
                          int kp = 0) {
   const long long o = c.o, v = c.v;
@@ -619,7 +620,30 @@ static int adjoint_gemms(const Ctx &c, int i, int j, int k, int pair = 0, const 
     PAMD_CHECK(run_gemm_group(c, ds, 6, kProfAdjOvoo, true));
   }
   // hole-term t2 reads: T2mb[x][p][q][m] (= dE/d t2[x,m,p,q])
-  {
+  if (pair != 0) {
+    const double *Wq = prev ? prev->gW : nullptr;
+    GemmDesc ds[3];
+    int n = 0;
+    if (pair == 2) {
+      ds[n++] = gemm(V * V, O, kAccumulate,
+                     {seg(opR(Wq, v2, 1, V, v, 1), op1(OOt(j, kp), 1, o, 1), V, -1.0),
+                      seg(opR(Wq, v2, v, V, 1, 0), op1(OOt(kp, j), 1, o, 1), V, -1.0),
+                      seg(opR(Wb, v2, 1, V, v, 1), op1(OOt(j, k), 1, o, 1), V, -1.0),
+                      seg(opR(Wb, v2, v, V, 1, 0), op1(OOt(k, j), 1, o, 1), V, -1.0)},
+                     out(T2mb(i), vo, o, V, 0, 1, kBig));
+      ds[n++] = gemm(V * V, O, kAccumulate,
+                     {seg(opR(Wq, v, 1, V, v2, 1), op1(OOt(i, kp), 1, o, 1), V, -1.0),
+                      seg(opR(Wq, v, v2, V, 1, 0), op1(OOt(kp, i), 1, o, 1), V, -1.0),
+                      seg(opR(Wb, v, 1, V, v2, 1), op1(OOt(i, k), 1, o, 1), V, -1.0),
+                      seg(opR(Wb, v, v2, V, 1, 0), op1(OOt(k, i), 1, o, 1), V, -1.0)},
+                     out(T2mb(j), vo, o, V, 0, 1, kBig));
+    }
+    ds[n++] = gemm(V * V, O, kAccumulate,
+                   {seg(opR(Wb, v, 1, V, v2, 1), op1(OOt(i, j), 1, o, 1), V, -1.0),
+                    seg(opR(Wb, v2, 1, V, v, 1), op1(OOt(j, i), 1, o, 1), V, -1.0)},
+                   out(T2mb(k), o, vo, V, 0, 1, kBig));
+    for (int q = 0; q < n; ++q) PAMD_CHECK(run_gemm(c, ds[q], kProfAdjT2H));
+  } else {
     const GemmDesc ds[3] = {
         // T2mb[i]: H4 (p=a,q=c,K=b), H6 (p=a,q=b,K=c)
         gemm(V * V, O, kAccumulate,
@@ -913,9 +937,11 @@ static int run_vjp_pipelined(Ctx &c, long long norb, PipeRes &pr) {
   // triple, which waits for its contractions.  Only where Wbar has its own
   // buffer (split forms; FP64 writes Wbar over W, which the W build of the
   // triple after next overwrites before the second triple's contractions).
-  // Measured (full walks, split-BF16x3): o16/v160 E+VJP 0.541 -> 0.532 s,
-  // o24/v240 5.39 -> 5.30 s, o40/v400 127.8 -> 125.2 s.  Only ovvv_bar
-  // changes (the same terms grouped in other FP32 sums, ~1e-8 relative).
+  // The hole-term t2 adjoints into T2mb[i], T2mb[j] pair the same way.
+  // Measured (full walks, split-BF16x3, ovvv adjoints paired): o16/v160 E+VJP
+  // 0.541 -> 0.532 s, o24/v240 5.39 -> 5.30 s, o40/v400 127.8 -> 125.2 s.
+  // Only ovvv_bar and t2_bar change (the same terms grouped in other FP32
+  // sums, ~1e-8 relative).
   static const bool pair_env = [] {
     const char *e = getenv("PYSCFAD_AMD_PAIR_OVVV");
     return !(e && e[0] == '0');

@@ -2,10 +2,11 @@
 the walk with one triple in flight (pamd_ccsdt_energy_vjp_noalloc, which never
 pipelines): every accumulator shared by the triples is written by one stream in
 walk order, so the energies and cotangent blocks are bitwise equal, except that
-in the split forms the two-stream walk also pairs the ovvv adjoints of
-consecutive triples (i, j, k), (i, j, k') into one launch each for Vpb[i] and
-Vpb[j]: ovvv_bar then holds the same terms in other FP32 sums (within 1e-6
-relative, about 1e-8 measured); FP64 never pairs."""
+in the split forms the two-stream walk also pairs the ovvv and hole-term t2
+adjoints of consecutive triples (i, j, k), (i, j, k') into one launch each for
+Vpb[i], Vpb[j], T2mb[i], T2mb[j]: ovvv_bar and t2_bar then hold the same terms
+in other FP32 sums (within 1e-6 relative, about 1e-8 measured); FP64 never
+pairs."""
 import ctypes
 
 import pytest
@@ -36,7 +37,7 @@ def test_pipelined_walk_matches_the_one_stream_walk(o, v, seed, precision, cuda)
     assert st == _lib.PAMD_OK
     assert torch.equal(e, e_ref.reshape(e.shape))
     for k, ref in g_ref.items():
-        if precision != 'fp64' and k == 'ovvv_bar':
+        if precision != 'fp64' and k in ('ovvv_bar', 't2_bar'):
             rel = float((out[k] - ref).norm() / ref.norm())
             assert rel <= 1e-6, (k, rel)
         else:
