@@ -19,6 +19,17 @@ int pamd_debug_gemm_descriptors(int nocc, int nvir, int i, int j, int k, int wit
                                 const unsigned long long *bases, void *out, int max_desc,
                                 int *n_desc, int *desc_bytes);
 
+/* Record the adjoint contractions of two consecutive triples (i, j, kp) and
+ * (i, j, k), kp < k, as the two-stream adjoint walk pairs them in the split
+ * forms: the first triple's launches without its ovvv and hole-term t2
+ * adjoints into index i and j, then the second triple's, those adjoints as
+ * four-segment launches that also take the first triple's terms.
+ * bases[11]: those of pamd_debug_gemm_descriptors, then the second triple's
+ * Wbar (the first triple's Wbar is at bases[3]). */
+int pamd_debug_paired_adjoint_descriptors(int nocc, int nvir, int i, int j, int kp, int k,
+                                          const unsigned long long *bases, void *out,
+                                          int max_desc, int *n_desc, int *desc_bytes);
+
 /* 1 when a recorded descriptor (one of pamd_debug_gemm_descriptors' outputs)
  * runs on the FP64 engine's direct-to-LDS path (16-byte element pairs: fast
  * strides 1, other strides and index divisors even, 16-byte aligned bases),

@@ -30,7 +30,8 @@ EXPORTS = ('pamd_version', 'pamd_ccsdt_workspace_size', 'pamd_ccsdt_energy',
            'pamd_ccsdt_projected_workspace_size', 'pamd_ccsdt_projected_energy',
            'pamd_ccsdt_projected_energy_vjp', 'pamd_profile_enable',
            'pamd_profile_reset', 'pamd_profile_progress', 'pamd_profile_num_kinds', 'pamd_profile_kind_name',
-           'pamd_profile_read', 'pamd_debug_gemm_descriptors', 'pamd_debug_split_factors',
+           'pamd_profile_read', 'pamd_debug_gemm_descriptors',
+           'pamd_debug_paired_adjoint_descriptors', 'pamd_debug_split_factors',
            'pamd_debug_split_factors_prec', 'pamd_debug_gemm_direct_ok', 'pamd_debug_gemm_b3p_ok',
            'pamd_bf16_gemm_tn', 'pamd_bf16_act_fwd', 'pamd_bf16_act_bwd', 'pamd_adam_step',
            'pamd_mse', 'pamd_f32_to_bf16', 'pamd_mp2_init_amps', 'pamd_df_eris_workspace_size',

@@ -1653,6 +1653,49 @@ int pamd_debug_gemm_descriptors(int nocc, int nvir, int i, int j, int k, int wit
   return PAMD_OK;
 }
 
+int pamd_debug_paired_adjoint_descriptors(int nocc, int nvir, int i, int j, int kp, int k,
+                                          const unsigned long long *bases, void *out,
+                                          int max_desc, int *n_desc, int *desc_bytes) {
+  if (nocc < 1 || nvir < 1 || !bases || !n_desc || !desc_bytes) return PAMD_ERR_ARG;
+  if (i < 0 || j < i || kp < j || k <= kp || k >= nocc) return PAMD_ERR_ARG;
+  Layout L = make_layout(nocc, nvir, 1, PAMD_PREC_FP64);
+  Ctx c1;
+  memset(&c1, 0, sizeof(c1));
+  c1.o = nocc;
+  c1.v = nvir;
+  c1.vjp = 1;
+  c1.Vp = (double *)bases[0];
+  c1.T2m = (double *)bases[1];
+  c1.OOt = (double *)bases[2];
+  c1.W = (double *)bases[3];
+  c1.gW = c1.W;
+  c1.t2 = (const double *)bases[4];
+  c1.gT2 = c1.t2;
+  c1.Vb = (double *)bases[5];
+  c1.Vpb = (double *)bases[6];
+  c1.T2mb = (double *)bases[7];
+  c1.part_p = (double *)bases[8];
+  c1.part_h = (double *)bases[9];
+  c1.S_p = L.S_p;
+  c1.S_h = L.S_h;
+  Ctx c2 = c1;
+  c2.W = (double *)bases[10];
+  c2.gW = c2.W;
+  std::vector<GemmDesc> rec;
+  g_record = &rec;
+  int st = adjoint_gemms(c1, i, j, kp, 1);
+  if (st == PAMD_OK) st = adjoint_gemms(c2, i, j, k, 2, &c1, kp);
+  g_record = nullptr;
+  if (st != PAMD_OK) return st;
+  *desc_bytes = (int)sizeof(GemmDesc);
+  *n_desc = (int)rec.size();
+  if (out) {
+    const int n = std::min((int)rec.size(), max_desc);
+    memcpy(out, rec.data(), sizeof(GemmDesc) * (size_t)n);
+  }
+  return PAMD_OK;
+}
+
 int pamd_profile_enable(int every) {
   if (every < 0) return PAMD_ERR_ARG;
   g_prof_every = every;

@@ -74,6 +74,24 @@ def record(o, v, i, j, k, with_vjp):
     return [buf[q] for q in range(n.value)]
 
 
+def record_paired(o, v, i, j, kp, k):
+    """Adjoint descriptors of the triples (i, j, kp), (i, j, k) as the two-stream
+    walk pairs them (pamd_debug_paired_adjoint_descriptors); the second
+    triple's Wbar is the array 'W2' (base address after the ten of NAMES)."""
+    lib = _lib.load()
+    bases = (ctypes.c_ulonglong * 11)(*[(n + 1) << 40 for n in range(11)])
+    f = lib.pamd_debug_paired_adjoint_descriptors
+    f.argtypes = [ctypes.c_int] * 6 + [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    n = ctypes.c_int()
+    nb = ctypes.c_int()
+    buf = (GemmDesc * 64)()
+    st = f(o, v, i, j, kp, k, bases, buf, 64, ctypes.byref(n), ctypes.byref(nb))
+    assert st == 0, st
+    assert nb.value == ctypes.sizeof(GemmDesc)
+    return [buf[q] for q in range(n.value)]
+
+
 def _decomp(idx, div, sx, sy):
     q = idx // div
     return q * sx + (idx - q * div) * sy

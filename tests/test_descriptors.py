@@ -11,7 +11,7 @@ import torch
 
 from pyscfad_amd.gen import make_problem
 from oracle import ccsd_t_oracle as O
-from tests.gemm_emulator import Memory, record, split_factors
+from tests.gemm_emulator import NAMES, Memory, record, record_paired, sizes, split_factors
 
 CASES = [(3, 12, 0, 1, 2), (3, 12, 0, 0, 2), (3, 12, 1, 2, 2), (4, 17, 0, 2, 3), (2, 20, 0, 1, 1),
          (5, 9, 1, 1, 3)]
@@ -243,3 +243,36 @@ def test_direct_engine_rfast80_groups_in_distinct_half_windows():
                 h0 = (_rfast80_at(4 * g + s, r16) * 8 // 128) % 2
                 h1 = (_rfast80_at(4 * (g + 1) + s, r16) * 8 // 128) % 2
                 assert h0 != h1, (r16, s, g)
+
+
+@pytest.mark.parametrize('o,v,i,j,kp,k', [(5, 12, 0, 1, 2, 3), (5, 12, 1, 1, 2, 4), (4, 10, 0, 2, 2, 3),
+                                          (4, 9, 0, 0, 1, 3)])
+def test_paired_adjoint_descriptors_match_the_unpaired_ones(o, v, i, j, kp, k):
+    """The two-stream walk's paired launches for consecutive triples (i, j, kp),
+    (i, j, k) (the second triple's ovvv and hole-term t2 adjoints into index i
+    and j as four-segment launches over both triples' Wbar) leave Vpb and T2mb
+    as the two triples' unpaired launches do, with every address in bounds and
+    no write race inside a launch."""
+    p = make_problem(o, v, 3 * v, seed=o * 31 + v + k)
+    rng = np.random.default_rng(o + v + kp)
+    w1, w2 = rng.normal(size=v ** 3), rng.normal(size=v ** 3)
+    sz = sizes(o, v)
+    sz['W2'] = v ** 3
+    mem = Memory(names=NAMES + ('W2',), array_sizes=sz)
+    _fill(mem, p)
+    mem.arrays['W'][:] = w1
+    mem.arrays['W2'][:] = w2
+    for d in record_paired(o, v, i, j, kp, k):
+        mem.run(d)
+    ref = Memory(o, v)
+    _fill(ref, p)
+    ref.arrays['W'][:] = w1
+    for d in record(o, v, i, j, kp, with_vjp=True)[3:]:
+        ref.run(d)
+    ref.arrays['W'][:] = w2
+    for d in record(o, v, i, j, k, with_vjp=True)[3:]:
+        ref.run(d)
+    for name in ('Vpb', 'T2mb'):
+        a, b = mem.arrays[name], ref.arrays[name]
+        assert np.abs(b).max() > 0
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max(), name
